@@ -1,0 +1,85 @@
+"""ctypes binding of libdfm_hip.so (the C ABI declared in include/dfm_hip.h).
+
+The library is the only compute path: if it is missing this module raises instead of
+falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdfm_hip.so")
+
+DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, DFM_ERR_NOMEM = range(7)
+FP64, FP32C = 0, 1
+OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF = range(6)
+
+PRECISIONS = {"fp64": FP64, "fp32c": FP32C}
+
+
+class DfmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__("dfm error %d: %s" % (code, message))
+        self.code = code
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load libdfm_hip.so, raising if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libdfm_hip.so is not built (%s); run `python -m deformetrica_code_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    i, l, d, sz = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
+    L.dfm_abi_version.restype = i
+    L.dfm_last_error.restype = ctypes.c_char_p
+    L.dfm_workspace_size.argtypes = [i, i, i, l, l, i]
+    L.dfm_workspace_size.restype = sz
+    L.dfm_profile_enable.argtypes = [i]
+    L.dfm_profile_enable.restype = i
+    L.dfm_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+    L.dfm_profile_read.restype = i
+    # device entry points
+    L.dfm_conv.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
+    L.dfm_grad1.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P, sz]
+    L.dfm_conv_grad1.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P, _P, sz]
+    L.dfm_varlin.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
+    L.dfm_graddim.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, i, _P, _P, sz]
+    L.dfm_graddiff.argtypes = [_P, i, i, d, i, _P, _P, _P, l, _P, _P, sz]
+    # host-pointer wrappers
+    L.dfm_gauss_conv.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
+    L.dfm_gauss_grad1conv.argtypes = [i, i, d, _D, _D, _D, _D, _D, i, i, i]
+    L.dfm_gauss_gradconv.argtypes = [i, i, d, _D, _D, _D, ctypes.c_uint, _D, i, i, i]
+    L.dfm_gauss_gradconv_varlin.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
+    L.dfm_gauss_graddiffconv.argtypes = [i, i, d, _D, _D, _D, _D, i, i]
+    L.dfm_gauss_conv_grad1.argtypes = [i, i, d, _D, _D, _D, _D, _D, _D, i, i, i]
+    for name in ("dfm_conv", "dfm_grad1", "dfm_conv_grad1", "dfm_varlin", "dfm_graddim",
+                 "dfm_graddiff", "dfm_gauss_conv", "dfm_gauss_grad1conv", "dfm_gauss_gradconv",
+                 "dfm_gauss_gradconv_varlin", "dfm_gauss_graddiffconv", "dfm_gauss_conv_grad1"):
+        getattr(L, name).restype = i
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != DFM_OK:
+        msg = load().dfm_last_error() or b""
+        raise DfmError(rc, msg.decode(errors="replace"))
+
+
+def precision_code(precision) -> int:
+    if isinstance(precision, int):
+        return precision
+    try:
+        return PRECISIONS[precision.lower()]
+    except KeyError:
+        raise ValueError("precision must be one of %s" % sorted(PRECISIONS)) from None

@@ -1,0 +1,370 @@
+// capi.hip — the extern "C" boundary of the kernel-sum engine (declared in dfm_hip.h).
+//
+// Device entry points validate their arguments the way the reference plugin does
+// (CUDAExactKernel.cxx: row-count mismatch, unsupported dimvect, dim out of range) and
+// return a status code; host-pointer wrappers reproduce the reference's
+// GaussGpu*Conv1D host functions (GpuConv1D.cu:88-133 and siblings): allocate, copy in,
+// launch, synchronise, copy out, free — with every HIP error checked.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "dfm_hip.h"
+#include "dfm_internal.h"
+
+using namespace dfm;
+
+namespace {
+
+constexpr long kMaxCount = 0x7fffffffL;
+
+int check_dims(int D, int DV) {
+  if (D != 2 && D != 3) return set_error(DFM_ERR_DIM, "unsupported dimpoint %d (2 or 3)", D);
+  if (DV < 1 || DV > 6) return set_error(DFM_ERR_DIM, "unsupported dimvect %d (1..6)", DV);
+  return DFM_OK;
+}
+
+int check_common(int D, int DV, int prec, double sigma, long nx, long ny) {
+  int rc = check_dims(D, DV);
+  if (rc) return rc;
+  if (prec != DFM_FP64 && prec != DFM_FP32C)
+    return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
+  if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
+  if (nx < 0 || ny < 0 || nx > kMaxCount || ny > kMaxCount)
+    return set_error(DFM_ERR_ARG, "point counts must be in [0, 2^31)");
+  return DFM_OK;
+}
+
+// AbstractKernel::SetKernelWidth clamps the width at 1e-10 (AbstractKernel.h:85-91).
+double clamp_width(double h) { return h < 1e-10 ? 1e-10 : h; }
+
+int need(const void *ptr, const char *name) {
+  if (!ptr) return set_error(DFM_ERR_ARG, "%s is NULL", name);
+  return DFM_OK;
+}
+
+// Common body of every device entry point.
+int run(void *stream, int op, int D, int DV, double sigma, int prec, KParams p, long out0_n,
+        long out1_n, void *work, size_t work_bytes) {
+  clear_error();
+  hipStream_t st = (hipStream_t)stream;
+  if (p.nx == 0) return DFM_OK;
+  if (p.ny == 0) {  // empty sums
+    fill(st, p.out0, out0_n, 0.0);
+    fill(st, p.out1, out1_n, 0.0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DFM_OK
+                           : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
+  }
+  Plan pl = make_plan(op, prec, D, DV, p.nx, p.ny);
+  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
+                     work ? work_bytes : (size_t)0);
+  p.sigma = clamp_width(sigma);
+  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
+}
+
+KParams blank() {
+  KParams p;
+  std::memset(&p, 0, sizeof(p));
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfm_abi_version(void) { return 1; }
+
+const char *dfm_last_error(void) { return last_error(); }
+
+int dfm_profile_enable(int on) {
+  profile_enable(on != 0);
+  return DFM_OK;
+}
+
+int dfm_profile_read(double *total_ms, long *launches) {
+  clear_error();
+  return profile_read(total_ms, launches);
+}
+
+size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision) {
+  if (check_common(dimpoint, dimvect, precision, 1.0, nx, ny) != DFM_OK) return 0;
+  if (op < DFM_OP_CONV || op > DFM_OP_GRADDIFF) return 0;
+  if (op == DFM_OP_GRADDIFF) ny = nx;
+  if (nx == 0 || ny == 0) return 0;
+  return make_plan(op, precision, dimpoint, dimvect, nx, ny).work_bytes;
+}
+
+int dfm_conv(void *stream, int D, int DV, double sigma, int prec, const double *x, long nx,
+             const double *y, const double *beta, long ny, double *gamma, void *work,
+             size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && nx > 0) rc = need(x, "x") ? DFM_ERR_ARG : need(gamma, "gamma");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny;
+  return run(stream, OP_CONV, D, DV, sigma, prec, p, nx * DV, 0, work, work_bytes);
+}
+
+int dfm_grad1(void *stream, int D, int DV, double sigma, int prec, const double *x,
+              const double *alpha, long nx, const double *y, const double *beta, long ny,
+              double *gamma, void *work, size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && nx > 0)
+    rc = (need(x, "x") || need(alpha, "alpha")) ? DFM_ERR_ARG : need(gamma, "gamma");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.ta = alpha; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny;
+  return run(stream, OP_GRAD1, D, DV, sigma, prec, p, nx * D, 0, work, work_bytes);
+}
+
+int dfm_conv_grad1(void *stream, int D, int DV, double sigma, int prec, const double *x,
+                   const double *alpha, long nx, const double *y, const double *beta, long ny,
+                   double *conv, double *grad, void *work, size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && nx > 0) rc = need(x, "x");
+  if (!rc && nx > 0 && grad) rc = need(alpha, "alpha");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  if (!conv && !grad) return DFM_OK;
+  KParams p = blank();
+  p.x = x; p.ta = alpha; p.y = y; p.w = beta; p.out0 = conv; p.out1 = grad; p.nx = nx; p.ny = ny;
+  if (!grad) return run(stream, OP_CONV, D, DV, sigma, prec, p, nx * DV, 0, work, work_bytes);
+  if (!conv) {
+    p.out0 = grad;
+    p.out1 = nullptr;
+    return run(stream, OP_GRAD1, D, DV, sigma, prec, p, nx * D, 0, work, work_bytes);
+  }
+  return run(stream, OP_CONVGRAD1, D, DV, sigma, prec, p, nx * DV, nx * D, work, work_bytes);
+}
+
+int dfm_varlin(void *stream, int D, int DV, double sigma, int prec, const double *x, long nx,
+               const double *y, const double *beta, long ny, double *gamma, void *work,
+               size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && nx > 0) rc = need(x, "x") ? DFM_ERR_ARG : need(gamma, "gamma");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny;
+  return run(stream, OP_VARLIN, D, DV, sigma, prec, p, nx * DV * D, 0, work, work_bytes);
+}
+
+int dfm_graddim(void *stream, int D, int DV, double sigma, int prec, const double *x, long nx,
+                const double *y, const double *beta, long ny, int dim, double *gamma, void *work,
+                size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && (dim < 0 || dim >= D))
+    rc = set_error(DFM_ERR_INDEX, "dimension index out of bounds");
+  if (!rc && nx > 0) rc = need(x, "x") ? DFM_ERR_ARG : need(gamma, "gamma");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny; p.dim = dim;
+  return run(stream, OP_GRADDIM, D, DV, sigma, prec, p, nx * DV, 0, work, work_bytes);
+}
+
+int dfm_graddiff(void *stream, int D, int DV, double sigma, int prec, const double *x,
+                 const double *beta, const double *eta, long n, double *gamma, void *work,
+                 size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, n, n);
+  if (!rc && DV != D)
+    rc = set_error(DFM_ERR_DIM, "Invalid number of columns of beta (needs dimvect == dimpoint)");
+  if (!rc && n > 0)
+    rc = (need(x, "x") || need(beta, "beta") || need(eta, "eta")) ? DFM_ERR_ARG
+                                                                  : need(gamma, "gamma");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.ta = beta; p.tb = eta; p.y = x; p.w = beta; p.se = eta; p.out0 = gamma;
+  p.nx = n; p.ny = n;
+  return run(stream, OP_GRADDIFF, D, DV, sigma, prec, p, n * D, 0, work, work_bytes);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// host-pointer wrappers
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int dev_alloc(DevBuf &b, size_t bytes) {
+  if (bytes == 0) return DFM_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return set_error(DFM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return DFM_OK;
+}
+
+int upload(DevBuf &b, const double *h, size_t n) {
+  if (n == 0) return DFM_OK;
+  int rc = dev_alloc(b, n * sizeof(double));
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice,
+                                hipStreamPerThread);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "H2D: %s", hipGetErrorString(e));
+}
+
+int download(double *h, const DevBuf &b, size_t n) {
+  if (n == 0) return DFM_OK;
+  hipError_t e = hipMemcpyAsync(h, b.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                hipStreamPerThread);
+  if (e == hipSuccess) e = hipStreamSynchronize(hipStreamPerThread);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "D2H: %s", hipGetErrorString(e));
+}
+
+int finish() {
+  hipError_t e = hipStreamSynchronize(hipStreamPerThread);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "sync: %s", hipGetErrorString(e));
+}
+
+#define TRY(expr)               \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != DFM_OK) return rc_; \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int dfm_gauss_conv(int D, int DV, double sigma, const double *x, const double *y,
+                   const double *beta, double *gamma, int nx, int ny, int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (nx == 0) return DFM_OK;
+  if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, dy, db, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  TRY(dev_alloc(dg, (size_t)nx * DV * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_CONV, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_conv(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
+               (double *)db.p, ny, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * DV));
+  return finish();
+}
+
+int dfm_gauss_grad1conv(int D, int DV, double sigma, const double *alpha, const double *x,
+                        const double *y, const double *beta, double *gamma, int nx, int ny,
+                        int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (nx == 0) return DFM_OK;
+  if (!x || !alpha || !gamma || (ny > 0 && (!y || !beta)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, da, dy, db, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(da, alpha, (size_t)nx * DV));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_GRAD1, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_grad1(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)da.p, nx,
+                (double *)dy.p, (double *)db.p, ny, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * D));
+  return finish();
+}
+
+int dfm_gauss_gradconv(int D, int DV, double sigma, const double *x, const double *y,
+                       const double *beta, unsigned dim, double *gamma, int nx, int ny,
+                       int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (dim >= (unsigned)D) return set_error(DFM_ERR_INDEX, "dimension index out of bounds");
+  if (nx == 0) return DFM_OK;
+  if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, dy, db, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  TRY(dev_alloc(dg, (size_t)nx * DV * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_GRADDIM, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_graddim(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
+                  (double *)db.p, ny, (int)dim, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * DV));
+  return finish();
+}
+
+int dfm_gauss_gradconv_varlin(int D, int DV, double sigma, const double *x, const double *y,
+                              const double *beta, double *gamma, int nx, int ny, int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (nx == 0) return DFM_OK;
+  if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, dy, db, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  TRY(dev_alloc(dg, (size_t)nx * DV * D * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_VARLIN, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_varlin(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
+                 (double *)db.p, ny, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * DV * D));
+  return finish();
+}
+
+int dfm_gauss_graddiffconv(int D, int DV, double sigma, const double *x, const double *beta,
+                           const double *eta, double *gamma, int nx, int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, nx));
+  if (DV != D) return set_error(DFM_ERR_DIM, "Invalid number of columns of beta");
+  if (nx == 0) return DFM_OK;
+  if (!x || !beta || !eta || !gamma) return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, db, de, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(db, beta, (size_t)nx * DV));
+  TRY(upload(de, eta, (size_t)nx * D));
+  TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_GRADDIFF, D, DV, nx, nx, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_graddiff(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)db.p,
+                   (double *)de.p, nx, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * D));
+  return finish();
+}
+
+int dfm_gauss_conv_grad1(int D, int DV, double sigma, const double *x, const double *alpha,
+                         const double *y, const double *beta, double *conv, double *grad, int nx,
+                         int ny, int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (nx == 0 || (!conv && !grad)) return DFM_OK;
+  if (!x || (grad && !alpha) || (ny > 0 && (!y || !beta)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, da, dy, db, dc, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  if (grad) TRY(upload(da, alpha, (size_t)nx * DV));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  if (conv) TRY(dev_alloc(dc, (size_t)nx * DV * sizeof(double)));
+  if (grad) TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  const int op = conv && grad ? DFM_OP_CONV_GRAD1 : conv ? DFM_OP_CONV : DFM_OP_GRAD1;
+  size_t ws = dfm_workspace_size(op, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_conv_grad1(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)da.p, nx,
+                     (double *)dy.p, (double *)db.p, ny, (double *)dc.p, (double *)dg.p, dw.p,
+                     ws));
+  if (conv) TRY(download(conv, dc, (size_t)nx * DV));
+  if (grad) TRY(download(grad, dg, (size_t)nx * D));
+  return finish();
+}
+
+}  // extern "C"

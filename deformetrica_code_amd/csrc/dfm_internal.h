@@ -1,0 +1,67 @@
+// dfm_internal.h — types shared by the kernel file and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dfm {
+
+enum : int {
+  OP_CONV = 0,
+  OP_GRAD1 = 1,
+  OP_CONVGRAD1 = 2,
+  OP_VARLIN = 3,
+  OP_GRADDIM = 4,
+  OP_GRADDIFF = 5
+};
+
+constexpr int kBBoxParts = 256;
+
+// Launch parameters of the pair-sum kernel (passed by value).
+struct KParams {
+  const double *x;   // targets (nx x D)
+  const double *ta;  // per-target extra: alpha (nx x DV) or beta (GRADDIFF)
+  const double *tb;  // per-target extra: eta (GRADDIFF)
+  const double *y;   // sources (ny x D)
+  const double *w;   // weights (ny x DV)
+  const double *se;  // per-source extra: eta_j (GRADDIFF)
+  const uint32_t *perm;
+  double *partial;
+  double *out0;
+  double *out1;
+  long nx, ny;
+  long split_len;
+  int nsplit;
+  int dim;
+  double sigma;
+  double s;    // coordinate scale
+  double cg;   // gradient factor  -2 / (sigma^2 s)
+  double ch1;  // GradDiff:  2 / sigma^2
+  double ch2;  // GradDiff: -4 / (sigma^4 s^2)
+};
+
+// Work decomposition and workspace layout of one call.
+struct Plan {
+  int R;            // targets per lane
+  long chunks;      // workgroups along the targets
+  int nsplit;       // source splits
+  long split_len;   // sources per split (multiple of the tile)
+  size_t off_bbox, off_keys_in, off_keys_out, off_vals_in, off_perm, off_sort_tmp, off_partial;
+  size_t sort_tmp_bytes;
+  size_t work_bytes;
+};
+
+int set_error(int code, const char *fmt, ...);
+const char *last_error();
+void clear_error();
+
+Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
+int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
+                   char *work);
+int fill(hipStream_t st, double *dst, long n, double v);
+void profile_enable(bool on);
+int profile_read(double *total_ms, long *launches);
+
+}  // namespace dfm

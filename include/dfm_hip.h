@@ -1,0 +1,162 @@
+/*
+ * dfm_hip.h — C ABI of the MI355X (gfx950) Gaussian kernel-sum engine.
+ *
+ * K(x, y) = exp(-|x - y|^2 / sigma^2)   (Deformetrica ExactKernel.h:71-86; sigma^2 in the
+ * denominator, no factor 2).  Every matrix crossing this boundary is FP64, row-major
+ * array-of-structures (x[i*dimpoint + k]) — the layout the reference's CUDA plugin hands
+ * its library after vectorise_row_wise (CUDAExactKernel.cxx:46-48).
+ *
+ * Two families of entry points:
+ *   dfm_gauss_*      host pointers, synchronous: drop-in replacements for the
+ *                    reference's GaussGpu*Conv1D<double, DP, DV> host wrappers
+ *                    (lib/cuda_convolutions/GpuConv1D.h), with the point and weight
+ *                    dimensions as run-time arguments and a status code instead of the
+ *                    reference's unconditional 0.
+ *   dfm_*            device pointers, stream-ordered, caller-provided workspace (size
+ *                    from the matching dfm_*_workspace query); they never allocate.
+ *
+ * Precision (int precision):
+ *   DFM_FP64   plain FP64 arithmetic and FP64 exp — the reference's arithmetic.
+ *   DFM_FP32C  FP32 exp and multiply-add over LDS-staged source tiles, with targets
+ *              spatially sorted and re-centred in FP64 per wavefront, FP32 tile sums
+ *              folded into FP64 accumulators (compensated, cascaded summation).
+ *
+ * Supported shapes: dimpoint in {2, 3}; dimvect in {1, ..., 6} (the reference supports
+ * dimvect in {D, 2D, D(D+1)/2}, CUDAExactKernel.cxx:56-82).  Counts are non-negative and
+ * below 2^31.  Status codes below; dfm_last_error() returns the calling thread's last
+ * message.  Entry points are thread-safe; calls that share a workspace buffer must be
+ * ordered on one stream.
+ */
+#ifndef DFM_HIP_H
+#define DFM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dfm_status {
+  DFM_OK = 0,
+  DFM_ERR_ARG = 1,      /* null pointer, negative count, undersized workspace */
+  DFM_ERR_MISMATCH = 2, /* "Sources and weights count mismatch" (ExactKernel.cxx:58-59) */
+  DFM_ERR_DIM = 3,      /* unsupported dimpoint / dimvect (CUDAExactKernel.cxx:80-82) */
+  DFM_ERR_INDEX = 4,    /* "dimension index out of bounds" (CUDAExactKernel.cxx:242-243) */
+  DFM_ERR_HIP = 5,      /* HIP runtime error (message in dfm_last_error) */
+  DFM_ERR_NOMEM = 6     /* device allocation failed (host-pointer wrappers only) */
+};
+
+enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1 };
+
+/* ABI version of this header (bumped on any signature change). */
+int dfm_abi_version(void);
+/* Last error message of the calling thread ("" when none). */
+const char *dfm_last_error(void);
+
+/* Kernel timing for measurement (no reference counterpart: the reference times nothing
+ * on the device).  While enabled on the calling thread, every pair-sum kernel launch
+ * (with its split reduction, when there is one) is bracketed by HIP events on the stream
+ * it runs on.  dfm_profile_read waits for the recorded events, returns their summed
+ * duration and launch count, and forgets them. */
+int dfm_profile_enable(int on);
+int dfm_profile_read(double *total_ms, long *launches);
+
+/* ------------------------------------------------------------------------------------ */
+/* Host-pointer wrappers (synchronous; allocate, copy, launch, copy back, free).        */
+
+/* gamma (nx x dv) = sum_j K(x_i, y_j) beta_j.
+ * Replaces GaussGpuEvalConv1D<TYPE, DIMPOINT, DIMVECT> (GpuConv1D.h:45-47,
+ * GpuConv1D.cu:148-155), called from CUDAExactKernel::Convolve (CUDAExactKernel.cxx:41-90). */
+int dfm_gauss_conv(int dimpoint, int dimvect, double sigma, const double *x, const double *y,
+                   const double *beta, double *gamma, int nx, int ny, int precision);
+
+/* gamma (nx x dp) = sum_j (alpha_i . beta_j) grad_x K(x_i, y_j).
+ * Replaces GaussGpuGrad1Conv1D (GpuConv1D.h:78-80), called from
+ * CUDAExactKernel::ConvolveGradient(X, alpha) (CUDAExactKernel.cxx:165-194). */
+int dfm_gauss_grad1conv(int dimpoint, int dimvect, double sigma, const double *alpha,
+                        const double *x, const double *y, const double *beta, double *gamma,
+                        int nx, int ny, int precision);
+
+/* gamma (nx x dv) = sum_j beta_j d/dx_dim K(x_i, y_j).
+ * Replaces GaussGpuGradConv1D (GpuConv1D.h:113-114), called from
+ * CUDAExactKernel::ConvolveGradient(X, dim) (CUDAExactKernel.cxx:228-259).  The reference
+ * form is the self-interaction y = x; this one takes the sources explicitly, as
+ * ExactKernel::ConvolveGradient(X, dim) does (ExactKernel.cxx:294-323). */
+int dfm_gauss_gradconv(int dimpoint, int dimvect, double sigma, const double *x, const double *y,
+                       const double *beta, unsigned dim, double *gamma, int nx, int ny,
+                       int precision);
+
+/* gamma (nx x dv x dp), gamma[i][k][l] = sum_j beta_j[k] d/dx_l K(x_i, y_j).
+ * Replaces GaussGpuGradConv_varlin_1D (GpuConv1D.h:147-148), called from
+ * CUDAExactKernel::ConvolveGradient(X) (CUDAExactKernel.cxx:261-320); output index
+ * i*dv*dp + k*dp + l as the reference reads it (CUDAExactKernel.cxx:312). */
+int dfm_gauss_gradconv_varlin(int dimpoint, int dimvect, double sigma, const double *x,
+                              const double *y, const double *beta, double *gamma, int nx, int ny,
+                              int precision);
+
+/* gamma (n x dp) = sum_j (beta_i . beta_j) H_x K(x_i, x_j) (eta_j - eta_i)
+ *               = sum_j (b_i.b_j) K_ij [(2/s^2)(eta_i-eta_j) - (4/s^4)((x_i-x_j).(eta_i-eta_j))(x_i-x_j)],
+ * i.e. the reference's GaussGpuGradDiffConv1D result already multiplied by -0.5, which is
+ * what CUDAExactKernel::ConvolveSpecialHessian returns (CUDAExactKernel.cxx:196-226;
+ * GpuConv1D.h:180-181) and what AbstractKernel::ConvolveSpecialHessian computes
+ * (AbstractKernel.cxx:22-88).  Self-interaction: sources = x, weights = beta (n x dv). */
+int dfm_gauss_graddiffconv(int dimpoint, int dimvect, double sigma, const double *x,
+                           const double *beta, const double *eta, double *gamma, int nx,
+                           int precision);
+
+/* Fused Convolve + ConvolveGradient(X, alpha) over one pass of the pairs: conv (nx x dv)
+ * and grad (nx x dp); either output may be NULL.  No single reference counterpart: it
+ * replaces the back-to-back GaussGpuEvalConv1D + GaussGpuGrad1Conv1D calls that the
+ * reference's callers make on the same sources (e.g. DeterministicAtlas.cxx:246-281). */
+int dfm_gauss_conv_grad1(int dimpoint, int dimvect, double sigma, const double *x,
+                         const double *alpha, const double *y, const double *beta, double *conv,
+                         double *grad, int nx, int ny, int precision);
+
+/* ------------------------------------------------------------------------------------ */
+/* Device-pointer, stream-ordered entry points.  `stream` is a hipStream_t (NULL = the  */
+/* default stream); `work` is device memory of at least the queried size (may be NULL  */
+/* when the query returns 0).  Pointers are device pointers; outputs are overwritten.  */
+
+/* op codes for the workspace query */
+enum dfm_op {
+  DFM_OP_CONV = 0,
+  DFM_OP_GRAD1 = 1,
+  DFM_OP_CONV_GRAD1 = 2,
+  DFM_OP_VARLIN = 3,
+  DFM_OP_GRADDIM = 4,
+  DFM_OP_GRADDIFF = 5
+};
+
+/* Bytes of workspace the device entry point for `op` needs at these sizes. */
+size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision);
+
+int dfm_conv(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+             const double *x, long nx, const double *y, const double *beta, long ny,
+             double *gamma, void *work, size_t work_bytes);
+
+int dfm_grad1(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+              const double *x, const double *alpha, long nx, const double *y,
+              const double *beta, long ny, double *gamma, void *work, size_t work_bytes);
+
+int dfm_conv_grad1(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                   const double *x, const double *alpha, long nx, const double *y,
+                   const double *beta, long ny, double *conv, double *grad, void *work,
+                   size_t work_bytes);
+
+int dfm_varlin(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+               const double *x, long nx, const double *y, const double *beta, long ny,
+               double *gamma, void *work, size_t work_bytes);
+
+int dfm_graddim(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                const double *x, long nx, const double *y, const double *beta, long ny,
+                int dim, double *gamma, void *work, size_t work_bytes);
+
+int dfm_graddiff(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                 const double *x, const double *beta, const double *eta, long n, double *gamma,
+                 void *work, size_t work_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFM_HIP_H */

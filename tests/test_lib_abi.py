@@ -1,0 +1,79 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol, and rejects
+bad arguments with the reference's error classes before touching the GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(dfm_\w+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from deformetrica_code_amd import _lib
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_symbols()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_abi_version(lib):
+    assert lib.dfm_abi_version() == 1
+
+
+def test_argument_errors_without_gpu(lib):
+    from deformetrica_code_amd import _lib
+    D = ctypes.POINTER(ctypes.c_double)
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, D)
+    # unsupported dimpoint / dimvect -> DFM_ERR_DIM (CUDAExactKernel.cxx:80-82)
+    assert lib.dfm_gauss_conv(4, 3, 1.0, p, p, p, p, 2, 2, 0) == _lib.DFM_ERR_DIM
+    assert lib.dfm_gauss_conv(3, 7, 1.0, p, p, p, p, 2, 2, 0) == _lib.DFM_ERR_DIM
+    assert b"dimvect" in lib.dfm_last_error()
+    # dim out of range -> DFM_ERR_INDEX (CUDAExactKernel.cxx:242-243)
+    assert lib.dfm_gauss_gradconv(3, 3, 1.0, p, p, p, 3, p, 2, 2, 0) == _lib.DFM_ERR_INDEX
+    # special hessian needs dimvect == dimpoint (CUDAExactKernel.cxx:212-221)
+    assert lib.dfm_gauss_graddiffconv(3, 2, 1.0, p, p, p, p, 2, 0) == _lib.DFM_ERR_DIM
+    # negative counts, NULL pointers, unknown precision
+    assert lib.dfm_gauss_conv(3, 3, 1.0, p, p, p, p, -1, 2, 0) == _lib.DFM_ERR_ARG
+    assert lib.dfm_gauss_conv(3, 3, 1.0, None, p, p, p, 2, 2, 0) == _lib.DFM_ERR_ARG
+    assert lib.dfm_gauss_conv(3, 3, 1.0, p, p, p, p, 2, 2, 9) == _lib.DFM_ERR_ARG
+    assert lib.dfm_gauss_conv(3, 3, float("nan"), p, p, p, p, 2, 2, 0) == _lib.DFM_ERR_ARG
+    # empty target set is a no-op
+    assert lib.dfm_gauss_conv(3, 3, 1.0, p, p, p, p, 0, 2, 0) == _lib.DFM_OK
+    # device entry point: undersized workspace is refused before any launch
+    need = lib.dfm_workspace_size(_lib.OP_CONV, 3, 3, 10, 100000, _lib.FP64)
+    assert need > 0
+    assert lib.dfm_conv(None, 3, 3, 1.0, 0, p, 10, p, p, 100000, p, None, 0) == _lib.DFM_ERR_ARG
+    assert b"workspace" in lib.dfm_last_error()
+
+
+def test_python_mirror_errors():
+    import numpy as np
+    from deformetrica_code_amd import HIPExactKernel
+    k = HIPExactKernel(3, 0.0)
+    assert k.GetKernelWidth() == 1e-10
+    k.SetSources(np.zeros((4, 3)))
+    k.SetWeights(np.zeros((5, 3)))
+    with pytest.raises(RuntimeError, match="mismatch"):
+        k.Convolve(np.zeros((2, 3)))
+    k.SetWeights(np.zeros((4, 3)))
+    with pytest.raises(RuntimeError, match="out of bounds"):
+        k.ConvolveGradient(np.zeros((2, 3)), 3)
+    with pytest.raises(ValueError):
+        HIPExactKernel(4)

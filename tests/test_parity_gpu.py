@@ -1,0 +1,226 @@
+"""GPU parity of the HIP kernels against the FP64 oracle, through the C ABI.
+
+The first block mirrors the reference's own GPU-vs-exact tests one to one
+(tests/unit_tests/kernels/TestKernelPrecisionCUDA.cxx:18-203): same generator, same
+shapes, same kernel width (1.88), same per-element metric min(|x-y|/|x|, |x-y|)
+(AbstractTestKernelPrecision.cxx:42) and the reference's tolerances: 1e-10 for FP64
+(AbstractTestKernelPrecision.h:78) and 1e-6 for single precision (ibid. :81), which is
+the bar for the FP32-compensated path.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = 1e-10   # AbstractTestKernelPrecision.h:78
+EPS32 = 1e-6    # AbstractTestKernelPrecision.h:81
+WIDTH = 1.88    # AbstractTestKernelPrecision.h:26
+PRECS = [("fp64", EPS64), ("fp32c", EPS32)]
+
+
+def ref_metric(exact, other):
+    exact = np.asarray(exact, dtype=np.float64)
+    other = np.asarray(other, dtype=np.float64)
+    assert exact.shape == other.shape
+    diff = np.abs(exact - other)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.where(exact != 0, diff / np.abs(exact), np.inf)
+    return float(np.max(np.minimum(rel, diff))) if exact.size else 0.0
+
+
+def normwise(exact, other):
+    den = np.linalg.norm(exact)
+    return float(np.linalg.norm(np.asarray(exact) - np.asarray(other)) / (den if den else 1.0))
+
+
+@pytest.fixture(scope="module")
+def fx(oracle_mod):
+    f = oracle_mod.reference_fixture
+    return {
+        "X2D": f(501, 2), "Y2D": f(201, 2), "W2D": f(201, 2), "Z2D": f(501, 2),
+        "X3D": f(501, 3), "Y3D": f(201, 3), "W3D": f(201, 3), "Z3D": f(501, 3),
+        "W4D": f(201, 4), "W6D": f(201, 6),
+    }
+
+
+def pair(oracle_mod, d, Y, W, prec, width=WIDTH):
+    from deformetrica_code_amd import HIPExactKernel
+    ex = oracle_mod.ExactKernel(d, width)
+    gp = HIPExactKernel(d, width, prec)
+    for k in (ex, gp):
+        k.SetSources(Y)
+        k.SetWeights(W)
+    return ex, gp
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("d,wname", [(2, "W2D"), (2, "W4D"), (3, "W3D"), (3, "W6D")])
+def test_convolve(oracle_mod, fx, prec, tol, d, wname):
+    ex, gp = pair(oracle_mod, d, fx["Y%dD" % d], fx[wname], prec)
+    X = fx["X%dD" % d]
+    assert ref_metric(ex.Convolve(X), gp.Convolve(X)) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("d", [2, 3])
+def test_convolve_gradient_alpha(oracle_mod, fx, prec, tol, d):
+    ex, gp = pair(oracle_mod, d, fx["Y%dD" % d], fx["W%dD" % d], prec)
+    X, Z = fx["X%dD" % d], fx["Z%dD" % d]
+    assert ref_metric(ex.ConvolveGradient(X, Z), gp.ConvolveGradient(X, Z)) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("d,wname", [(2, "W2D"), (2, "W4D"), (3, "W3D"), (3, "W6D")])
+def test_convolve_gradient_varlin(oracle_mod, fx, prec, tol, d, wname):
+    ex, gp = pair(oracle_mod, d, fx["Y%dD" % d], fx[wname], prec)
+    X = fx["X%dD" % d]
+    e, g = ex.ConvolveGradient(X), gp.ConvolveGradient(X)
+    for j in range(X.shape[0]):   # one (Dv x d) matrix per point, as the reference compares them
+        assert ref_metric(e[j], g[j]) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("d", [2, 3])
+def test_convolve_gradient_dim(oracle_mod, fx, prec, tol, d):
+    ex, gp = pair(oracle_mod, d, fx["Y%dD" % d], fx["W%dD" % d], prec)
+    Y = fx["Y%dD" % d]
+    for dim in range(d):
+        assert ref_metric(ex.ConvolveGradient(Y, dim), gp.ConvolveGradient(Y, dim)) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("d", [2, 3])
+def test_convolve_special_hessian(oracle_mod, fx, prec, tol, d):
+    W = fx["W%dD" % d]
+    ex, gp = pair(oracle_mod, d, fx["Y%dD" % d], W, prec)
+    assert ref_metric(ex.ConvolveSpecialHessian(W), gp.ConvolveSpecialHessian(W)) <= tol
+
+
+# ---------------------------------------------------------------------------------------
+# beyond the reference's fixture: the scales the engine is used at
+
+
+def _data(seed, n, m, d, dv, lo=0.0, hi=1.0):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(lo, hi, size=(n, d))
+    Y = rng.uniform(lo, hi, size=(m, d))
+    W = rng.normal(size=(m, dv))
+    A = rng.normal(size=(n, dv))
+    return X, Y, W, A
+
+
+@pytest.mark.parametrize("n,m,sigma", [(4096, 4096, 0.05), (3000, 5000, 0.2), (2500, 1300, 0.02)])
+def test_conv_grad1_fp32c_normwise(oracle_mod, n, m, sigma):
+    """Fused Convolve + ConvolveGradient(X, alpha): norm-wise relative error <= 1e-6 and
+    every element within 1e-6 of the output's RMS."""
+    from deformetrica_code_amd import HIPExactKernel
+    X, Y, W, A = _data(n + m, n, m, 3, 3)
+    ex = oracle_mod.ExactKernel(3, sigma)
+    ex.SetSources(Y)
+    ex.SetWeights(W)
+    c_ref, g_ref = ex.Convolve(X), ex.ConvolveGradient(X, A)
+    for prec, tol in (("fp32c", 1e-6), ("fp64", 1e-12)):
+        gp = HIPExactKernel(3, sigma, prec)
+        gp.SetSources(Y)
+        gp.SetWeights(W)
+        c, g = gp.ConvolveAndGradient(X, A)
+        for ref, got in ((c_ref, c), (g_ref, g)):
+            assert normwise(ref, got) <= tol, (prec, normwise(ref, got))
+            rms = np.sqrt(np.mean(ref ** 2))
+            assert np.max(np.abs(ref - got)) <= tol * rms * 10
+
+
+def test_fp32c_far_from_origin(oracle_mod):
+    """Coordinates ~1e3 with sigma 0.05: FP32 absolute coordinates would lose ~1e-4 of the
+    differences; the per-wavefront FP64 re-centring keeps the result at FP32 accuracy."""
+    from deformetrica_code_amd import HIPExactKernel
+    X, Y, W, A = _data(7, 1500, 2000, 3, 3)
+    X += 1000.0
+    Y += 1000.0
+    X, Y = X * 0.3, Y * 0.3
+    ex = oracle_mod.ExactKernel(3, 0.05)
+    gp = HIPExactKernel(3, 0.05, "fp32c")
+    for k in (ex, gp):
+        k.SetSources(Y)
+        k.SetWeights(W)
+    assert normwise(ex.Convolve(X), gp.Convolve(X)) <= 1e-6
+    assert normwise(ex.ConvolveGradient(X, A), gp.ConvolveGradient(X, A)) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("n,m", [(1, 1), (63, 65), (257, 255), (10, 50000), (70000, 3)])
+def test_ragged_sizes_and_source_splits(oracle_mod, prec, n, m):
+    from deformetrica_code_amd import HIPExactKernel
+    X, Y, W, A = _data(n * 31 + m, n, m, 2, 2)
+    sigma = 0.1
+    ex = oracle_mod.ExactKernel(2, sigma)
+    gp = HIPExactKernel(2, sigma, prec)
+    for k in (ex, gp):
+        k.SetSources(Y)
+        k.SetWeights(W)
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    assert normwise(ex.Convolve(X), gp.Convolve(X)) <= tol
+    assert normwise(ex.ConvolveGradient(X, A), gp.ConvolveGradient(X, A)) <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_empty_sources_give_zeros(prec):
+    from deformetrica_code_amd import HIPExactKernel
+    gp = HIPExactKernel(3, 0.3, prec)
+    gp.SetSources(np.zeros((0, 3)))
+    gp.SetWeights(np.zeros((0, 3)))
+    X = np.random.default_rng(0).uniform(size=(100, 3))
+    np.testing.assert_array_equal(gp.Convolve(X), np.zeros((100, 3)))
+    np.testing.assert_array_equal(gp.ConvolveGradient(X, np.ones((100, 3))), np.zeros((100, 3)))
+    assert gp.Convolve(np.zeros((0, 3))).shape == (0, 3)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_coincident_and_duplicate_points(oracle_mod, prec):
+    from deformetrica_code_amd import HIPExactKernel
+    rng = np.random.default_rng(5)
+    P = rng.uniform(size=(50, 3))
+    Y = np.concatenate([P, P, P[:10]])   # duplicates
+    W = rng.normal(size=(Y.shape[0], 3))
+    X = np.concatenate([P, P[:5]])       # coincident with sources
+    ex = oracle_mod.ExactKernel(3, 0.2)
+    gp = HIPExactKernel(3, 0.2, prec)
+    for k in (ex, gp):
+        k.SetSources(Y)
+        k.SetWeights(W)
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    assert normwise(ex.Convolve(X), gp.Convolve(X)) <= tol
+    assert normwise(ex.ConvolveGradient(X), gp.ConvolveGradient(X)) <= tol * 5
+
+
+def test_device_entry_points_match_host_wrappers(oracle_mod):
+    import torch
+    from deformetrica_code_amd import HIPExactKernel, device
+    X, Y, W, A = _data(11, 2000, 3000, 3, 3)
+    sigma = 0.1
+    t = lambda a: torch.from_numpy(a).cuda()
+    for prec in ("fp64", "fp32c"):
+        gp = HIPExactKernel(3, sigma, prec)
+        gp.SetSources(Y)
+        gp.SetWeights(W)
+        c_h, g_h = gp.ConvolveAndGradient(X, A)
+        c_d, g_d = device.conv_grad1(t(X), t(A), t(Y), t(W), sigma, prec)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(c_d.cpu().numpy(), c_h)
+        np.testing.assert_array_equal(g_d.cpu().numpy(), g_h)
+        v_d = device.varlin(t(X), t(Y), t(W), sigma, prec).cpu().numpy()
+        np.testing.assert_array_equal(v_d, gp.ConvolveGradient(X))
+        s_d = device.graddiff(t(Y), t(W), t(W * 0.5), sigma, prec).cpu().numpy()
+        np.testing.assert_array_equal(s_d, gp.ConvolveSpecialHessian(W * 0.5))
+
+
+def test_fp32c_is_deterministic():
+    from deformetrica_code_amd import HIPExactKernel
+    X, Y, W, A = _data(12, 20000, 20000, 3, 3)
+    gp = HIPExactKernel(3, 0.05, "fp32c")
+    gp.SetSources(Y)
+    gp.SetWeights(W)
+    a = gp.ConvolveAndGradient(X, A)
+    b = gp.ConvolveAndGradient(X, A)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
