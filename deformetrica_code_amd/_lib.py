@@ -12,10 +12,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libdfm_hip.so")
 
 DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, DFM_ERR_NOMEM = range(7)
-FP64, FP32C = 0, 1
+FP64, FP32C, FP32X = 0, 1, 2
 OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF = range(6)
 
-PRECISIONS = {"fp64": FP64, "fp32c": FP32C}
+PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X}
 
 
 class DfmError(RuntimeError):

@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libdfm_hip.so")
 
-SOURCES = ["pairsum.hip", "capi.hip"]
+SOURCES = ["pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -19,6 +19,18 @@ enum : int {
 
 constexpr int kBBoxParts = 256;
 
+// Source tile (LDS) per wavefront and, for FP32, sources summed in FP32 before each fold
+// into the FP64 accumulators.
+constexpr int kTileF64 = 128;
+constexpr int kTileF32 = 256;
+constexpr int kFoldF32 = 16;
+
+// FP32 paths: a wavefront whose targets span more than this many scaled kernel units
+// (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
+constexpr double kExactExtent = 4.0;
+
+enum : int { MODE_ADAPTIVE = 0, MODE_EXACT = 1 };
+
 // Launch parameters of the pair-sum kernel (passed by value).
 struct KParams {
   const double *x;   // targets (nx x D)
@@ -40,6 +52,7 @@ struct KParams {
   double cg;   // gradient factor  -2 / (sigma^2 s)
   double ch1;  // GradDiff:  2 / sigma^2
   double ch2;  // GradDiff: -4 / (sigma^4 s^2)
+  double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
 };
 
 // Work decomposition and workspace layout of one call.
@@ -61,6 +74,11 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
                    char *work);
 int fill(hipStream_t st, double *dst, long n, double v);
+
+// one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)
+int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
+int launch_op_f32c(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
+int launch_op_f32x(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
 void profile_enable(bool on);
 int profile_read(double *total_ms, long *launches);
 

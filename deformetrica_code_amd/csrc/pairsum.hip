@@ -1,33 +1,7 @@
-// pairsum.hip — Gaussian kernel sums on gfx950 (MI355X).
-//
-// One templated wavefront kernel evaluates every op of the engine: each 64-lane
-// workgroup owns R*64 targets held in registers, streams the sources through a private
-// LDS tile, evaluates K(x_i, y_j) = exp(-|x_i - y_j|^2 / sigma^2) once per pair and
-// feeds it to the op's accumulators:
-//
-//   CONV       sum_j K w_j                                  ExactKernel.cxx:51-74
-//   GRAD1      sum_j (a_i.w_j) grad_x K                      ExactKernel.cxx:264-275
-//   CONVGRAD1  both of the above in one pass
-//   VARLIN     sum_j w_j[k] d/dx_l K                         ExactKernel.cxx:166-202
-//   GRADDIM    sum_j w_j d/dx_dim K                          ExactKernel.cxx:294-323
-//   GRADDIFF   sum_j (b_i.b_j) H_x K (eta_j - eta_i)         AbstractKernel.cxx:22-88
-//
-// FP64 path: coordinates scaled by 1/sigma, K = exp(-r^2) in FP64 — the arithmetic of the
-// reference's ExactKernel.
-// FP32C path: targets are ordered along a Morton curve (radix sort of 30-bit keys) so
-// each wavefront's targets are spatially compact; each wavefront re-centres its targets
-// and every staged source on the centre of its targets' bounding box in FP64, scales by
-// sqrt(log2 e)/sigma and only then rounds to FP32, so the FP32 differences carry the
-// rounding of the small centred coordinates rather than of the absolute ones.
-// K = exp2(-r'^2) on the transcendental unit (v_exp_f32).  Each lane accumulates one
-// source tile in FP32 and folds the tile sum into FP64 accumulators (cascaded,
-// compensated summation: the FP32 rounding is bounded by the tile's partial sum, never by
-// the running total).
-//
-// Work split: grid.x = target chunks of R*64, grid.y = source splits.  With one split the
-// kernel writes the scaled results straight to the caller's output (through the sort
-// permutation); with several it writes FP64 partials that k_reduce_splits sums in split
-// order, so the result is deterministic.
+// pairsum.hip — host side of the pair-sum engine: per-thread error and timing state,
+// the Morton ordering of the targets (bounding box, 30-bit keys, rocPRIM radix sort),
+// the work plan and workspace layout of a call, and the launch of the per-precision
+// kernel instantiations (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip).
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -94,321 +68,6 @@ int profile_read(double *total_ms, long *launches) {
   if (total_ms) *total_ms = tot;
   if (launches) *launches = cnt;
   return rc;
-}
-
-// ------------------------------------------------------------------------------------
-// precision traits
-
-template <typename T>
-struct Prec;
-
-template <>
-struct Prec<float> {
-  static constexpr bool kCenter = true;
-  static constexpr int kTile = 256;
-  __device__ __forceinline__ static float kexp(float r2) { return __builtin_amdgcn_exp2f(-r2); }
-  static double scale(double sigma) { return std::sqrt(1.4426950408889634074) / sigma; }
-};
-
-template <>
-struct Prec<double> {
-  static constexpr bool kCenter = false;
-  static constexpr int kTile = 128;
-  __device__ __forceinline__ static double kexp(double r2) { return exp(-r2); }
-  static double scale(double sigma) { return 1.0 / sigma; }
-};
-
-// ------------------------------------------------------------------------------------
-// op shapes
-
-template <int OP, int D, int DV>
-struct Shape {
-  // per-target extra inputs: alpha (GRAD1, CONVGRAD1) or beta + eta (GRADDIFF)
-  static constexpr int NTE = (OP == OP_GRAD1 || OP == OP_CONVGRAD1) ? DV
-                             : (OP == OP_GRADDIFF)                  ? DV + D
-                                                                    : 0;
-  // per-source record: y, w (+ eta_j for GRADDIFF)
-  static constexpr int NSRC = D + DV + (OP == OP_GRADDIFF ? D : 0);
-  static constexpr int NSP = (NSRC + 3) / 4 * 4;
-  static constexpr int NACC = OP == OP_CONV        ? DV
-                              : OP == OP_GRAD1     ? D
-                              : OP == OP_CONVGRAD1 ? DV + D
-                              : OP == OP_VARLIN    ? DV * D
-                              : OP == OP_GRADDIM   ? DV
-                                                   : 2 * D;
-};
-
-// ------------------------------------------------------------------------------------
-// per-pair update.  xi: scaled (centred) target; te: per-target extras; rec: staged source
-// record [y (D) | w (DV) | eta_j (D)]; acc: the op's FP accumulators.
-
-template <int OP, int D, int DV, int AUX, typename T>
-__device__ __forceinline__ void pair_update(const T *xi, const T *te, const T *rec, T *acc) {
-  T d[D];
-  T r2 = T(0);
-#pragma unroll
-  for (int k = 0; k < D; k++) {
-    d[k] = xi[k] - rec[k];
-    r2 += d[k] * d[k];
-  }
-  const T K = Prec<T>::kexp(r2);
-  const T *w = rec + D;
-  if constexpr (OP == OP_CONV) {
-#pragma unroll
-    for (int k = 0; k < DV; k++) acc[k] += K * w[k];
-  } else if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
-    constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
-    if constexpr (OP == OP_CONVGRAD1) {
-#pragma unroll
-      for (int k = 0; k < DV; k++) acc[k] += K * w[k];
-    }
-    T sa = te[0] * w[0];
-#pragma unroll
-    for (int k = 1; k < DV; k++) sa += te[k] * w[k];
-    const T t = K * sa;
-#pragma unroll
-    for (int l = 0; l < D; l++) acc[G + l] += t * d[l];
-  } else if constexpr (OP == OP_VARLIN) {
-#pragma unroll
-    for (int k = 0; k < DV; k++) {
-      const T t = K * w[k];
-#pragma unroll
-      for (int l = 0; l < D; l++) acc[k * D + l] += t * d[l];
-    }
-  } else if constexpr (OP == OP_GRADDIM) {
-    const T t = K * d[AUX];
-#pragma unroll
-    for (int k = 0; k < DV; k++) acc[k] += t * w[k];
-  } else {  // OP_GRADDIFF
-    T bb = te[0] * w[0];
-#pragma unroll
-    for (int k = 1; k < DV; k++) bb += te[k] * w[k];
-    const T *eta_i = te + DV;
-    const T *eta_j = rec + D + DV;
-    T e[D];
-    T dot = T(0);
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      e[k] = eta_i[k] - eta_j[k];
-      dot += d[k] * e[k];
-    }
-    const T t = K * bb;
-    const T t2 = t * dot;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      acc[k] += t * e[k];
-      acc[D + k] += t2 * d[k];
-    }
-  }
-}
-
-// Scale the FP64 accumulators of one target and store them at original index `o`.
-template <int OP, int D, int DV>
-__device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
-  if constexpr (OP == OP_CONV) {
-#pragma unroll
-    for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
-  } else if constexpr (OP == OP_GRAD1) {
-#pragma unroll
-    for (int l = 0; l < D; l++) p.out0[o * D + l] = p.cg * a[l];
-  } else if constexpr (OP == OP_CONVGRAD1) {
-    if (p.out0) {
-#pragma unroll
-      for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
-    }
-    if (p.out1) {
-#pragma unroll
-      for (int l = 0; l < D; l++) p.out1[o * D + l] = p.cg * a[DV + l];
-    }
-  } else if constexpr (OP == OP_VARLIN) {
-#pragma unroll
-    for (int k = 0; k < DV * D; k++) p.out0[o * DV * D + k] = p.cg * a[k];
-  } else if constexpr (OP == OP_GRADDIM) {
-#pragma unroll
-    for (int k = 0; k < DV; k++) p.out0[o * DV + k] = p.cg * a[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < D; k++) p.out0[o * D + k] = p.ch1 * a[k] + p.ch2 * a[D + k];
-  }
-}
-
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  return v;
-}
-
-// ------------------------------------------------------------------------------------
-// the pair-sum kernel
-
-template <typename T, int OP, int D, int DV, int AUX, int R>
-__global__ __launch_bounds__(64) void k_pairsum(KParams p) {
-  using S = Shape<OP, D, DV>;
-  constexpr int NSP = S::NSP;
-  constexpr int NACC = S::NACC;
-  constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
-  constexpr int TS = Prec<T>::kTile;
-  constexpr int REC_PER_LANE = TS / 64;
-  static_assert(NSP % 4 == 0, "records are whole 16-byte groups");
-
-  __shared__ T sm[TS * NSP];
-
-  const int lane = threadIdx.x;
-  const long base = (long)blockIdx.x * (64 * R);
-  const int split = blockIdx.y;
-
-  // ---- targets -> registers
-  long pos[R];
-  long orig[R];
-  bool valid[R];
-  double xd[R][D];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    pos[r] = base + r * 64 + lane;
-    valid[r] = pos[r] < p.nx;
-    orig[r] = valid[r] ? (p.perm ? (long)p.perm[pos[r]] : pos[r]) : 0;
-#pragma unroll
-    for (int k = 0; k < D; k++) xd[r][k] = valid[r] ? p.x[orig[r] * D + k] : 0.0;
-  }
-
-  double c[D];
-#pragma unroll
-  for (int k = 0; k < D; k++) c[k] = 0.0;
-  if constexpr (Prec<T>::kCenter) {
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      double lo = INFINITY, hi = -INFINITY;
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        if (valid[r]) {
-          lo = fmin(lo, xd[r][k]);
-          hi = fmax(hi, xd[r][k]);
-        }
-      lo = wave_min(lo);
-      hi = wave_max(hi);
-      c[k] = (lo <= hi) ? 0.5 * (lo + hi) : 0.0;
-    }
-  }
-
-  const double s = p.s;
-  T xt[R][D];
-  T te[R][NTE];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-#pragma unroll
-    for (int k = 0; k < D; k++) xt[r][k] = (T)((xd[r][k] - c[k]) * s);
-    if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
-#pragma unroll
-      for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
-    } else if constexpr (OP == OP_GRADDIFF) {
-#pragma unroll
-      for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
-#pragma unroll
-      for (int k = 0; k < D; k++) te[r][DV + k] = valid[r] ? (T)p.tb[orig[r] * D + k] : T(0);
-    } else {
-      te[r][0] = T(0);
-    }
-  }
-
-  T acc[R][NACC];
-  double accd[R][NACC];
-#pragma unroll
-  for (int r = 0; r < R; r++)
-#pragma unroll
-    for (int a = 0; a < NACC; a++) {
-      acc[r][a] = T(0);
-      accd[r][a] = 0.0;
-    }
-
-  // ---- sources, one LDS tile at a time
-  const long js = (long)split * p.split_len;
-  const long je = std::min(p.ny, js + p.split_len);
-  // a padded source sits far outside the support of K: exp(-|x - far|^2) is exactly 0
-  const T far = T(1.0e4);
-
-  for (long j0 = js; j0 < je; j0 += TS) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < REC_PER_LANE; q++) {
-      const long j = j0 + q * 64 + lane;
-      T rec[NSP];
-#pragma unroll
-      for (int k = 0; k < NSP; k++) rec[k] = T(0);
-      if (j < je) {
-#pragma unroll
-        for (int k = 0; k < D; k++) rec[k] = (T)((p.y[j * D + k] - c[k]) * s);
-#pragma unroll
-        for (int k = 0; k < DV; k++) rec[D + k] = (T)p.w[j * DV + k];
-        if constexpr (OP == OP_GRADDIFF) {
-#pragma unroll
-          for (int k = 0; k < D; k++) rec[D + DV + k] = (T)p.se[j * D + k];
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < D; k++) rec[k] = far;
-      }
-      T *dst = sm + (q * 64 + lane) * NSP;
-#pragma unroll
-      for (int k = 0; k < NSP; k++) dst[k] = rec[k];
-    }
-    __syncthreads();
-
-    const long left = je - j0;
-    const int cnt = (int)(left < TS ? ((left + 3) & ~3L) : TS);  // padded records are zero
-    for (int jj = 0; jj < cnt; jj += 4) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        T rec[NSP];
-        const T *srcp = sm + (jj + u) * NSP;
-#pragma unroll
-        for (int k = 0; k < NSP; k++) rec[k] = srcp[k];
-#pragma unroll
-        for (int r = 0; r < R; r++) pair_update<OP, D, DV, AUX, T>(xt[r], te[r], rec, acc[r]);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-      for (int a = 0; a < NACC; a++) {
-        accd[r][a] += (double)acc[r][a];
-        acc[r][a] = T(0);
-      }
-  }
-
-  // ---- results
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    if (!valid[r]) continue;
-    if (p.nsplit == 1) {
-      finalize<OP, D, DV>(accd[r], p, orig[r]);
-    } else {
-      double *dst = p.partial + ((long)split * p.nx + pos[r]) * NACC;
-#pragma unroll
-      for (int a = 0; a < NACC; a++) dst[a] = accd[r][a];
-    }
-  }
-}
-
-template <int OP, int D, int DV>
-__global__ __launch_bounds__(256) void k_reduce_splits(KParams p) {
-  constexpr int NACC = Shape<OP, D, DV>::NACC;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.nx) return;
-  double a[NACC];
-#pragma unroll
-  for (int k = 0; k < NACC; k++) a[k] = 0.0;
-  for (int sp = 0; sp < p.nsplit; sp++) {
-    const double *src = p.partial + ((long)sp * p.nx + i) * NACC;
-#pragma unroll
-    for (int k = 0; k < NACC; k++) a[k] += src[k];
-  }
-  const long o = p.perm ? (long)p.perm[i] : i;
-  finalize<OP, D, DV>(a, p, o);
 }
 
 // ------------------------------------------------------------------------------------
@@ -509,7 +168,7 @@ static int targets_per_lane(int op, int prec, int D, int DV) {
   }
 }
 
-static int tile_of(int prec) { return prec == DFM_FP64 ? Prec<double>::kTile : Prec<float>::kTile; }
+static int tile_of(int prec) { return prec == DFM_FP64 ? kTileF64 : kTileF32; }
 
 static int nacc_of(int op, int D, int DV) {
   switch (op) {
@@ -539,7 +198,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
   pl.split_len = std::max(1L, per) * TS;
   pl.nsplit = (int)std::max(1L, (ny + pl.split_len - 1) / pl.split_len);
   size_t off = 0;
-  if (prec == DFM_FP32C) {
+  if (prec != DFM_FP64) {
     pl.off_bbox = off;
     off = align256(off + sizeof(double) * 6 * kBBoxParts);
     pl.off_keys_in = off;
@@ -590,80 +249,13 @@ int fill(hipStream_t st, double *dst, long n, double v) {
   return DFM_OK;
 }
 
-// ------------------------------------------------------------------------------------
-// dispatch over (precision, op, D, DV, AUX)
-
-template <typename T, int OP, int D, int DV, int AUX>
-static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
-  dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit);
-  switch (pl.R) {
-    case 1: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1>), grid, dim3(64), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2>), grid, dim3(64), 0, st, p); break;
-    default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4>), grid, dim3(64), 0, st, p); break;
-  }
-  if (pl.nsplit > 1)
-    hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
-                       dim3(256), 0, st, p);
-}
-
-template <typename T, int OP, int D, int DV>
-static int launch_aux(hipStream_t st, const KParams &p, const Plan &pl) {
-  if constexpr (OP == OP_GRADDIM) {
-    switch (p.dim) {
-      case 0: launch_one<T, OP, D, DV, 0>(st, p, pl); break;
-      case 1: launch_one<T, OP, D, DV, 1>(st, p, pl); break;
-      default:
-        if constexpr (D == 3) launch_one<T, OP, D, DV, 2>(st, p, pl);
-        break;
-    }
-  } else {
-    launch_one<T, OP, D, DV, 0>(st, p, pl);
-  }
-  return DFM_OK;
-}
-
-template <typename T, int OP, int D>
-static int launch_dv(hipStream_t st, int DV, const KParams &p, const Plan &pl) {
-  if constexpr (OP == OP_GRADDIFF) {
-    if (DV != D) return set_error(DFM_ERR_DIM, "GradDiff needs dimvect == dimpoint");
-    return launch_aux<T, OP, D, D>(st, p, pl);
-  } else {
-    switch (DV) {
-      case 1: return launch_aux<T, OP, D, 1>(st, p, pl);
-      case 2: return launch_aux<T, OP, D, 2>(st, p, pl);
-      case 3: return launch_aux<T, OP, D, 3>(st, p, pl);
-      case 4: return launch_aux<T, OP, D, 4>(st, p, pl);
-      case 5: return launch_aux<T, OP, D, 5>(st, p, pl);
-      case 6: return launch_aux<T, OP, D, 6>(st, p, pl);
-    }
-  }
-  return set_error(DFM_ERR_DIM, "unsupported dimvect %d", DV);
-}
-
-template <typename T, int OP>
-static int launch_d(hipStream_t st, int D, int DV, const KParams &p, const Plan &pl) {
-  if (D == 2) return launch_dv<T, OP, 2>(st, DV, p, pl);
-  if (D == 3) return launch_dv<T, OP, 3>(st, DV, p, pl);
-  return set_error(DFM_ERR_DIM, "unsupported dimpoint %d", D);
-}
-
-template <typename T>
-static int launch_op(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl) {
-  switch (op) {
-    case OP_CONV: return launch_d<T, OP_CONV>(st, D, DV, p, pl);
-    case OP_GRAD1: return launch_d<T, OP_GRAD1>(st, D, DV, p, pl);
-    case OP_CONVGRAD1: return launch_d<T, OP_CONVGRAD1>(st, D, DV, p, pl);
-    case OP_VARLIN: return launch_d<T, OP_VARLIN>(st, D, DV, p, pl);
-    case OP_GRADDIM: return launch_d<T, OP_GRADDIM>(st, D, DV, p, pl);
-    case OP_GRADDIFF: return launch_d<T, OP_GRADDIFF>(st, D, DV, p, pl);
-  }
-  return set_error(DFM_ERR_ARG, "unknown op %d", op);
-}
-
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
                    const Plan &pl, char *work) {
   const double sigma = p.sigma;
-  p.s = prec == DFM_FP64 ? Prec<double>::scale(sigma) : Prec<float>::scale(sigma);
+  // FP64: K = exp(-|x' - y'|^2) with x' = x / sigma; FP32: K = exp2(-|x' - y'|^2) with
+  // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
+  p.s = prec == DFM_FP64 ? 1.0 / sigma : std::sqrt(1.4426950408889634074) / sigma;
+  p.exact_extent = kExactExtent;
   const double s2 = sigma * sigma;
   p.cg = -2.0 / (s2 * p.s);
   p.ch1 = 2.0 / s2;
@@ -672,7 +264,7 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
   p.split_len = pl.split_len;
   p.perm = nullptr;
   p.partial = pl.nsplit > 1 ? (double *)(work + pl.off_partial) : nullptr;
-  if (prec == DFM_FP32C) {
+  if (prec != DFM_FP64) {
     int rc = sort_targets(st, p.x, p.nx, D, work, pl);
     if (rc != DFM_OK) return rc;
     p.perm = (const uint32_t *)(work + pl.off_perm);
@@ -683,8 +275,9 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
     (void)hipEventRecord(ev0, st);
   }
-  int rc = prec == DFM_FP64 ? launch_op<double>(st, op, D, DV, p, pl)
-                            : launch_op<float>(st, op, D, DV, p, pl);
+  int rc = prec == DFM_FP64    ? launch_op_f64(st, op, D, DV, p, pl)
+           : prec == DFM_FP32X ? launch_op_f32x(st, op, D, DV, p, pl)
+                               : launch_op_f32c(st, op, D, DV, p, pl);
   if (g_profile.on) {
     (void)hipEventRecord(ev1, st);
     g_profile.events.push_back({ev0, ev1});

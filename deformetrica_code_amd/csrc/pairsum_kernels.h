@@ -1,0 +1,492 @@
+// pairsum_kernels.h — the pair-sum kernel templates, instantiated once per precision in
+// pairsum_f64.hip, pairsum_f32c.hip and pairsum_f32x.hip (compiled in parallel).
+//
+// One templated wavefront kernel evaluates every op of the engine: each 64-lane
+// workgroup owns R*64 targets held in registers, streams the sources through a private
+// LDS tile, evaluates K(x_i, y_j) = exp(-|x_i - y_j|^2 / sigma^2) once per pair and
+// feeds it to the op's accumulators:
+//
+//   CONV       sum_j K w_j                                  ExactKernel.cxx:51-74
+//   GRAD1      sum_j (a_i.w_j) grad_x K                      ExactKernel.cxx:264-275
+//   CONVGRAD1  both of the above in one pass
+//   VARLIN     sum_j w_j[k] d/dx_l K                         ExactKernel.cxx:166-202
+//   GRADDIM    sum_j w_j d/dx_dim K                          ExactKernel.cxx:294-323
+//   GRADDIFF   sum_j (b_i.b_j) H_x K (eta_j - eta_i)         AbstractKernel.cxx:22-88
+//
+// FP64 path: coordinates scaled by 1/sigma, K = exp(-r^2) in FP64 — the arithmetic of the
+// reference's ExactKernel.
+// FP32C path: targets are ordered along a Morton curve (radix sort of 30-bit keys) so
+// each wavefront's targets are spatially compact; each wavefront re-centres its targets
+// and every staged source on the centre of its targets' bounding box in FP64, scales by
+// sqrt(log2 e)/sigma and only then rounds to FP32, so the FP32 differences carry the
+// rounding of the small centred coordinates rather than of the absolute ones.
+// K = exp2(-r'^2) on the transcendental unit (v_exp_f32).  Each lane accumulates 16
+// sources in FP32 and folds that partial sum into FP64 accumulators (cascaded,
+// compensated summation: the FP32 rounding is bounded by a 16-term partial sum, never by
+// the running total).  A wavefront whose targets span more than kExactExtent scaled
+// units (sparse targets, narrow kernels) instead forms each difference from FP32 hi and
+// lo parts, (xh - yh) + (xl - yl); FP32X takes that path in every wavefront.
+//
+// Work split: grid.x = target chunks of R*64, grid.y = source splits.  With one split the
+// kernel writes the scaled results straight to the caller's output (through the sort
+// permutation); with several it writes FP64 partials that k_reduce_splits sums in split
+// order, so the result is deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "dfm_hip.h"
+#include "dfm_internal.h"
+
+namespace dfm {
+
+// ------------------------------------------------------------------------------------
+// precision traits
+
+template <typename T>
+struct Prec;
+
+// kFold: sources summed in the working precision before the partial sum is folded into
+// the FP64 accumulator.  In FP32 a 16-source partial keeps the rounding of each add
+// proportional to 16 terms even when a wide kernel makes every pair significant and the
+// sum cancels.
+template <>
+struct Prec<float> {
+  static constexpr bool kCenter = true;
+  static constexpr int kTile = kTileF32;
+  static constexpr int kFold = kFoldF32;
+  __device__ __forceinline__ static float kexp(float r2) { return __builtin_amdgcn_exp2f(-r2); }
+};
+
+template <>
+struct Prec<double> {
+  static constexpr bool kCenter = false;
+  static constexpr int kTile = kTileF64;
+  static constexpr int kFold = kTileF64;
+  __device__ __forceinline__ static double kexp(double r2) { return exp(-r2); }
+};
+
+// ------------------------------------------------------------------------------------
+// op shapes
+
+template <int OP, int D, int DV>
+struct Shape {
+  // per-target extra inputs: alpha (GRAD1, CONVGRAD1) or beta + eta (GRADDIFF)
+  static constexpr int NTE = (OP == OP_GRAD1 || OP == OP_CONVGRAD1) ? DV
+                             : (OP == OP_GRADDIFF)                  ? DV + D
+                                                                    : 0;
+  // per-source record: y, w (+ eta_j for GRADDIFF)
+  static constexpr int NSRC = D + DV + (OP == OP_GRADDIFF ? D : 0);
+  static constexpr int NSP = (NSRC + 3) / 4 * 4;
+  // with the FP32 residuals of the source coordinates appended (exact-difference path)
+  static constexpr int NSPX = (NSRC + D + 3) / 4 * 4;
+  static constexpr int NACC = OP == OP_CONV        ? DV
+                              : OP == OP_GRAD1     ? D
+                              : OP == OP_CONVGRAD1 ? DV + D
+                              : OP == OP_VARLIN    ? DV * D
+                              : OP == OP_GRADDIM   ? DV
+                                                   : 2 * D;
+};
+
+// ------------------------------------------------------------------------------------
+// per-pair update.  xi: scaled (centred) target, xl: its FP32 rounding residual (exact
+// path); te: per-target extras; rec: staged source record
+// [y (D) | w (DV) | eta_j (D, GRADDIFF) | y residual (D, exact path)]; acc: the op's
+// working-precision accumulators.
+//
+// Exact path: with x = xh + xl and y = yh + yl (FP32 hi parts and residuals of the FP64
+// centred coordinates) the difference is formed as (xh - yh) + (xl - yl), which carries
+// the rounding of the difference itself instead of the rounding of the coordinates.
+
+template <int OP, int D, int DV, int AUX, bool EXACT, typename T>
+__device__ __forceinline__ void pair_update(const T *xi, const T *xl, const T *te, const T *rec,
+                                            T *acc) {
+  constexpr int LO = Shape<OP, D, DV>::NSRC;
+  T d[D];
+  T r2 = T(0);
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    if constexpr (EXACT)
+      d[k] = (xi[k] - rec[k]) + (xl[k] - rec[LO + k]);
+    else
+      d[k] = xi[k] - rec[k];
+    r2 += d[k] * d[k];
+  }
+  const T K = Prec<T>::kexp(r2);
+  const T *w = rec + D;
+  if constexpr (OP == OP_CONV) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) acc[k] += K * w[k];
+  } else if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
+    constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
+    if constexpr (OP == OP_CONVGRAD1) {
+#pragma unroll
+      for (int k = 0; k < DV; k++) acc[k] += K * w[k];
+    }
+    T sa = te[0] * w[0];
+#pragma unroll
+    for (int k = 1; k < DV; k++) sa += te[k] * w[k];
+    const T t = K * sa;
+#pragma unroll
+    for (int l = 0; l < D; l++) acc[G + l] += t * d[l];
+  } else if constexpr (OP == OP_VARLIN) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) {
+      const T t = K * w[k];
+#pragma unroll
+      for (int l = 0; l < D; l++) acc[k * D + l] += t * d[l];
+    }
+  } else if constexpr (OP == OP_GRADDIM) {
+    const T t = K * d[AUX];
+#pragma unroll
+    for (int k = 0; k < DV; k++) acc[k] += t * w[k];
+  } else {  // OP_GRADDIFF
+    T bb = te[0] * w[0];
+#pragma unroll
+    for (int k = 1; k < DV; k++) bb += te[k] * w[k];
+    const T *eta_i = te + DV;
+    const T *eta_j = rec + D + DV;
+    T e[D];
+    T dot = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      e[k] = eta_i[k] - eta_j[k];
+      dot += d[k] * e[k];
+    }
+    const T t = K * bb;
+    const T t2 = t * dot;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      acc[k] += t * e[k];
+      acc[D + k] += t2 * d[k];
+    }
+  }
+}
+
+// Scale the FP64 accumulators of one target and store them at original index `o`.
+template <int OP, int D, int DV>
+__device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
+  if constexpr (OP == OP_CONV) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
+  } else if constexpr (OP == OP_GRAD1) {
+#pragma unroll
+    for (int l = 0; l < D; l++) p.out0[o * D + l] = p.cg * a[l];
+  } else if constexpr (OP == OP_CONVGRAD1) {
+    if (p.out0) {
+#pragma unroll
+      for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
+    }
+    if (p.out1) {
+#pragma unroll
+      for (int l = 0; l < D; l++) p.out1[o * D + l] = p.cg * a[DV + l];
+    }
+  } else if constexpr (OP == OP_VARLIN) {
+#pragma unroll
+    for (int k = 0; k < DV * D; k++) p.out0[o * DV * D + k] = p.cg * a[k];
+  } else if constexpr (OP == OP_GRADDIM) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) p.out0[o * DV + k] = p.cg * a[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; k++) p.out0[o * D + k] = p.ch1 * a[k] + p.ch2 * a[D + k];
+  }
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// ------------------------------------------------------------------------------------
+// the pair-sum kernel
+
+// Stream the sources [js, je) through the wavefront's LDS tile and accumulate.
+template <typename T, int OP, int D, int DV, int AUX, int R, bool EXACT, int NTE, int NACC>
+__device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, long js, long je,
+                                          const double *c, double s, const T (&xt)[R][D],
+                                          const T (&xl)[R][D], const T (&te)[R][NTE],
+                                          double (&accd)[R][NACC]) {
+  using S = Shape<OP, D, DV>;
+  constexpr int NSP = EXACT ? S::NSPX : S::NSP;
+  constexpr int LO = S::NSRC;
+  constexpr int TS = Prec<T>::kTile;
+  constexpr int FOLD = Prec<T>::kFold;
+  constexpr int REC_PER_LANE = TS / 64;
+  // a padded source sits far outside the support of K: exp(-|x - far|^2) is exactly 0
+  const T far = T(1.0e4);
+
+  T acc[R][NACC];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int a = 0; a < NACC; a++) acc[r][a] = T(0);
+
+  for (long j0 = js; j0 < je; j0 += TS) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < REC_PER_LANE; q++) {
+      const long j = j0 + q * 64 + lane;
+      T rec[NSP];
+#pragma unroll
+      for (int k = 0; k < NSP; k++) rec[k] = T(0);
+      if (j < je) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          const double v = (p.y[j * D + k] - c[k]) * s;
+          rec[k] = (T)v;
+          if constexpr (EXACT) rec[LO + k] = (T)(v - (double)rec[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < DV; k++) rec[D + k] = (T)p.w[j * DV + k];
+        if constexpr (OP == OP_GRADDIFF) {
+#pragma unroll
+          for (int k = 0; k < D; k++) rec[D + DV + k] = (T)p.se[j * D + k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; k++) rec[k] = far;
+      }
+      T *dst = sm + (q * 64 + lane) * NSP;
+#pragma unroll
+      for (int k = 0; k < NSP; k++) dst[k] = rec[k];
+    }
+    __syncthreads();
+
+    const long left = je - j0;
+    // records past the end of the sources were staged with zero weight far away
+    const int cnt = (int)(left < TS ? ((left + FOLD - 1) / FOLD) * FOLD : TS);
+    for (int jf = 0; jf < cnt; jf += FOLD) {
+      for (int jj = jf; jj < jf + FOLD; jj += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          T rec[NSP];
+          const T *srcp = sm + (jj + u) * NSP;
+#pragma unroll
+          for (int k = 0; k < NSP; k++) rec[k] = srcp[k];
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            pair_update<OP, D, DV, AUX, EXACT, T>(xt[r], xl[r], te[r], rec, acc[r]);
+        }
+      }
+      // fold the FOLD-source partial sums into the FP64 accumulators
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int a = 0; a < NACC; a++) {
+          accd[r][a] += (double)acc[r][a];
+          acc[r][a] = T(0);
+        }
+    }
+  }
+}
+
+template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
+__global__ __launch_bounds__(64) void k_pairsum(KParams p) {
+  using S = Shape<OP, D, DV>;
+  constexpr int NACC = S::NACC;
+  constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
+  constexpr int TS = Prec<T>::kTile;
+  constexpr int FOLD = Prec<T>::kFold;
+  static_assert(TS % FOLD == 0 && FOLD % 4 == 0, "fold interval divides the tile");
+  static_assert(S::NSP % 4 == 0 && S::NSPX % 4 == 0, "records are whole 16-byte groups");
+
+  __shared__ __attribute__((aligned(16))) T sm[TS * (Prec<T>::kCenter ? S::NSPX : S::NSP)];
+
+  const int lane = threadIdx.x;
+  const long base = (long)blockIdx.x * (64 * R);
+  const int split = blockIdx.y;
+
+  // ---- targets -> registers
+  long pos[R];
+  long orig[R];
+  bool valid[R];
+  double xd[R][D];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    pos[r] = base + r * 64 + lane;
+    valid[r] = pos[r] < p.nx;
+    orig[r] = valid[r] ? (p.perm ? (long)p.perm[pos[r]] : pos[r]) : 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) xd[r][k] = valid[r] ? p.x[orig[r] * D + k] : 0.0;
+  }
+
+  // centre of the wavefront's targets (FP32 paths) and their extent in kernel units
+  double c[D];
+  double extent = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) c[k] = 0.0;
+  if constexpr (Prec<T>::kCenter) {
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      double lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (valid[r]) {
+          lo = fmin(lo, xd[r][k]);
+          hi = fmax(hi, xd[r][k]);
+        }
+      lo = wave_min(lo);
+      hi = wave_max(hi);
+      c[k] = (lo <= hi) ? 0.5 * (lo + hi) : 0.0;
+      extent = (lo <= hi) ? fmax(extent, hi - lo) : extent;
+    }
+  }
+
+  const double s = p.s;
+  T xt[R][D];
+  T xl[R][D];
+  T te[R][NTE];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const double v = (xd[r][k] - c[k]) * s;
+      xt[r][k] = (T)v;
+      xl[r][k] = (T)(v - (double)xt[r][k]);
+    }
+    if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
+#pragma unroll
+      for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
+    } else if constexpr (OP == OP_GRADDIFF) {
+#pragma unroll
+      for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
+#pragma unroll
+      for (int k = 0; k < D; k++) te[r][DV + k] = valid[r] ? (T)p.tb[orig[r] * D + k] : T(0);
+    } else {
+      te[r][0] = T(0);
+    }
+  }
+
+  double accd[R][NACC];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
+
+  const long js = (long)split * p.split_len;
+  const long je = std::min(p.ny, js + p.split_len);
+  if constexpr (Prec<T>::kCenter) {
+    // wavefront-uniform choice: coordinates re-centred on a box wider than a few kernel
+    // widths lose too much to FP32 rounding, so such wavefronts form exact differences
+    const bool exact = MODE == MODE_EXACT || extent * s > p.exact_extent;
+    if (exact)
+      run_tiles<T, OP, D, DV, AUX, R, true, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
+    else
+      run_tiles<T, OP, D, DV, AUX, R, false, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
+  } else {
+    run_tiles<T, OP, D, DV, AUX, R, false, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
+  }
+
+  // ---- results
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (!valid[r]) continue;
+    if (p.nsplit == 1) {
+      finalize<OP, D, DV>(accd[r], p, orig[r]);
+    } else {
+      double *dst = p.partial + ((long)split * p.nx + pos[r]) * NACC;
+#pragma unroll
+      for (int a = 0; a < NACC; a++) dst[a] = accd[r][a];
+    }
+  }
+}
+
+template <int OP, int D, int DV>
+__global__ __launch_bounds__(256) void k_reduce_splits(KParams p) {
+  constexpr int NACC = Shape<OP, D, DV>::NACC;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nx) return;
+  double a[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; k++) a[k] = 0.0;
+  for (int sp = 0; sp < p.nsplit; sp++) {
+    const double *src = p.partial + ((long)sp * p.nx + i) * NACC;
+#pragma unroll
+    for (int k = 0; k < NACC; k++) a[k] += src[k];
+  }
+  const long o = p.perm ? (long)p.perm[i] : i;
+  finalize<OP, D, DV>(a, p, o);
+}
+
+// ------------------------------------------------------------------------------------
+// dispatch over (op, D, DV, AUX, R) for one (T, MODE)
+
+template <typename T, int MODE, int OP, int D, int DV, int AUX>
+static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
+  dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit);
+  switch (pl.R) {
+    case 1: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE>), grid, dim3(64), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE>), grid, dim3(64), 0, st, p); break;
+    default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4, MODE>), grid, dim3(64), 0, st, p); break;
+  }
+  if (pl.nsplit > 1)
+    hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
+                       dim3(256), 0, st, p);
+}
+
+template <typename T, int MODE, int OP, int D, int DV>
+static int launch_aux(hipStream_t st, const KParams &p, const Plan &pl) {
+  if constexpr (OP == OP_GRADDIM) {
+    switch (p.dim) {
+      case 0: launch_one<T, MODE, OP, D, DV, 0>(st, p, pl); break;
+      case 1: launch_one<T, MODE, OP, D, DV, 1>(st, p, pl); break;
+      default:
+        if constexpr (D == 3) launch_one<T, MODE, OP, D, DV, 2>(st, p, pl);
+        break;
+    }
+  } else {
+    launch_one<T, MODE, OP, D, DV, 0>(st, p, pl);
+  }
+  return DFM_OK;
+}
+
+template <typename T, int MODE, int OP, int D>
+static int launch_dv(hipStream_t st, int DV, const KParams &p, const Plan &pl) {
+  if constexpr (OP == OP_GRADDIFF) {
+    if (DV != D) return set_error(DFM_ERR_DIM, "GradDiff needs dimvect == dimpoint");
+    return launch_aux<T, MODE, OP, D, D>(st, p, pl);
+  } else {
+    switch (DV) {
+      case 1: return launch_aux<T, MODE, OP, D, 1>(st, p, pl);
+      case 2: return launch_aux<T, MODE, OP, D, 2>(st, p, pl);
+      case 3: return launch_aux<T, MODE, OP, D, 3>(st, p, pl);
+      case 4: return launch_aux<T, MODE, OP, D, 4>(st, p, pl);
+      case 5: return launch_aux<T, MODE, OP, D, 5>(st, p, pl);
+      case 6: return launch_aux<T, MODE, OP, D, 6>(st, p, pl);
+    }
+  }
+  return set_error(DFM_ERR_DIM, "unsupported dimvect %d", DV);
+}
+
+template <typename T, int MODE, int OP>
+static int launch_d(hipStream_t st, int D, int DV, const KParams &p, const Plan &pl) {
+  if (D == 2) return launch_dv<T, MODE, OP, 2>(st, DV, p, pl);
+  if (D == 3) return launch_dv<T, MODE, OP, 3>(st, DV, p, pl);
+  return set_error(DFM_ERR_DIM, "unsupported dimpoint %d", D);
+}
+
+template <typename T, int MODE>
+static int launch_op(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl) {
+  switch (op) {
+    case OP_CONV: return launch_d<T, MODE, OP_CONV>(st, D, DV, p, pl);
+    case OP_GRAD1: return launch_d<T, MODE, OP_GRAD1>(st, D, DV, p, pl);
+    case OP_CONVGRAD1: return launch_d<T, MODE, OP_CONVGRAD1>(st, D, DV, p, pl);
+    case OP_VARLIN: return launch_d<T, MODE, OP_VARLIN>(st, D, DV, p, pl);
+    case OP_GRADDIM: return launch_d<T, MODE, OP_GRADDIM>(st, D, DV, p, pl);
+    case OP_GRADDIFF: return launch_d<T, MODE, OP_GRADDIFF>(st, D, DV, p, pl);
+  }
+  return set_error(DFM_ERR_ARG, "unknown op %d", op);
+}
+
+}  // namespace dfm

@@ -18,8 +18,11 @@
  * Precision (int precision):
  *   DFM_FP64   plain FP64 arithmetic and FP64 exp — the reference's arithmetic.
  *   DFM_FP32C  FP32 exp and multiply-add over LDS-staged source tiles, with targets
- *              spatially sorted and re-centred in FP64 per wavefront, FP32 tile sums
- *              folded into FP64 accumulators (compensated, cascaded summation).
+ *              spatially sorted and re-centred in FP64 per wavefront, FP32 partial sums
+ *              of 16 sources folded into FP64 accumulators (compensated, cascaded
+ *              summation).  A wavefront whose targets span more than 4 scaled kernel
+ *              widths forms its coordinate differences exactly (FP32 hi + lo parts).
+ *   DFM_FP32X  as DFM_FP32C with exact coordinate differences in every wavefront.
  *
  * Supported shapes: dimpoint in {2, 3}; dimvect in {1, ..., 6} (the reference supports
  * dimvect in {D, 2D, D(D+1)/2}, CUDAExactKernel.cxx:56-82).  Counts are non-negative and
@@ -46,7 +49,7 @@ enum dfm_status {
   DFM_ERR_NOMEM = 6     /* device allocation failed (host-pointer wrappers only) */
 };
 
-enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1 };
+enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2 };
 
 /* ABI version of this header (bumped on any signature change). */
 int dfm_abi_version(void);
