@@ -57,14 +57,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    import ctypes
     from deformetrica_code_amd import _lib, device
 
-    if not torch.cuda.is_available():
+    if device.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     L = _lib.load()
-    dev = torch.device("cuda:0")
     N, M, sigma = args.n, args.m, args.sigma
 
     rng_x = np.random.default_rng(20261203)
@@ -73,61 +72,68 @@ def main():
     Y = rng_x.uniform(0.0, 1.0, size=(M, 3))
     W = rng_x.normal(size=(M, 3))
     A = rng_a.normal(size=(N, 3))
-    tX, tY, tW, tA = (torch.from_numpy(a).to(dev) for a in (X, Y, W, A))
-    conv = torch.empty((N, 3), dtype=torch.float64, device=dev)
-    grad = torch.empty((N, 3), dtype=torch.float64, device=dev)
-    ws = device.Workspace(dev)
+    stream = device.Stream()
+    dX, dY, dW, dA = (device.DeviceArray.from_numpy(a, stream) for a in (X, Y, W, A))
+    conv = device.DeviceArray((N, 3))
+    grad = device.DeviceArray((N, 3))
+    ws = device.Workspace()
 
     def step(prec):
-        device.conv_grad1(tX, tA, tY, tW, sigma, prec, conv=conv, grad=grad, workspace=ws)
+        device.conv_grad1(dX, dA, dY, dW, sigma, prec, conv=conv, grad=grad, stream=stream,
+                          workspace=ws)
+
+    def timed(prec, steps):
+        """Wall time per step over `steps` calls, and the pair-sum kernel's average duration
+        from HIP events recorded on its stream."""
+        stream.synchronize()
+        L.dfm_profile_read(None, None)
+        tot, cnt = ctypes.c_double(), ctypes.c_long()
+        L.dfm_profile_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(prec)
+        stream.synchronize()
+        t1 = time.perf_counter()
+        L.dfm_profile_enable(0)
+        _lib.check(L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt)))
+        return (t1 - t0) / steps, tot.value / 1e3 / max(cnt.value, 1)
 
     # ---- FP32-compensated (headline)
     for _ in range(args.warmup):
         step("fp32c")
-    torch.cuda.synchronize()
-    L.dfm_profile_read(None, None)  # drop anything recorded before
-    import ctypes
-    tot = ctypes.c_double()
-    cnt = ctypes.c_long()
-    L.dfm_profile_enable(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step("fp32c")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    L.dfm_profile_enable(0)
-    _lib.check(L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt)))
-    s_per_step = (t1 - t0) / args.steps
-    kern_s = tot.value / 1e3 / max(cnt.value, 1)
-    c32 = conv.cpu().numpy()
-    g32 = grad.cpu().numpy()
-    print("fp32c: %.4f s/step, kernel %.4f s, %d launches" % (s_per_step, kern_s, cnt.value),
-          file=sys.stderr, flush=True)
+    s_per_step, kern_s = timed("fp32c", args.steps)
+    c32 = conv.numpy(stream)
+    g32 = grad.numpy(stream)
+    print("fp32c: %.4f s/step, kernel %.4f s" % (s_per_step, kern_s), file=sys.stderr, flush=True)
+
+    # ---- FP32 with exact differences in every wavefront
+    step("fp32x")
+    s_x, kern_x = timed("fp32x", max(1, args.steps // 4))
+    cx = conv.numpy(stream)
+    gx = grad.numpy(stream)
+    print("fp32x: %.4f s/step, kernel %.4f s" % (s_x, kern_x), file=sys.stderr, flush=True)
 
     # ---- plain FP64 form on the GPU
     step("fp64")
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    for _ in range(args.fp64_steps):
-        step("fp64")
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    s64 = (t3 - t2) / args.fp64_steps
-    c64 = conv.cpu().numpy()
-    g64 = grad.cpu().numpy()
+    s64, kern64 = timed("fp64", args.fp64_steps)
+    c64 = conv.numpy(stream)
+    g64 = grad.numpy(stream)
     print("fp64: %.4f s/step" % s64, file=sys.stderr, flush=True)
 
     def nrel(ref, got):
         return float(np.linalg.norm(ref - got) / np.linalg.norm(ref))
 
     def maxrms(ref, got):
-        return float(np.max(np.abs(ref - got)) / np.sqrt(np.mean(ref ** 2)))
+        """largest element error relative to max(|element|, RMS of the output)"""
+        scale = np.maximum(np.abs(ref), np.sqrt(np.mean(ref ** 2)))
+        return float(np.max(np.abs(ref - got) / scale))
 
     err_fp64gpu = {
         "rows": N,
         "conv_normwise": nrel(c64, c32), "grad_normwise": nrel(g64, g32),
         "conv_max_over_rms": maxrms(c64, c32), "grad_max_over_rms": maxrms(g64, g32),
+        "fp32x_conv_normwise": nrel(c64, cx), "fp32x_grad_normwise": nrel(g64, gx),
+        "fp32x_conv_max_over_rms": maxrms(c64, cx), "fp32x_grad_max_over_rms": maxrms(g64, gx),
     }
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of target rows
@@ -197,8 +203,10 @@ def main():
             "flops_per_pair": FLOPS_PER_PAIR,
         },
         "cpu_baseline": cpu,
-        "fp64_plain_gpu": {"s_per_step": s64, "pair_evals_per_s": pairs / s64,
+        "fp64_plain_gpu": {"s_per_step": s64, "kernel_s": kern64, "pair_evals_per_s": pairs / s64,
                            "speedup_fp32c_over_fp64": s64 / s_per_step},
+        "fp32x_exact_differences": {"s_per_step": s_x, "kernel_s": kern_x,
+                                    "pair_evals_per_s": pairs / s_x},
         "error_vs_fp64_gpu": err_fp64gpu,
         "error_vs_fp64_oracle": err_oracle,
     }

@@ -44,6 +44,17 @@ def load():
     L.dfm_last_error.restype = ctypes.c_char_p
     L.dfm_workspace_size.argtypes = [i, i, i, l, l, i]
     L.dfm_workspace_size.restype = sz
+    vp = ctypes.c_void_p
+    L.dfm_device_count.argtypes = [ctypes.POINTER(i)]
+    L.dfm_malloc.argtypes = [ctypes.POINTER(vp), sz]
+    L.dfm_free.argtypes = [vp]
+    L.dfm_memcpy.argtypes = [vp, vp, sz, i, vp]
+    L.dfm_stream_create.argtypes = [ctypes.POINTER(vp)]
+    L.dfm_stream_destroy.argtypes = [vp]
+    L.dfm_stream_synchronize.argtypes = [vp]
+    for name in ("dfm_device_count", "dfm_malloc", "dfm_free", "dfm_memcpy", "dfm_stream_create",
+                 "dfm_stream_destroy", "dfm_stream_synchronize"):
+        getattr(L, name).restype = i
     L.dfm_profile_enable.argtypes = [i]
     L.dfm_profile_enable.restype = i
     L.dfm_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]

@@ -79,6 +79,76 @@ int dfm_abi_version(void) { return 1; }
 
 const char *dfm_last_error(void) { return last_error(); }
 
+int dfm_device_count(int *count) {
+  clear_error();
+  if (!count) return set_error(DFM_ERR_ARG, "count is NULL");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return set_error(DFM_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  return DFM_OK;
+}
+
+int dfm_malloc(void **ptr, size_t bytes) {
+  clear_error();
+  if (!ptr) return set_error(DFM_ERR_ARG, "ptr is NULL");
+  *ptr = nullptr;
+  if (bytes == 0) return DFM_OK;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return set_error(DFM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return DFM_OK;
+}
+
+int dfm_free(void *ptr) {
+  clear_error();
+  if (!ptr) return DFM_OK;
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipFree: %s", hipGetErrorString(e));
+}
+
+int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+  clear_error();
+  if (bytes == 0) return DFM_OK;
+  if (!dst || !src) return set_error(DFM_ERR_ARG, "NULL pointer");
+  hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost
+                    : kind == 2 ? hipMemcpyDeviceToDevice
+                                : hipMemcpyDefault;
+  if (kind < 0 || kind > 2) return set_error(DFM_ERR_ARG, "unknown copy kind %d", kind);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, k, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "memcpy: %s", hipGetErrorString(e));
+}
+
+int dfm_stream_create(void **stream) {
+  clear_error();
+  if (!stream) return set_error(DFM_ERR_ARG, "stream is NULL");
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return set_error(DFM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  *stream = (void *)st;
+  return DFM_OK;
+}
+
+int dfm_stream_destroy(void *stream) {
+  clear_error();
+  if (!stream) return DFM_OK;
+  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipStreamDestroy: %s", hipGetErrorString(e));
+}
+
+int dfm_stream_synchronize(void *stream) {
+  clear_error();
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+}
+
 int dfm_profile_enable(int on) {
   profile_enable(on != 0);
   return DFM_OK;

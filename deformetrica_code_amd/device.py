@@ -1,56 +1,116 @@
-"""Stream-ordered device entry points on torch tensors.
+"""Stream-ordered device entry points on device-resident arrays.
 
-torch provides device memory and the current stream only; every op is one call into
-libdfm_hip.so.  Inputs are float64, contiguous, (rows x cols) CUDA tensors on the current
-device; results are written into new (or caller-provided) tensors.
+``DeviceArray`` is float64 device memory allocated through the library's own HIP runtime
+(dfm_malloc); ``Stream`` wraps dfm_stream_create.  Every op is one call into
+libdfm_hip.so with device pointers.  No other GPU runtime is involved, so these work in
+any process (torch bundles a separate HIP runtime and must not drive the same GPU in the
+same process).
 """
 from __future__ import annotations
 
-import torch
+import ctypes
+
+import numpy as np
 
 from . import _lib
 
 
-def _ptr(t):
-    return ctypes_ptr(t) if t is not None else None
+class DeviceArray:
+    """A C-contiguous float64 array in device memory."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self._ptr = ctypes.c_void_p()
+        _lib.check(_lib.load().dfm_malloc(ctypes.byref(self._ptr), self.nbytes))
+
+    @property
+    def ptr(self):
+        return self._ptr.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError("shape mismatch %s vs %s" % (a.shape, self.shape))
+        _lib.check(_lib.load().dfm_memcpy(self._ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                          self.nbytes, 0, _sp(stream)))
+
+    def numpy(self, stream=None) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _lib.check(_lib.load().dfm_memcpy(out.ctypes.data_as(ctypes.c_void_p), self._ptr,
+                                          self.nbytes, 1, _sp(stream)))
+        return out
+
+    def free(self):
+        if self._ptr.value:
+            _lib.load().dfm_free(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
-def ctypes_ptr(t: torch.Tensor):
-    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
-        raise ValueError("expected a contiguous float64 CUDA tensor")
-    return t.data_ptr()
+class Stream:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.load().dfm_stream_create(ctypes.byref(self._h)))
 
+    @property
+    def handle(self):
+        return self._h.value
 
-def _stream(stream):
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return s.cuda_stream
+    def synchronize(self):
+        _lib.check(_lib.load().dfm_stream_synchronize(self._h))
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                _lib.load().dfm_stream_destroy(self._h)
+        except Exception:
+            pass
 
 
 class Workspace:
     """Grow-only device scratch for the device entry points."""
 
-    def __init__(self, device=None):
-        self.device = device
+    def __init__(self):
         self.buf = None
 
     def get(self, nbytes: int):
         if nbytes == 0:
             return None, 0
-        if self.buf is None or self.buf.numel() < nbytes:
-            self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device or "cuda")
-        return self.buf.data_ptr(), self.buf.numel()
+        if self.buf is None or self.buf.nbytes < nbytes:
+            if self.buf is not None:
+                self.buf.free()
+            self.buf = DeviceArray(((nbytes + 7) // 8,))
+        return self.buf.ptr, self.buf.nbytes
 
 
-_default_ws = {}
+def _sp(stream):
+    return stream.handle if stream is not None else None
 
 
-def _ws(workspace, device):
-    if workspace is not None:
-        return workspace
-    key = device.index if device.index is not None else torch.cuda.current_device()
-    if key not in _default_ws:
-        _default_ws[key] = Workspace(device)
-    return _default_ws[key]
+def _p(a):
+    return a.ptr if a is not None else None
+
+
+_default_ws = Workspace()
+
+
+def device_count() -> int:
+    n = ctypes.c_int()
+    rc = _lib.load().dfm_device_count(ctypes.byref(n))
+    return n.value if rc == _lib.DFM_OK else 0
 
 
 def conv_grad1(x, alpha, y, w, sigma, precision="fp32c", conv=True, grad=True, stream=None,
@@ -60,15 +120,15 @@ def conv_grad1(x, alpha, y, w, sigma, precision="fp32c", conv=True, grad=True, s
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = w.shape[1]
-    cout = torch.empty((n, dv), dtype=torch.float64, device=x.device) if conv is True else conv
-    gout = torch.empty((n, d), dtype=torch.float64, device=x.device) if grad is True else grad
+    cout = DeviceArray((n, dv)) if conv is True else (conv or None)
+    gout = DeviceArray((n, d)) if grad is True else (grad or None)
     op = _lib.OP_CONV_GRAD1 if (cout is not None and gout is not None) else (
         _lib.OP_CONV if cout is not None else _lib.OP_GRAD1)
-    need = L.dfm_workspace_size(op, d, dv, n, y.shape[0], prec)
-    wp, wb = _ws(workspace, x.device).get(need)
-    _lib.check(L.dfm_conv_grad1(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x),
-                                _ptr(alpha) if gout is not None else None, n, ctypes_ptr(y),
-                                ctypes_ptr(w), y.shape[0], _ptr(cout), _ptr(gout), wp, wb))
+    ws = workspace or _default_ws
+    wp, wb = ws.get(L.dfm_workspace_size(op, d, dv, n, y.shape[0], prec))
+    _lib.check(L.dfm_conv_grad1(_sp(stream), d, dv, float(sigma), prec, x.ptr,
+                                _p(alpha) if gout is not None else None, n, y.ptr, w.ptr,
+                                y.shape[0], _p(cout), _p(gout), wp, wb))
     return cout, gout
 
 
@@ -77,10 +137,11 @@ def convolve(x, y, w, sigma, precision="fp32c", stream=None, workspace=None):
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = w.shape[1]
-    out = torch.empty((n, dv), dtype=torch.float64, device=x.device)
-    wp, wb = _ws(workspace, x.device).get(L.dfm_workspace_size(_lib.OP_CONV, d, dv, n, y.shape[0], prec))
-    _lib.check(L.dfm_conv(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x), n,
-                          ctypes_ptr(y), ctypes_ptr(w), y.shape[0], ctypes_ptr(out), wp, wb))
+    out = DeviceArray((n, dv))
+    wp, wb = (workspace or _default_ws).get(
+        L.dfm_workspace_size(_lib.OP_CONV, d, dv, n, y.shape[0], prec))
+    _lib.check(L.dfm_conv(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
+                          y.shape[0], out.ptr, wp, wb))
     return out
 
 
@@ -89,11 +150,11 @@ def grad1(x, alpha, y, w, sigma, precision="fp32c", stream=None, workspace=None)
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = w.shape[1]
-    out = torch.empty((n, d), dtype=torch.float64, device=x.device)
-    wp, wb = _ws(workspace, x.device).get(L.dfm_workspace_size(_lib.OP_GRAD1, d, dv, n, y.shape[0], prec))
-    _lib.check(L.dfm_grad1(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x),
-                           ctypes_ptr(alpha), n, ctypes_ptr(y), ctypes_ptr(w), y.shape[0],
-                           ctypes_ptr(out), wp, wb))
+    out = DeviceArray((n, d))
+    wp, wb = (workspace or _default_ws).get(
+        L.dfm_workspace_size(_lib.OP_GRAD1, d, dv, n, y.shape[0], prec))
+    _lib.check(L.dfm_grad1(_sp(stream), d, dv, float(sigma), prec, x.ptr, alpha.ptr, n, y.ptr,
+                           w.ptr, y.shape[0], out.ptr, wp, wb))
     return out
 
 
@@ -102,10 +163,11 @@ def varlin(x, y, w, sigma, precision="fp32c", stream=None, workspace=None):
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = w.shape[1]
-    out = torch.empty((n, dv, d), dtype=torch.float64, device=x.device)
-    wp, wb = _ws(workspace, x.device).get(L.dfm_workspace_size(_lib.OP_VARLIN, d, dv, n, y.shape[0], prec))
-    _lib.check(L.dfm_varlin(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x), n,
-                            ctypes_ptr(y), ctypes_ptr(w), y.shape[0], ctypes_ptr(out), wp, wb))
+    out = DeviceArray((n, dv, d))
+    wp, wb = (workspace or _default_ws).get(
+        L.dfm_workspace_size(_lib.OP_VARLIN, d, dv, n, y.shape[0], prec))
+    _lib.check(L.dfm_varlin(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
+                            y.shape[0], out.ptr, wp, wb))
     return out
 
 
@@ -114,11 +176,11 @@ def graddim(x, y, w, sigma, dim, precision="fp32c", stream=None, workspace=None)
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = w.shape[1]
-    out = torch.empty((n, dv), dtype=torch.float64, device=x.device)
-    wp, wb = _ws(workspace, x.device).get(L.dfm_workspace_size(_lib.OP_GRADDIM, d, dv, n, y.shape[0], prec))
-    _lib.check(L.dfm_graddim(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x), n,
-                             ctypes_ptr(y), ctypes_ptr(w), y.shape[0], int(dim), ctypes_ptr(out),
-                             wp, wb))
+    out = DeviceArray((n, dv))
+    wp, wb = (workspace or _default_ws).get(
+        L.dfm_workspace_size(_lib.OP_GRADDIM, d, dv, n, y.shape[0], prec))
+    _lib.check(L.dfm_graddim(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
+                             y.shape[0], int(dim), out.ptr, wp, wb))
     return out
 
 
@@ -128,8 +190,9 @@ def graddiff(x, beta, eta, sigma, precision="fp32c", stream=None, workspace=None
     prec = _lib.precision_code(precision)
     n, d = x.shape
     dv = beta.shape[1]
-    out = torch.empty((n, d), dtype=torch.float64, device=x.device)
-    wp, wb = _ws(workspace, x.device).get(L.dfm_workspace_size(_lib.OP_GRADDIFF, d, dv, n, n, prec))
-    _lib.check(L.dfm_graddiff(_stream(stream), d, dv, float(sigma), prec, ctypes_ptr(x),
-                              ctypes_ptr(beta), ctypes_ptr(eta), n, ctypes_ptr(out), wp, wb))
+    out = DeviceArray((n, d))
+    wp, wb = (workspace or _default_ws).get(
+        L.dfm_workspace_size(_lib.OP_GRADDIFF, d, dv, n, n, prec))
+    _lib.check(L.dfm_graddiff(_sp(stream), d, dv, float(sigma), prec, x.ptr, beta.ptr, eta.ptr,
+                              n, out.ptr, wp, wb))
     return out

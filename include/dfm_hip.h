@@ -56,6 +56,20 @@ int dfm_abi_version(void);
 /* Last error message of the calling thread ("" when none). */
 const char *dfm_last_error(void);
 
+/* Device memory and streams for callers that keep data resident between calls (no
+ * reference counterpart: the reference's wrappers allocate and copy inside every call,
+ * GpuConv1D.cu:98-130).  They use the same HIP runtime as the kernels, so a caller that
+ * has no HIP runtime of its own (Python through ctypes, the C++ plugin) needs nothing
+ * else.  dfm_memcpy is ordered on `stream` and returns after the copy completes.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+int dfm_device_count(int *count);
+int dfm_malloc(void **ptr, size_t bytes);
+int dfm_free(void *ptr);
+int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int dfm_stream_create(void **stream);
+int dfm_stream_destroy(void *stream);
+int dfm_stream_synchronize(void *stream);
+
 /* Kernel timing for measurement (no reference counterpart: the reference times nothing
  * on the device).  While enabled on the calling thread, every pair-sum kernel launch
  * (with its split reduction, when there is one) is bracketed by HIP events on the stream

@@ -13,9 +13,11 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
+    """A GPU visible to the library's own HIP runtime (torch is not used on the GPU here:
+    it bundles a second HIP runtime that cannot share the device within one process)."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        from deformetrica_code_amd import device
+        return device.device_count() > 0
     except Exception:
         return False
 

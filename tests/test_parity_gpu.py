@@ -229,23 +229,29 @@ def test_coincident_and_duplicate_points(oracle_mod, prec):
 
 
 def test_device_entry_points_match_host_wrappers(oracle_mod):
-    import torch
     from deformetrica_code_amd import HIPExactKernel, device
     X, Y, W, A = _data(11, 2000, 3000, 3, 3)
     sigma = 0.1
-    t = lambda a: torch.from_numpy(a).cuda()
-    for prec in ("fp64", "fp32c"):
+    t = device.DeviceArray.from_numpy
+    st = device.Stream()
+    for prec in ("fp64", "fp32c", "fp32x"):
         gp = HIPExactKernel(3, sigma, prec)
         gp.SetSources(Y)
         gp.SetWeights(W)
         c_h, g_h = gp.ConvolveAndGradient(X, A)
-        c_d, g_d = device.conv_grad1(t(X), t(A), t(Y), t(W), sigma, prec)
-        torch.cuda.synchronize()
-        np.testing.assert_array_equal(c_d.cpu().numpy(), c_h)
-        np.testing.assert_array_equal(g_d.cpu().numpy(), g_h)
-        v_d = device.varlin(t(X), t(Y), t(W), sigma, prec).cpu().numpy()
+        c_d, g_d = device.conv_grad1(t(X), t(A), t(Y), t(W), sigma, prec, stream=st)
+        st.synchronize()
+        np.testing.assert_array_equal(c_d.numpy(), c_h)
+        np.testing.assert_array_equal(g_d.numpy(), g_h)
+        np.testing.assert_array_equal(device.convolve(t(X), t(Y), t(W), sigma, prec).numpy(),
+                                      gp.Convolve(X))
+        np.testing.assert_array_equal(device.grad1(t(X), t(A), t(Y), t(W), sigma, prec).numpy(),
+                                      gp.ConvolveGradient(X, A))
+        np.testing.assert_array_equal(device.graddim(t(X), t(Y), t(W), sigma, 1, prec).numpy(),
+                                      gp.ConvolveGradient(X, 1))
+        v_d = device.varlin(t(X), t(Y), t(W), sigma, prec).numpy()
         np.testing.assert_array_equal(v_d, gp.ConvolveGradient(X))
-        s_d = device.graddiff(t(Y), t(W), t(W * 0.5), sigma, prec).cpu().numpy()
+        s_d = device.graddiff(t(Y), t(W), t(W * 0.5), sigma, prec).numpy()
         np.testing.assert_array_equal(s_d, gp.ConvolveSpecialHessian(W * 0.5))
 
 
