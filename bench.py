@@ -77,6 +77,12 @@ def main():
     conv = device.DeviceArray((N, 3))
     grad = device.DeviceArray((N, 3))
     ws = device.Workspace()
+    # one plain device copy of known size (the sources, 24 MB): the reference kernel
+    # (k_copy) for scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run
+    copy_dst = device.DeviceArray((M, 3))
+    _lib.check(L.dfm_memcpy(ctypes.c_void_p(copy_dst.ptr), ctypes.c_void_p(dY.ptr), M * 3 * 8, 2,
+                            ctypes.c_void_p(stream.handle)))
+    del copy_dst
 
     def step(prec):
         device.conv_grad1(dX, dA, dY, dW, sigma, prec, conv=conv, grad=grad, stream=stream,
@@ -142,7 +148,8 @@ def main():
     if not args.no_cpu:
         from oracle import oracle
         threads = max(1, min(16, os.cpu_count() or 1))
-        probe = 16
+        probe = threads * 2
+        oracle.convolve_grad1_threaded(X[:threads], A[:threads], Y, W, sigma, threads)  # warm
         ta = time.perf_counter()
         oracle.convolve_grad1_threaded(X[:probe], A[:probe], Y, W, sigma, threads)
         tb = time.perf_counter()

@@ -121,8 +121,15 @@ int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream)
                                 : hipMemcpyDefault;
   if (kind < 0 || kind > 2) return set_error(DFM_ERR_ARG, "unknown copy kind %d", kind);
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e = hipMemcpyAsync(dst, src, bytes, k, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipError_t e;
+  if (kind == 2) {
+    int rc = copy_d2d(st, dst, src, bytes);
+    if (rc != DFM_OK) return rc;
+    e = hipStreamSynchronize(st);
+  } else {
+    e = hipMemcpyAsync(dst, src, bytes, k, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "memcpy: %s", hipGetErrorString(e));
 }
 

@@ -74,6 +74,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
                    char *work);
 int fill(hipStream_t st, double *dst, long n, double v);
+int copy_d2d(hipStream_t st, void *dst, const void *src, size_t bytes);
 
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)
 int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);

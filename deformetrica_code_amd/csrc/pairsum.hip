@@ -151,6 +151,16 @@ __global__ __launch_bounds__(256) void k_morton_keys(const double *x, long n, in
   vals[i] = (uint32_t)i;
 }
 
+// Device-to-device copy of 16-byte groups (plus a byte tail); also the known-size
+// reference kernel for scaling HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, long n16,
+                                              unsigned char *dtail, const unsigned char *stail,
+                                              int ntail) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
 __global__ __launch_bounds__(256) void k_fill(double *dst, long n, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -241,6 +251,24 @@ int sort_targets(hipStream_t st, const double *x, long nx, int D, char *work, co
                                            perm, (unsigned)nx, 0u, 30u, st);
   if (e != hipSuccess) return set_error(DFM_ERR_HIP, "radix sort: %s", hipGetErrorString(e));
   return DFM_OK;
+}
+
+int copy_d2d(hipStream_t st, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return DFM_OK;
+  const bool aligned = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0);
+  const long n16 = aligned ? (long)(bytes / 16) : 0;
+  const size_t done = (size_t)n16 * 16;
+  const size_t rest = bytes - done;
+  if (rest > 256) {  // unaligned buffers: let the runtime copy
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+    return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "memcpy: %s", hipGetErrorString(e));
+  }
+  const long blocks = std::max(1L, std::min<long>(8192, (n16 + 255) / 256));
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (uint4 *)dst,
+                     (const uint4 *)src, n16, (unsigned char *)dst + done,
+                     (const unsigned char *)src + done, (int)rest);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "copy: %s", hipGetErrorString(e));
 }
 
 int fill(hipStream_t st, double *dst, long n, double v) {

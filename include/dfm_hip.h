@@ -61,7 +61,8 @@ const char *dfm_last_error(void);
  * GpuConv1D.cu:98-130).  They use the same HIP runtime as the kernels, so a caller that
  * has no HIP runtime of its own (Python through ctypes, the C++ plugin) needs nothing
  * else.  dfm_memcpy is ordered on `stream` and returns after the copy completes.
- * kind: 0 host->device, 1 device->host, 2 device->device. */
+ * kind: 0 host->device, 1 device->host, 2 device->device (a plain copy kernel, k_copy,
+ * for 16-byte-aligned buffers). */
 int dfm_device_count(int *count);
 int dfm_malloc(void **ptr, size_t bytes);
 int dfm_free(void *ptr);

@@ -385,7 +385,12 @@ int orc_convolve_grad1_threaded(int nthreads, int d, int dv, double h, const dou
                                 const double *alpha, long nx, const double *Y,
                                 const double *W, long ny, double *conv, double *grad) {
   if (nthreads < 1) nthreads = 1;
-  pool_job J = {d, dv, h, X, alpha, Y, W, nx, ny, conv, grad, 8, 0,
+  /* blocks of up to 8 rows, at least four blocks per worker so that small row counts
+   * still keep every worker busy */
+  long block = nx / (4L * nthreads);
+  if (block < 1) block = 1;
+  if (block > 8) block = 8;
+  pool_job J = {d, dv, h, X, alpha, Y, W, nx, ny, conv, grad, block, 0,
                 PTHREAD_MUTEX_INITIALIZER};
   pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
   if (!th) return -2;
