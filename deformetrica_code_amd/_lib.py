@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(PKG, "libdfm_hip.so")
 
 DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, DFM_ERR_NOMEM = range(7)
 FP64, FP32C, FP32X = 0, 1, 2
-OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF = range(6)
+OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP = range(7)
+EULER, HEUN = 0, 1
 
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X}
 
@@ -54,6 +55,20 @@ def load():
     L.dfm_stream_synchronize.argtypes = [vp]
     for name in ("dfm_device_count", "dfm_malloc", "dfm_free", "dfm_memcpy", "dfm_stream_create",
                  "dfm_stream_destroy", "dfm_stream_synchronize"):
+        getattr(L, name).restype = i
+    # integrators
+    L.dfm_shoot_workspace.argtypes = [i, l, i]
+    L.dfm_shoot_workspace.restype = sz
+    L.dfm_shoot.argtypes = [vp, i, d, i, i, vp, vp, l, i, d, d, vp, vp, vp, sz]
+    L.dfm_flow_workspace.argtypes = [i, l, l, i]
+    L.dfm_flow_workspace.restype = sz
+    L.dfm_flow.argtypes = [vp, i, d, i, i, vp, vp, l, vp, l, i, d, d, vp, vp, sz]
+    L.dfm_shoot_vjp.argtypes = [vp, i, d, i, vp, vp, vp, vp, l, vp, vp, vp, sz]
+    L.dfm_adjoint_workspace.argtypes = [i, l, l, i, i, i]
+    L.dfm_adjoint_workspace.restype = sz
+    L.dfm_shoot_flow_adjoint.argtypes = [vp, i, d, i, i, i, vp, vp, l, vp, l, i, d, d, i, vp, vp,
+                                         vp, vp, vp, sz]
+    for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint"):
         getattr(L, name).restype = i
     L.dfm_profile_enable.argtypes = [i]
     L.dfm_profile_enable.restype = i

@@ -21,7 +21,10 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libdfm_hip.so")
 
-SOURCES = ["pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip"]
+SOURCES = [
+    "pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip",
+    "integrators.hip",
+]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -36,9 +36,6 @@ int check_common(int D, int DV, int prec, double sigma, long nx, long ny) {
   return DFM_OK;
 }
 
-// AbstractKernel::SetKernelWidth clamps the width at 1e-10 (AbstractKernel.h:85-91).
-double clamp_width(double h) { return h < 1e-10 ? 1e-10 : h; }
-
 int need(const void *ptr, const char *name) {
   if (!ptr) return set_error(DFM_ERR_ARG, "%s is NULL", name);
   return DFM_OK;
@@ -48,21 +45,8 @@ int need(const void *ptr, const char *name) {
 int run(void *stream, int op, int D, int DV, double sigma, int prec, KParams p, long out0_n,
         long out1_n, void *work, size_t work_bytes) {
   clear_error();
-  hipStream_t st = (hipStream_t)stream;
-  if (p.nx == 0) return DFM_OK;
-  if (p.ny == 0) {  // empty sums
-    fill(st, p.out0, out0_n, 0.0);
-    fill(st, p.out1, out1_n, 0.0);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? DFM_OK
-                           : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
-  }
-  Plan pl = make_plan(op, prec, D, DV, p.nx, p.ny);
-  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
-    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
-                     work ? work_bytes : (size_t)0);
-  p.sigma = clamp_width(sigma);
-  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
+  return pairsum((hipStream_t)stream, op, prec, D, DV, sigma, p, out0_n, out1_n, work,
+                 work_bytes);
 }
 
 KParams blank() {
@@ -168,8 +152,8 @@ int dfm_profile_read(double *total_ms, long *launches) {
 
 size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision) {
   if (check_common(dimpoint, dimvect, precision, 1.0, nx, ny) != DFM_OK) return 0;
-  if (op < DFM_OP_CONV || op > DFM_OP_GRADDIFF) return 0;
-  if (op == DFM_OP_GRADDIFF) ny = nx;
+  if (op < DFM_OP_CONV || op > DFM_OP_SHOOT_VJP) return 0;
+  if (op == DFM_OP_GRADDIFF || op == DFM_OP_SHOOT_VJP) ny = nx;
   if (nx == 0 || ny == 0) return 0;
   return make_plan(op, precision, dimpoint, dimvect, nx, ny).work_bytes;
 }

@@ -14,7 +14,8 @@ enum : int {
   OP_CONVGRAD1 = 2,
   OP_VARLIN = 3,
   OP_GRADDIM = 4,
-  OP_GRADDIFF = 5
+  OP_GRADDIFF = 5,
+  OP_SHOOTVJP = 6
 };
 
 constexpr int kBBoxParts = 256;
@@ -23,7 +24,7 @@ constexpr int kBBoxParts = 256;
 // into the FP64 accumulators.
 constexpr int kTileF64 = 128;
 constexpr int kTileF32 = 256;
-constexpr int kFoldF32 = 16;
+constexpr int kFoldF32 = 32;
 
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
 // (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
@@ -35,10 +36,12 @@ enum : int { MODE_ADAPTIVE = 0, MODE_EXACT = 1 };
 struct KParams {
   const double *x;   // targets (nx x D)
   const double *ta;  // per-target extra: alpha (nx x DV) or beta (GRADDIFF)
-  const double *tb;  // per-target extra: eta (GRADDIFF)
+  const double *tb;  // per-target extra: eta (GRADDIFF), mu_p (SHOOTVJP)
+  const double *tc;  // per-target extra: mu_a (SHOOTVJP)
   const double *y;   // sources (ny x D)
   const double *w;   // weights (ny x DV)
-  const double *se;  // per-source extra: eta_j (GRADDIFF)
+  const double *se;  // per-source extra: eta_j (GRADDIFF), mu_p_j (SHOOTVJP)
+  const double *se2; // per-source extra: mu_a_j (SHOOTVJP)
   const uint32_t *perm;
   double *partial;
   double *out0;
@@ -74,6 +77,12 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
                    char *work);
 int fill(hipStream_t st, double *dst, long n, double v);
+
+// One pair-sum call on device pointers: plans the work, checks the workspace, clamps the
+// kernel width and launches (empty source sets give zero outputs).
+int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
+            long out0_n, long out1_n, void *work, size_t work_bytes);
+size_t pairsum_workspace(int op, int prec, int D, int DV, long nx, long ny);
 int copy_d2d(hipStream_t st, void *dst, const void *src, size_t bytes);
 
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)

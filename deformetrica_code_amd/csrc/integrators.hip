@@ -1,0 +1,495 @@
+// integrators.hip — device-resident geodesic shooting, landmark flow, and the discrete
+// adjoint (backward pass) of the shoot + flow scheme, recomputed from checkpoints.
+//
+// Forward (Diffeos.cxx:589-710), all state in HBM, every kernel sum one pair-sum launch:
+//   shoot  z = (p, a):  F(z) = (K(p,p) a, -g(p, a)),  g_i = sum_j (a_i.a_j) grad_1 K(p_i, p_j)
+//          Euler  z_{t+1} = z_t + dt F(z_t)                         (Diffeos::Shoot)
+//          Heun   z* = z_t + dt F(z_t); z_{t+1} = z_t + dt/2 (F(z_t) + F(z*))
+//          One F evaluation is one fused Convolve + ConvolveGradient(X, a) call with
+//          targets = sources = p.
+//   flow   v(x; z) = sum_j K(x, p_j) a_j;  Euler or Heun (improved Euler, the default)
+//          (Diffeos::FlowLandmarkPointsTrajectory)
+// Backward: the exact transpose of those discrete steps (not the reference's continuous
+// adjoint, AdjointEquationsIntegrator, which agrees with it to O(dt)):
+//   J_F(z)^T (mu_p, mu_a) = (GradSym(mu_p) + SpecialHessian(mu_a), Convolve(mu_p) - Diff(mu_a))
+//     one fused OP_SHOOTVJP pair sum
+//   J_x v(x; z)^T th = Grad1(targets x, alpha th; sources p, weights a)
+//   J_z v(x; z)^T th = (Grad1, Convolve)(targets p, alpha a; sources x, weights th)
+//     one fused OP_CONVGRAD1 pair sum
+// The forward stores (p, a, x) every k-th step; the backward recomputes each segment of
+// at most k steps from its checkpoint and walks it in reverse.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "dfm_hip.h"
+#include "dfm_internal.h"
+
+namespace dfm {
+namespace {
+
+// |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
+constexpr double kZeroMomenta2 = 1e-40;
+
+// out = ca a + cb b + cc c (b, c may be null).  With a gate (squared norm of the initial
+// momenta) below kZeroMomenta2, out = a, or 0 when gate_zero is set.
+__global__ __launch_bounds__(256) void k_lin(double *out, const double *a, double ca,
+                                             const double *b, double cb, const double *c,
+                                             double cc, long n, const double *gate,
+                                             int gate_zero) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (gate && *gate < kZeroMomenta2) {
+    out[i] = gate_zero ? 0.0 : a[i];
+    return;
+  }
+  double v = ca * a[i];
+  if (b) v += cb * b[i];
+  if (c) v += cc * c[i];
+  out[i] = v;
+}
+
+// *out = sum a_i^2 (one workgroup)
+__global__ __launch_bounds__(256) void k_sumsq(const double *a, long n, double *out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = threadIdx.x; i < n; i += 256) s += a[i] * a[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Bump allocator over the caller's workspace; a dry run (base == nullptr) only sizes it.
+struct Arena {
+  char *base;
+  size_t off = 0;
+  explicit Arena(void *b) : base((char *)b) {}
+  double *take(size_t count) {
+    double *p = base ? (double *)(base + off) : nullptr;
+    off += align256(count * sizeof(double));
+    return p;
+  }
+  void *take_bytes(size_t bytes) {
+    void *p = base ? (void *)(base + off) : nullptr;
+    off += align256(bytes);
+    return p;
+  }
+};
+
+struct Ctx {
+  hipStream_t st;
+  int D;
+  double sigma;
+  int prec;
+  void *pw;      // pair-sum workspace
+  size_t pw_bytes;
+};
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int lin(const Ctx &cx, double *out, const double *a, double ca, const double *b, double cb,
+        const double *c, double cc, long n, const double *gate = nullptr, int gate_zero = 0) {
+  if (n <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.st, out, a, ca, b,
+                     cb, c, cc, n, gate, gate_zero);
+  return check_launch("k_lin");
+}
+
+int copy(const Ctx &cx, double *out, const double *a, long n) {
+  return lin(cx, out, a, 1.0, nullptr, 0.0, nullptr, 0.0, n);
+}
+
+KParams blank() {
+  KParams p;
+  std::memset(&p, 0, sizeof(p));
+  return p;
+}
+
+// conv (nx x D) and/or grad (nx x D) of sum_j K(x_i, y_j) w_j with alpha (nx x D)
+int conv_grad1(const Ctx &cx, const double *x, const double *alpha, long nx, const double *y,
+               const double *w, long ny, double *conv, double *grad) {
+  KParams p = blank();
+  p.x = x; p.ta = alpha; p.y = y; p.w = w; p.nx = nx; p.ny = ny;
+  if (conv && grad) {
+    p.out0 = conv; p.out1 = grad;
+    return pairsum(cx.st, OP_CONVGRAD1, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, nx * cx.D,
+                   cx.pw, cx.pw_bytes);
+  }
+  if (conv) {
+    p.out0 = conv;
+    return pairsum(cx.st, OP_CONV, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, 0, cx.pw,
+                   cx.pw_bytes);
+  }
+  p.out0 = grad;
+  return pairsum(cx.st, OP_GRAD1, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, 0, cx.pw,
+                 cx.pw_bytes);
+}
+
+int shoot_vjp(const Ctx &cx, const double *p_, const double *a, const double *mp,
+              const double *ma, long n, double *jp, double *ja) {
+  KParams p = blank();
+  p.x = p_; p.ta = a; p.tb = mp; p.tc = ma;
+  p.y = p_; p.w = a; p.se = mp; p.se2 = ma;
+  p.out0 = jp; p.out1 = ja; p.nx = n; p.ny = n;
+  return pairsum(cx.st, OP_SHOOTVJP, cx.prec, cx.D, cx.D, cx.sigma, p, n * cx.D, n * cx.D, cx.pw,
+                 cx.pw_bytes);
+}
+
+size_t pair_ws(int prec, int D, long n, long m) {
+  size_t w = 0;
+  w = std::max(w, pairsum_workspace(OP_CONVGRAD1, prec, D, D, n, n));
+  w = std::max(w, pairsum_workspace(OP_SHOOTVJP, prec, D, D, n, n));
+  if (m > 0) {
+    w = std::max(w, pairsum_workspace(OP_CONV, prec, D, D, m, n));
+    w = std::max(w, pairsum_workspace(OP_GRAD1, prec, D, D, m, n));
+    w = std::max(w, pairsum_workspace(OP_CONVGRAD1, prec, D, D, n, m));
+  }
+  return w;
+}
+
+// ---- forward steps ------------------------------------------------------------------
+
+struct ShootTmp {
+  double *dpos, *dmom, *ps, *as, *dpos2, *dmom2;
+  void take(Arena &ar, long nd) {
+    dpos = ar.take(nd); dmom = ar.take(nd); ps = ar.take(nd); as = ar.take(nd);
+    dpos2 = ar.take(nd); dmom2 = ar.take(nd);
+  }
+};
+
+// (p, a) -> (p1, a1) by one Euler or Heun step.
+int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, long n, double dt,
+               double *p1, double *a1, const ShootTmp &t, const double *gate) {
+  const long nd = n * cx.D;
+  int rc = conv_grad1(cx, p, a, n, p, a, n, t.dpos, t.dmom);
+  if (rc) return rc;
+  if (scheme == DFM_EULER) {
+    if ((rc = lin(cx, p1, p, 1.0, t.dpos, dt, nullptr, 0.0, nd, gate, 0))) return rc;
+    return lin(cx, a1, a, 1.0, t.dmom, -dt, nullptr, 0.0, nd, gate, 1);
+  }
+  if ((rc = lin(cx, t.ps, p, 1.0, t.dpos, dt, nullptr, 0.0, nd))) return rc;
+  if ((rc = lin(cx, t.as, a, 1.0, t.dmom, -dt, nullptr, 0.0, nd))) return rc;
+  if ((rc = conv_grad1(cx, t.ps, t.as, n, t.ps, t.as, n, t.dpos2, t.dmom2))) return rc;
+  if ((rc = lin(cx, p1, p, 1.0, t.dpos, 0.5 * dt, t.dpos2, 0.5 * dt, nd, gate, 0))) return rc;
+  return lin(cx, a1, a, 1.0, t.dmom, -0.5 * dt, t.dmom2, -0.5 * dt, nd, gate, 1);
+}
+
+struct FlowTmp {
+  double *v1, *v2, *xs;
+  void take(Arena &ar, long md) { v1 = ar.take(md); v2 = ar.take(md); xs = ar.take(md); }
+};
+
+// x -> x1 in the flow from (p, a) to (p1, a1).
+int flow_step(const Ctx &cx, int improved, const double *p, const double *a, const double *p1,
+              const double *a1, long n, const double *x, long m, double dt, double *x1,
+              const FlowTmp &t, const double *gate) {
+  const long md = m * cx.D;
+  int rc = conv_grad1(cx, x, nullptr, m, p, a, n, t.v1, nullptr);
+  if (rc) return rc;
+  if (!improved) return lin(cx, x1, x, 1.0, t.v1, dt, nullptr, 0.0, md, gate, 0);
+  if ((rc = lin(cx, t.xs, x, 1.0, t.v1, dt, nullptr, 0.0, md))) return rc;
+  if ((rc = conv_grad1(cx, t.xs, nullptr, m, p1, a1, n, t.v2, nullptr))) return rc;
+  return lin(cx, x1, x, 1.0, t.v1, 0.5 * dt, t.v2, 0.5 * dt, md, gate, 0);
+}
+
+int norm2(const Ctx &cx, const double *a, long n, double *out) {
+  hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, cx.st, a, n, out);
+  return check_launch("k_sumsq");
+}
+
+int check_common(int D, int prec, double sigma, long n, long m, int T) {
+  if (D != 2 && D != 3) return set_error(DFM_ERR_DIM, "unsupported dimpoint %d (2 or 3)", D);
+  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+    return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
+  if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
+  if (n < 0 || m < 0 || n > 0x7fffffffL || m > 0x7fffffffL)
+    return set_error(DFM_ERR_ARG, "point counts must be in [0, 2^31)");
+  if (T < 1) return set_error(DFM_ERR_ARG, "number of time points must be >= 1");
+  return DFM_OK;
+}
+
+// ---- layouts (shared by the workspace queries and the drivers) ----------------------
+
+struct ShootLayout {
+  double *gate;
+  ShootTmp t;
+  void *pw;
+  size_t pw_bytes;
+  size_t total;
+  ShootLayout(void *base, int prec, int D, long n) {
+    Arena ar(base);
+    gate = ar.take(1);
+    t.take(ar, n * D);
+    pw_bytes = pair_ws(prec, D, n, 0);
+    pw = ar.take_bytes(pw_bytes);
+    total = ar.off;
+  }
+};
+
+struct FlowLayout {
+  double *gate;
+  FlowTmp t;
+  void *pw;
+  size_t pw_bytes;
+  size_t total;
+  FlowLayout(void *base, int prec, int D, long n, long m) {
+    Arena ar(base);
+    gate = ar.take(1);
+    t.take(ar, m * D);
+    pw_bytes = pair_ws(prec, D, n, m);
+    pw = ar.take_bytes(pw_bytes);
+    total = ar.off;
+  }
+};
+
+struct AdjointLayout {
+  int segs, k;
+  double *gate;
+  double *ckpt;   // segs x (2n + m) x D
+  double *seg;    // (k + 1) x (2n + m) x D
+  ShootTmp st;
+  FlowTmp ft;
+  double *lp, *la, *lpn, *lan, *lpf, *laf, *jp, *ja, *jsp, *jsa, *nup, *nua;  // n x D
+  double *th, *thn, *ths, *mu;                                               // m x D
+  void *pw;
+  size_t pw_bytes;
+  size_t total;
+  AdjointLayout(void *base, int prec, int D, long n, long m, int T, int kk) {
+    Arena ar(base);
+    k = std::max(1, kk);
+    segs = T > 1 ? (T - 1 + k - 1) / k : 0;
+    const long rec = (2 * n + m) * D;
+    gate = ar.take(1);
+    ckpt = ar.take((size_t)std::max(segs, 1) * rec);
+    seg = ar.take((size_t)(std::min(k, std::max(T - 1, 0)) + 1) * rec);
+    st.take(ar, n * D);
+    ft.take(ar, m * D);
+    double **vn[] = {&lp, &la, &lpn, &lan, &lpf, &laf, &jp, &ja, &jsp, &jsa, &nup, &nua};
+    for (double **v : vn) *v = ar.take(n * D);
+    double **vm[] = {&th, &thn, &ths, &mu};
+    for (double **v : vm) *v = ar.take(m * D);
+    pw_bytes = pair_ws(prec, D, n, m);
+    pw = ar.take_bytes(pw_bytes);
+    total = ar.off;
+  }
+};
+
+}  // namespace
+}  // namespace dfm
+
+using namespace dfm;
+
+extern "C" {
+
+size_t dfm_shoot_workspace(int dimpoint, long n, int precision) {
+  if (check_common(dimpoint, precision, 1.0, n, 0, 1)) return 0;
+  return ShootLayout(nullptr, precision, dimpoint, n).total;
+}
+
+int dfm_shoot(void *stream, int D, double sigma, int prec, int scheme, const double *p0,
+              const double *a0, long n, int T, double t0, double tN, double *P, double *A,
+              void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_common(D, prec, sigma, n, 0, T);
+  if (rc) return rc;
+  if (scheme != DFM_EULER && scheme != DFM_HEUN) return set_error(DFM_ERR_ARG, "unknown scheme %d", scheme);
+  if (n == 0) return DFM_OK;
+  if (!p0 || !a0 || !P || !A) return set_error(DFM_ERR_ARG, "NULL argument");
+  ShootLayout need(nullptr, prec, D, n);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  ShootLayout L(work, prec, D, n);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
+  const long nd = n * D;
+  if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, nd, L.gate)))
+    return rc;
+  const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
+  for (int t = 0; t + 1 < T; t++)
+    if ((rc = shoot_step(cx, scheme, P + t * nd, A + t * nd, n, dt, P + (t + 1) * nd,
+                         A + (t + 1) * nd, L.t, L.gate)))
+      return rc;
+  return DFM_OK;
+}
+
+size_t dfm_flow_workspace(int dimpoint, long n, long m, int precision) {
+  if (check_common(dimpoint, precision, 1.0, n, m, 1)) return 0;
+  return FlowLayout(nullptr, precision, dimpoint, n, m).total;
+}
+
+int dfm_flow(void *stream, int D, double sigma, int prec, int improved_euler, const double *P,
+             const double *A, long n, const double *x0, long m, int T, double t0, double tN,
+             double *X, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_common(D, prec, sigma, n, m, T);
+  if (rc) return rc;
+  if (m == 0) return DFM_OK;
+  if (!x0 || !X || (n > 0 && (!P || !A))) return set_error(DFM_ERR_ARG, "NULL argument");
+  FlowLayout need(nullptr, prec, D, n, m);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  FlowLayout L(work, prec, D, n, m);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
+  const long nd = n * D, md = m * D;
+  if ((rc = copy(cx, X, x0, md))) return rc;
+  if (n == 0) {
+    for (int t = 1; t < T; t++)
+      if ((rc = copy(cx, X + t * md, x0, md))) return rc;
+    return DFM_OK;
+  }
+  if ((rc = norm2(cx, A, nd, L.gate))) return rc;
+  const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
+  for (int t = 0; t + 1 < T; t++)
+    if ((rc = flow_step(cx, improved_euler, P + t * nd, A + t * nd, P + (t + 1) * nd,
+                        A + (t + 1) * nd, n, X + t * md, m, dt, X + (t + 1) * md, L.t, L.gate)))
+      return rc;
+  return DFM_OK;
+}
+
+size_t dfm_adjoint_workspace(int dimpoint, long n, long m, int T, int checkpoint_every,
+                             int precision) {
+  if (check_common(dimpoint, precision, 1.0, n, m, T) || checkpoint_every < 1) return 0;
+  return AdjointLayout(nullptr, precision, dimpoint, n, m, T, checkpoint_every).total;
+}
+
+int dfm_shoot_vjp(void *stream, int D, double sigma, int prec, const double *p, const double *a,
+                  const double *mu_p, const double *mu_a, long n, double *jp, double *ja,
+                  void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_common(D, prec, sigma, n, 0, 1);
+  if (rc) return rc;
+  if (n == 0) return DFM_OK;
+  if (!p || !a || !mu_p || !mu_a || !jp || !ja) return set_error(DFM_ERR_ARG, "NULL argument");
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, work, work_bytes};
+  return shoot_vjp(cx, p, a, mu_p, mu_a, n, jp, ja);
+}
+
+int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int scheme,
+                           int improved_euler, const double *p0, const double *a0, long n,
+                           const double *x0, long m, int T, double t0, double tN,
+                           int checkpoint_every, const double *theta_T, double *grad_p0,
+                           double *grad_a0, double *grad_x0, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_common(D, prec, sigma, n, m, T);
+  if (rc) return rc;
+  if (scheme != DFM_EULER && scheme != DFM_HEUN) return set_error(DFM_ERR_ARG, "unknown scheme %d", scheme);
+  if (checkpoint_every < 1) return set_error(DFM_ERR_ARG, "checkpoint interval must be >= 1");
+  if (n == 0) return DFM_OK;
+  if (!p0 || !a0 || !grad_p0 || !grad_a0 || (m > 0 && (!x0 || !theta_T)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  AdjointLayout need(nullptr, prec, D, n, m, T, checkpoint_every);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  AdjointLayout L(work, prec, D, n, m, T, checkpoint_every);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
+  const long nd = n * D, md = m * D, rec = 2 * nd + md;
+  const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
+  // state record layout: [p (nd) | a (nd) | x (md)]
+  auto P_ = [&](double *r) { return r; };
+  auto A_ = [&](double *r) { return r + nd; };
+  auto X_ = [&](double *r) { return r + 2 * nd; };
+
+  if ((rc = norm2(cx, a0, nd, L.gate))) return rc;
+  // terminal cotangents
+  if ((rc = lin(cx, L.lp, p0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
+  if ((rc = lin(cx, L.la, a0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
+  if (m > 0 && (rc = copy(cx, L.th, theta_T, md))) return rc;
+
+  // ---- forward: checkpoints at steps 0, k, 2k, ...
+  double *cur = L.seg, *nxt = L.seg + rec;
+  if ((rc = copy(cx, P_(cur), p0, nd)) || (rc = copy(cx, A_(cur), a0, nd))) return rc;
+  if (m > 0 && (rc = copy(cx, X_(cur), x0, md))) return rc;
+  for (int t = 0; t + 1 < T; t++) {
+    if (t % L.k == 0 && (rc = copy(cx, L.ckpt + (size_t)(t / L.k) * rec, cur, rec))) return rc;
+    if ((rc = shoot_step(cx, scheme, P_(cur), A_(cur), n, dt, P_(nxt), A_(nxt), L.st, L.gate)))
+      return rc;
+    if (m > 0 && (rc = flow_step(cx, improved_euler, P_(cur), A_(cur), P_(nxt), A_(nxt), n, X_(cur),
+                                 m, dt, X_(nxt), L.ft, L.gate)))
+      return rc;
+    std::swap(cur, nxt);
+  }
+
+  // ---- backward, one checkpoint segment at a time
+  for (int s = L.segs - 1; s >= 0; s--) {
+    const int tb = s * L.k, te = std::min(T - 1, tb + L.k);
+    double *S = L.seg;  // states tb..te
+    if ((rc = copy(cx, S, L.ckpt + (size_t)s * rec, rec))) return rc;
+    for (int t = tb; t < te; t++) {
+      double *a_ = S + (size_t)(t - tb) * rec, *b_ = S + (size_t)(t - tb + 1) * rec;
+      if ((rc = shoot_step(cx, scheme, P_(a_), A_(a_), n, dt, P_(b_), A_(b_), L.st, L.gate)))
+        return rc;
+      if (m > 0 && (rc = flow_step(cx, improved_euler, P_(a_), A_(a_), P_(b_), A_(b_), n, X_(a_), m,
+                                   dt, X_(b_), L.ft, L.gate)))
+        return rc;
+    }
+    for (int t = te - 1; t >= tb; t--) {
+      double *zt = S + (size_t)(t - tb) * rec, *z1 = S + (size_t)(t - tb + 1) * rec;
+      // flow step t: cotangent th of x_{t+1} -> thn of x_t, plus its pull onto z_t, z_{t+1}
+      if (m > 0) {
+        if ((rc = conv_grad1(cx, X_(zt), nullptr, m, P_(zt), A_(zt), n, L.ft.v1, nullptr))) return rc;
+        if (improved_euler) {
+          if ((rc = lin(cx, L.ft.xs, X_(zt), 1.0, L.ft.v1, dt, nullptr, 0.0, md))) return rc;
+          // ths = dt/2 J_x v(xs; z_{t+1})^T th
+          if ((rc = conv_grad1(cx, L.ft.xs, L.th, m, P_(z1), A_(z1), n, nullptr, L.ths))) return rc;
+          if ((rc = lin(cx, L.ths, L.ths, 0.5 * dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
+          // lambda(z_{t+1}) += dt/2 J_z v(xs; z_{t+1})^T th
+          if ((rc = conv_grad1(cx, P_(z1), A_(z1), n, L.ft.xs, L.th, m, L.laf, L.lpf))) return rc;
+          if ((rc = lin(cx, L.lp, L.lp, 1.0, L.lpf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
+          if ((rc = lin(cx, L.la, L.la, 1.0, L.laf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
+          // mu = dt/2 th + dt ths
+          if ((rc = lin(cx, L.mu, L.th, 0.5 * dt, L.ths, dt, nullptr, 0.0, md))) return rc;
+        } else {
+          if ((rc = lin(cx, L.mu, L.th, dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
+        }
+        // thn = th (+ ths) + J_x v(x_t; z_t)^T mu
+        if ((rc = conv_grad1(cx, X_(zt), L.mu, m, P_(zt), A_(zt), n, nullptr, L.thn))) return rc;
+        if ((rc = lin(cx, L.thn, L.thn, 1.0, L.th, 1.0, improved_euler ? L.ths : nullptr, 1.0, md)))
+          return rc;
+        // flow pull onto z_t: J_z v(x_t; z_t)^T mu
+        if ((rc = conv_grad1(cx, P_(zt), A_(zt), n, X_(zt), L.mu, m, L.laf, L.lpf))) return rc;
+      }
+      // shoot step t: cotangent (lp, la) of z_{t+1} -> (lpn, lan) of z_t
+      if (scheme == DFM_EULER) {
+        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.lp, L.la, n, L.jp, L.ja))) return rc;
+        if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jp, dt, nullptr, 0.0, nd))) return rc;
+        if ((rc = lin(cx, L.lan, L.la, 1.0, L.ja, dt, nullptr, 0.0, nd))) return rc;
+      } else {
+        // z* = z_t + dt F(z_t)
+        if ((rc = conv_grad1(cx, P_(zt), A_(zt), n, P_(zt), A_(zt), n, L.st.dpos, L.st.dmom))) return rc;
+        if ((rc = lin(cx, L.st.ps, P_(zt), 1.0, L.st.dpos, dt, nullptr, 0.0, nd))) return rc;
+        if ((rc = lin(cx, L.st.as, A_(zt), 1.0, L.st.dmom, -dt, nullptr, 0.0, nd))) return rc;
+        // l* = dt/2 J_F(z*)^T l
+        if ((rc = shoot_vjp(cx, L.st.ps, L.st.as, L.lp, L.la, n, L.jsp, L.jsa))) return rc;
+        // nu = dt/2 l + dt l* = dt/2 l + dt^2/2 J_F(z*)^T l
+        if ((rc = lin(cx, L.nup, L.lp, 0.5 * dt, L.jsp, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
+        if ((rc = lin(cx, L.nua, L.la, 0.5 * dt, L.jsa, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
+        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.nup, L.nua, n, L.jp, L.ja))) return rc;
+        // lpn = l + l* + J_F(z_t)^T nu
+        if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jsp, 0.5 * dt, L.jp, 1.0, nd))) return rc;
+        if ((rc = lin(cx, L.lan, L.la, 1.0, L.jsa, 0.5 * dt, L.ja, 1.0, nd))) return rc;
+      }
+      if (m > 0) {
+        if ((rc = lin(cx, L.lp, L.lpn, 1.0, L.lpf, 1.0, nullptr, 0.0, nd))) return rc;
+        if ((rc = lin(cx, L.la, L.lan, 1.0, L.laf, 1.0, nullptr, 0.0, nd))) return rc;
+        std::swap(L.th, L.thn);
+      } else {
+        std::swap(L.lp, L.lpn);
+        std::swap(L.la, L.lan);
+      }
+    }
+  }
+  if ((rc = copy(cx, grad_p0, L.lp, nd)) || (rc = copy(cx, grad_a0, L.la, nd))) return rc;
+  if (grad_x0 && m > 0 && (rc = copy(cx, grad_x0, L.th, md))) return rc;
+  return DFM_OK;
+}
+
+}  // extern "C"

@@ -174,6 +174,7 @@ static int targets_per_lane(int op, int prec, int D, int DV) {
   switch (op) {
     case OP_VARLIN: return DV * D > 9 ? 1 : 2;
     case OP_GRADDIFF: return 2;
+    case OP_SHOOTVJP: return 1;
     default: return 4;
   }
 }
@@ -187,7 +188,8 @@ static int nacc_of(int op, int D, int DV) {
     case OP_CONVGRAD1: return DV + D;
     case OP_VARLIN: return DV * D;
     case OP_GRADDIM: return DV;
-    default: return 2 * D;
+    case OP_GRADDIFF: return 2 * D;
+    default: return 5 * D;
   }
 }
 
@@ -314,6 +316,31 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
   return DFM_OK;
+}
+
+// AbstractKernel::SetKernelWidth clamps the width at 1e-10 (AbstractKernel.h:85-91).
+static double clamp_width(double h) { return h < 1e-10 ? 1e-10 : h; }
+
+size_t pairsum_workspace(int op, int prec, int D, int DV, long nx, long ny) {
+  if (nx <= 0 || ny <= 0) return 0;
+  return make_plan(op, prec, D, DV, nx, ny).work_bytes;
+}
+
+int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
+            long out0_n, long out1_n, void *work, size_t work_bytes) {
+  if (p.nx == 0) return DFM_OK;
+  if (p.ny == 0) {  // empty sums
+    fill(st, p.out0, out0_n, 0.0);
+    fill(st, p.out1, out1_n, 0.0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
+  }
+  Plan pl = make_plan(op, prec, D, DV, p.nx, p.ny);
+  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
+                     work ? work_bytes : (size_t)0);
+  p.sigma = clamp_width(sigma);
+  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
 }
 
 }  // namespace dfm

@@ -12,6 +12,7 @@
 //   VARLIN     sum_j w_j[k] d/dx_l K                         ExactKernel.cxx:166-202
 //   GRADDIM    sum_j w_j d/dx_dim K                          ExactKernel.cxx:294-323
 //   GRADDIFF   sum_j (b_i.b_j) H_x K (eta_j - eta_i)         AbstractKernel.cxx:22-88
+//   SHOOTVJP   transpose Jacobian of the shooting field (integrators.hip), fused
 //
 // FP64 path: coordinates scaled by 1/sigma, K = exp(-r^2) in FP64 — the arithmetic of the
 // reference's ExactKernel.
@@ -20,9 +21,9 @@
 // and every staged source on the centre of its targets' bounding box in FP64, scales by
 // sqrt(log2 e)/sigma and only then rounds to FP32, so the FP32 differences carry the
 // rounding of the small centred coordinates rather than of the absolute ones.
-// K = exp2(-r'^2) on the transcendental unit (v_exp_f32).  Each lane accumulates 16
+// K = exp2(-r'^2) on the transcendental unit (v_exp_f32).  Each lane accumulates 32
 // sources in FP32 and folds that partial sum into FP64 accumulators (cascaded,
-// compensated summation: the FP32 rounding is bounded by a 16-term partial sum, never by
+// compensated summation: the FP32 rounding is bounded by a 32-term partial sum, never by
 // the running total).  A wavefront whose targets span more than kExactExtent scaled
 // units (sparse targets, narrow kernels) instead forms each difference from FP32 hi and
 // lo parts, (xh - yh) + (xl - yl); FP32X takes that path in every wavefront.
@@ -51,8 +52,8 @@ template <typename T>
 struct Prec;
 
 // kFold: sources summed in the working precision before the partial sum is folded into
-// the FP64 accumulator.  In FP32 a 16-source partial keeps the rounding of each add
-// proportional to 16 terms even when a wide kernel makes every pair significant and the
+// the FP64 accumulator.  In FP32 a 32-source partial keeps the rounding of each add
+// proportional to 32 terms even when a wide kernel makes every pair significant and the
 // sum cancels.
 template <>
 struct Prec<float> {
@@ -75,12 +76,15 @@ struct Prec<double> {
 
 template <int OP, int D, int DV>
 struct Shape {
-  // per-target extra inputs: alpha (GRAD1, CONVGRAD1) or beta + eta (GRADDIFF)
+  // per-target extra inputs: alpha (GRAD1, CONVGRAD1), beta + eta (GRADDIFF) or
+  // alpha + mu_p + mu_a (SHOOTVJP)
   static constexpr int NTE = (OP == OP_GRAD1 || OP == OP_CONVGRAD1) ? DV
                              : (OP == OP_GRADDIFF)                  ? DV + D
+                             : (OP == OP_SHOOTVJP)                  ? DV + 2 * D
                                                                     : 0;
-  // per-source record: y, w (+ eta_j for GRADDIFF)
-  static constexpr int NSRC = D + DV + (OP == OP_GRADDIFF ? D : 0);
+  // per-source record: y, w (+ eta_j for GRADDIFF, + mu_p_j, mu_a_j for SHOOTVJP)
+  static constexpr int NSRC =
+      D + DV + (OP == OP_GRADDIFF ? D : 0) + (OP == OP_SHOOTVJP ? 2 * D : 0);
   static constexpr int NSP = (NSRC + 3) / 4 * 4;
   // with the FP32 residuals of the source coordinates appended (exact-difference path)
   static constexpr int NSPX = (NSRC + D + 3) / 4 * 4;
@@ -89,7 +93,8 @@ struct Shape {
                               : OP == OP_CONVGRAD1 ? DV + D
                               : OP == OP_VARLIN    ? DV * D
                               : OP == OP_GRADDIM   ? DV
-                                                   : 2 * D;
+                              : OP == OP_GRADDIFF  ? 2 * D
+                                                   : 5 * D;
 };
 
 // ------------------------------------------------------------------------------------
@@ -144,6 +149,40 @@ __device__ __forceinline__ void pair_update(const T *xi, const T *xl, const T *t
     const T t = K * d[AUX];
 #pragma unroll
     for (int k = 0; k < DV; k++) acc[k] += t * w[k];
+  } else if constexpr (OP == OP_SHOOTVJP) {
+    // transpose of the shooting field's Jacobian at (p, a) applied to (mu_p, mu_a):
+    //   conv:  K mu_p_j                                   -> Convolve(mu_p)
+    //   gsym:  K (mu_p_i . a_j + a_i . mu_p_j) d          -> GradSym(mu_p)
+    //   diff:  K ((mu_a_i - mu_a_j) . d) a_j               -> Diff(mu_a)
+    //   sh1/2: K (a_i . a_j) e,  K (a_i . a_j)(d . e) d   -> SpecialHessian(mu_a)
+    const T *a_i = te;
+    const T *mp_i = te + DV;
+    const T *ma_i = te + DV + D;
+    const T *mp_j = rec + D + DV;
+    const T *ma_j = rec + D + DV + D;
+    T s1 = mp_i[0] * w[0] + a_i[0] * mp_j[0];
+    T bb = a_i[0] * w[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) {
+      s1 += mp_i[k] * w[k] + a_i[k] * mp_j[k];
+      bb += a_i[k] * w[k];
+    }
+    T e[D];
+    T ed = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      e[k] = ma_i[k] - ma_j[k];
+      ed += e[k] * d[k];
+    }
+    const T t1 = K * s1, t2 = K * ed, t3 = K * bb, t4 = t3 * ed;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      acc[k] += K * mp_j[k];
+      acc[D + k] += t1 * d[k];
+      acc[2 * D + k] += t2 * w[k];
+      acc[3 * D + k] += t3 * e[k];
+      acc[4 * D + k] += t4 * d[k];
+    }
   } else {  // OP_GRADDIFF
     T bb = te[0] * w[0];
 #pragma unroll
@@ -191,9 +230,15 @@ __device__ __forceinline__ void finalize(const double *a, const KParams &p, long
   } else if constexpr (OP == OP_GRADDIM) {
 #pragma unroll
     for (int k = 0; k < DV; k++) p.out0[o * DV + k] = p.cg * a[k];
-  } else {
+  } else if constexpr (OP == OP_GRADDIFF) {
 #pragma unroll
     for (int k = 0; k < D; k++) p.out0[o * D + k] = p.ch1 * a[k] + p.ch2 * a[D + k];
+  } else {  // OP_SHOOTVJP: out0 = J_p, out1 = J_a
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      p.out0[o * D + k] = p.cg * a[D + k] + p.ch1 * a[3 * D + k] + p.ch2 * a[4 * D + k];
+      p.out1[o * D + k] = a[k] - p.cg * a[2 * D + k];
+    }
   }
 }
 
@@ -249,9 +294,13 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
         }
 #pragma unroll
         for (int k = 0; k < DV; k++) rec[D + k] = (T)p.w[j * DV + k];
-        if constexpr (OP == OP_GRADDIFF) {
+        if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
 #pragma unroll
           for (int k = 0; k < D; k++) rec[D + DV + k] = (T)p.se[j * D + k];
+        }
+        if constexpr (OP == OP_SHOOTVJP) {
+#pragma unroll
+          for (int k = 0; k < D; k++) rec[D + DV + D + k] = (T)p.se2[j * D + k];
         }
       } else {
 #pragma unroll
@@ -358,11 +407,15 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
     if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
 #pragma unroll
       for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
-    } else if constexpr (OP == OP_GRADDIFF) {
+    } else if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
 #pragma unroll
       for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
 #pragma unroll
       for (int k = 0; k < D; k++) te[r][DV + k] = valid[r] ? (T)p.tb[orig[r] * D + k] : T(0);
+      if constexpr (OP == OP_SHOOTVJP) {
+#pragma unroll
+        for (int k = 0; k < D; k++) te[r][DV + D + k] = valid[r] ? (T)p.tc[orig[r] * D + k] : T(0);
+      }
     } else {
       te[r][0] = T(0);
     }
@@ -453,8 +506,8 @@ static int launch_aux(hipStream_t st, const KParams &p, const Plan &pl) {
 
 template <typename T, int MODE, int OP, int D>
 static int launch_dv(hipStream_t st, int DV, const KParams &p, const Plan &pl) {
-  if constexpr (OP == OP_GRADDIFF) {
-    if (DV != D) return set_error(DFM_ERR_DIM, "GradDiff needs dimvect == dimpoint");
+  if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
+    if (DV != D) return set_error(DFM_ERR_DIM, "self-interaction ops need dimvect == dimpoint");
     return launch_aux<T, MODE, OP, D, D>(st, p, pl);
   } else {
     switch (DV) {
@@ -485,6 +538,7 @@ static int launch_op(hipStream_t st, int op, int D, int DV, const KParams &p, co
     case OP_VARLIN: return launch_d<T, MODE, OP_VARLIN>(st, D, DV, p, pl);
     case OP_GRADDIM: return launch_d<T, MODE, OP_GRADDIM>(st, D, DV, p, pl);
     case OP_GRADDIFF: return launch_d<T, MODE, OP_GRADDIFF>(st, D, DV, p, pl);
+    case OP_SHOOTVJP: return launch_d<T, MODE, OP_SHOOTVJP>(st, D, DV, p, pl);
   }
   return set_error(DFM_ERR_ARG, "unknown op %d", op);
 }

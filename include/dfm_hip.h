@@ -19,7 +19,7 @@
  *   DFM_FP64   plain FP64 arithmetic and FP64 exp — the reference's arithmetic.
  *   DFM_FP32C  FP32 exp and multiply-add over LDS-staged source tiles, with targets
  *              spatially sorted and re-centred in FP64 per wavefront, FP32 partial sums
- *              of 16 sources folded into FP64 accumulators (compensated, cascaded
+ *              of 32 sources folded into FP64 accumulators (compensated, cascaded
  *              summation).  A wavefront whose targets span more than 4 scaled kernel
  *              widths forms its coordinate differences exactly (FP32 hi + lo parts).
  *   DFM_FP32X  as DFM_FP32C with exact coordinate differences in every wavefront.
@@ -142,7 +142,8 @@ enum dfm_op {
   DFM_OP_CONV_GRAD1 = 2,
   DFM_OP_VARLIN = 3,
   DFM_OP_GRADDIM = 4,
-  DFM_OP_GRADDIFF = 5
+  DFM_OP_GRADDIFF = 5,
+  DFM_OP_SHOOT_VJP = 6
 };
 
 /* Bytes of workspace the device entry point for `op` needs at these sizes. */
@@ -172,6 +173,56 @@ int dfm_graddim(void *stream, int dimpoint, int dimvect, double sigma, int preci
 int dfm_graddiff(void *stream, int dimpoint, int dimvect, double sigma, int precision,
                  const double *x, const double *beta, const double *eta, long n, double *gamma,
                  void *work, size_t work_bytes);
+
+/* ------------------------------------------------------------------------------------ */
+/* Device-resident integrators (device pointers, stream-ordered, caller workspace).     */
+
+enum dfm_scheme { DFM_EULER = 0, DFM_HEUN = 1 };
+
+/* Geodesic shooting of n control points: P, A (T x n x d) receive the trajectory,
+ * P[0] = p0, A[0] = a0, dt = (tN - t0) / (T - 1).  DFM_EULER is Diffeos::Shoot
+ * (Diffeos.cxx:589-659): p += dt K(p,p) a, a -= dt G^T a, one fused
+ * Convolve + ConvolveGradient per step; DFM_HEUN is the Heun variant the reference keeps
+ * commented out (Diffeos.cxx:634-652).  |a0|_F < 1e-20 leaves the points in place
+ * (Diffeos.cxx:609-611). */
+size_t dfm_shoot_workspace(int dimpoint, long n, int precision);
+int dfm_shoot(void *stream, int dimpoint, double sigma, int precision, int scheme,
+              const double *p0, const double *a0, long n, int T, double t0, double tN,
+              double *P, double *A, void *work, size_t work_bytes);
+
+/* Flow of m landmark points in the deformation (P, A) (T x n x d): X (T x m x d), X[0] = x0.
+ * Diffeos::FlowLandmarkPointsTrajectory (Diffeos.cxx:661-710); improved_euler != 0 is the
+ * reference's default Heun scheme. */
+size_t dfm_flow_workspace(int dimpoint, long n, long m, int precision);
+int dfm_flow(void *stream, int dimpoint, double sigma, int precision, int improved_euler,
+             const double *P, const double *A, long n, const double *x0, long m, int T,
+             double t0, double tN, double *X, void *work, size_t work_bytes);
+
+/* Transpose of the shooting field's Jacobian, F(p, a) = (K(p,p) a, -G(p,a)^T a), at (p, a)
+ * applied to (mu_p, mu_a):  jp = GradSym(mu_p) + SpecialHessian(mu_a),
+ * ja = Convolve(mu_p) - Diff(mu_a), one fused pair sum.  No single reference counterpart:
+ * the reference's adjoint assembles the same kernel sums from Convolve, ConvolveGradient
+ * and ConvolveSpecialHessian calls (AdjointEquationsIntegrator.cxx:476-567).  Workspace
+ * from dfm_workspace_size(DFM_OP_SHOOT_VJP, ...). */
+int dfm_shoot_vjp(void *stream, int dimpoint, double sigma, int precision, const double *p,
+                  const double *a, const double *mu_p, const double *mu_a, long n, double *jp,
+                  double *ja, void *work, size_t work_bytes);
+
+/* Gradient of a terminal cost E(x_{T-1}) with respect to the initial control points p0,
+ * momenta a0 and landmarks x0, given theta_T = dE/dx_{T-1} (m x d): the exact discrete
+ * adjoint (backward pass) of dfm_shoot + dfm_flow with the same scheme, precision and
+ * time grid.  The forward pass is run once, storing (p, a, x) every checkpoint_every
+ * steps; the backward pass recomputes each segment from its checkpoint.  Replaces
+ * Diffeos::IntegrateAdjointEquations / AdjointEquationsIntegrator::Update
+ * (Diffeos.cxx:1390-1443, AdjointEquationsIntegrator.cxx:33-136, 414-567), whose continuous
+ * adjoint agrees with this gradient to O(dt).  grad_x0 may be NULL. */
+size_t dfm_adjoint_workspace(int dimpoint, long n, long m, int T, int checkpoint_every,
+                             int precision);
+int dfm_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precision, int scheme,
+                           int improved_euler, const double *p0, const double *a0, long n,
+                           const double *x0, long m, int T, double t0, double tN,
+                           int checkpoint_every, const double *theta_T, double *grad_p0,
+                           double *grad_a0, double *grad_x0, void *work, size_t work_bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,236 @@
+"""ORACLE — test infrastructure only.
+
+FP64 CPU restatement of Deformetrica's deformation integrators that call the kernel-sum
+engine, on top of the oracle ExactKernel (oracle/exact_kernel.c):
+
+  shoot            Diffeos::Shoot — explicit Euler (Diffeos.cxx:589-659), with the
+                   commented-out Heun variant (Diffeos.cxx:634-652) as scheme="heun"
+  flow             Diffeos::FlowLandmarkPointsTrajectory (Diffeos.cxx:661-710)
+  parallel_transport  Diffeos::ParallelTransport (Diffeos.cxx:293-582)
+  adjoint_reference   AdjointEquationsIntegrator (AdjointEquationsIntegrator.cxx:33-136,
+                   414-567): the reference's continuous adjoint, discretised with Heun
+
+Pinned by tests/test_deformations.py: the reference's own parallel-transport golden
+vectors (tests/unit_tests/parallel-transport/data, GroundTruthMom.txt at 1e-4), the
+Hamiltonian's conservation to O(dt), and central differences of the shooting objective
+against the reference adjoint (to O(dt), the order at which a continuous adjoint
+discretised independently of the forward scheme agrees with the discrete gradient).
+Matrices are numpy float64 (rows x cols).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .oracle import ExactKernel
+
+
+def _kernel(dim, width, Y, W):
+    k = ExactKernel(dim, width)
+    k.SetSources(Y)
+    k.SetWeights(W)
+    return k
+
+
+def time_step(T, t0=0.0, tN=1.0):
+    return (tN - t0) / (T - 1)
+
+
+def shoot(p0, a0, width, T, t0=0.0, tN=1.0, scheme="euler"):
+    """Control-point and momenta trajectories, lists of T (n x d) arrays."""
+    p0 = np.asarray(p0, dtype=np.float64)
+    a0 = np.asarray(a0, dtype=np.float64)
+    n, d = p0.shape
+    P = [p0.copy() for _ in range(T)]
+    A = [np.zeros((n, d)) for _ in range(T)]
+    A[0] = a0.copy()
+    if np.linalg.norm(a0) < 1e-20:  # Diffeos.cxx:609-611
+        return P, A
+    dt = time_step(T, t0, tN)
+
+    def field(p, a):
+        k = _kernel(d, width, p, a)
+        G = k.ConvolveGradient(p)           # (n, d, d) varlin, Diffeos.cxx:624
+        dpos = k.Convolve(p)                # Diffeos.cxx:626
+        dmom = np.einsum("ikl,ik->il", G, a)  # G_i^T a_i, Diffeos.cxx:628-629
+        return dpos, dmom
+
+    for t in range(T - 1):
+        dpos, dmom = field(P[t], A[t])
+        P[t + 1] = P[t] + dpos * dt
+        A[t + 1] = A[t] - dmom * dt
+        if scheme == "heun":
+            dpos2, dmom2 = field(P[t + 1], A[t + 1])
+            P[t + 1] = P[t] + (dpos + dpos2) * (dt * 0.5)
+            A[t + 1] = A[t] - (dmom + dmom2) * (dt * 0.5)
+    return P, A
+
+
+def flow(P, A, x0, width, T, t0=0.0, tN=1.0, improved_euler=True):
+    """Landmark trajectory (list of T (m x d) arrays) in the flow of (P, A)."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    d = x0.shape[1]
+    X = [x0.copy() for _ in range(T)]
+    if np.linalg.norm(A[0]) < 1e-20:  # Diffeos.cxx:684-685
+        return X
+    dt = time_step(T, t0, tN)
+    for t in range(T - 1):
+        v = _kernel(d, width, P[t], A[t]).Convolve(X[t])
+        X[t + 1] = X[t] + v * dt
+        if improved_euler:
+            v2 = _kernel(d, width, P[t + 1], A[t + 1]).Convolve(X[t + 1])
+            X[t + 1] = X[t] + (v + v2) * (dt * 0.5)
+    return X
+
+
+def parallel_transport(P, A, width, T, initial_momenta, initial_cp, initial_time, target_times,
+                       t0=0.0, tN=1.0):
+    """Diffeos::ParallelTransport (Diffeos.cxx:293-582): momenta transported along the
+    geodesic (P, A) at the target times."""
+    d = P[0].shape[1]
+    nb = len(target_times)
+    if (np.sum(A[0] ** 2) < 1e-20 or np.sum(initial_momenta ** 2) < 1e-20 or nb == 0 or T <= 1):
+        return [initial_momenta.copy() for _ in range(nb)]
+    h = (tN - t0) / (T - 1)
+    eps = h
+
+    def to_index(t):
+        c = (t - t0) / h
+        i = int(c)
+        return i + 1 if (c - i) >= 0.5 else i
+
+    initial_index = to_index(initial_time)
+    target_indices = [to_index(t) for t in target_times]
+    final_index = target_indices[-1]
+    n_steps = final_index - initial_index + 1
+    if final_index == initial_index:
+        return [initial_momenta.copy() for _ in range(nb)]
+
+    def conv(Y, W, X):
+        return _kernel(d, width, Y, W).Convolve(X)
+
+    def dot(a, b):
+        return float(np.sum(a * b))
+
+    initial_velocity = conv(initial_cp, initial_momenta, P[0])
+    velocities = [initial_velocity.copy() for _ in range(n_steps)]
+    transport = [None] * n_steps
+    init_wv = init_w2 = 0.0
+    wv = dot(initial_momenta, conv(P[initial_index], A[initial_index], initial_cp))
+    w2 = dot(initial_momenta, conv(initial_cp, initial_momenta, initial_cp))
+    v2 = 0.0
+    alpha, beta = 1.0, 0.0
+    index = 0
+    for t in range(initial_index, final_index):
+        k = ExactKernel(d, width)
+        k.SetSources(P[t])
+        k.SetWeights(A[t])
+        dpos = k.Convolve(P[t])
+        kinv = k.ConvolveInverse(velocities[index])
+        if t == initial_index:
+            init_wv = dot(kinv, k.Convolve(P[t]))
+            init_w2 = dot(kinv, conv(P[t], kinv, P[t]))
+        if t > initial_index:
+            q = conv(P[t], kinv, P[t])
+            prop_n2 = dot(kinv, q)
+            prop_wv = dot(A[t], q)
+            alpha = np.sqrt((init_w2 * v2 - init_wv * wv) / (prop_n2 * v2 - prop_wv * prop_wv))
+            beta = (init_wv - alpha * prop_wv) / v2
+        transport[index] = alpha * kinv + beta * A[t]
+        q = conv(P[t], transport[index], P[t])
+        wv = dot(A[t], q)
+        w2 = dot(kinv, q)
+        v2 = dot(A[t], conv(P[t], A[t], P[t]))
+        mom_pos = A[t] + eps * kinv
+        mom_neg = A[t] - eps * kinv
+
+        def dmom(mom):
+            G = _kernel(d, width, P[t], mom).ConvolveGradient(P[t])
+            return np.einsum("ikl,ik->il", G, mom)
+
+        dm_pos, dm_neg = dmom(mom_pos), dmom(mom_neg)
+        cp1 = P[t] + h / 2 * (dpos + eps * velocities[index])
+        mom1 = mom_pos - h / 2 * dm_pos
+        cp2 = P[t] + h * conv(cp1, mom1, cp1)
+        plus = cp2 * (1 / (2 * eps))
+        cp1 = P[t] + h / 2 * (dpos - eps * velocities[index])
+        mom1 = mom_neg - h / 2 * dm_neg
+        cp2 = P[t] + h * conv(cp1, mom1, cp1)
+        minus = cp2 * (1 / (2 * eps))
+        velocities[index + 1] = (plus - minus) / eps
+        index += 1
+    k = ExactKernel(d, width)
+    k.SetSources(P[final_index - 1])
+    k.SetWeights(A[final_index - 1])
+    transport[index] = k.ConvolveInverse(velocities[index])
+    out = []
+    oi = 0
+    for t in range(initial_index, final_index + 1):
+        if oi < nb and t == target_indices[oi]:
+            out.append(transport[t])
+            oi += 1
+    return out
+
+
+def hamiltonian(p, a, width):
+    """H = 1/2 a^T K(p, p) a, conserved along the geodesic."""
+    return 0.5 * float(np.sum(a * _kernel(p.shape[1], width, p, a).Convolve(p)))
+
+
+def adjoint_reference(P, A, X, theta_T, width, T, t0=0.0, tN=1.0, improved_euler=True):
+    """AdjointEquationsIntegrator with landmark points only (AEI.cxx:59-136, 414-567):
+    returns (xi_pos_0, xi_mom_0), the reference's gradient of E(x_T) w.r.t. the initial
+    control points and momenta for dE/dx_T = theta_T."""
+    d = P[0].shape[1]
+    n = P[0].shape[0]
+    dt = time_step(T, t0, tN)
+    theta = [None] * T
+    theta[T - 1] = np.asarray(theta_T, dtype=np.float64).copy()
+    for t in range(T - 1, 0, -1):  # AEI.cxx:96-133
+        dth = _kernel(d, width, P[t], A[t]).ConvolveGradient(X[t], theta[t])
+        theta[t - 1] = theta[t] + dth * dt
+        if improved_euler:
+            dth2 = _kernel(d, width, P[t - 1], A[t - 1]).ConvolveGradient(X[t - 1], theta[t - 1])
+            theta[t - 1] = theta[t] + (dth + dth2) * (dt * 0.5)
+    xp = [np.zeros((n, d)) for _ in range(T)]
+    xm = [np.zeros((n, d)) for _ in range(T)]
+
+    def update(s):  # ComputeUpdateAt, AEI.cxx:476-567
+        eta = _kernel(d, width, X[s], theta[s])
+        dxi1 = eta.ConvolveGradient(P[s], A[s])
+        dxi2 = eta.Convolve(P[s])
+        axp = np.concatenate([A[s], xp[s]], axis=1)
+        k = _kernel(d, width, P[s], axp)
+        kaxp = k.Convolve(P[s])
+        g = k.ConvolveGradient(P[s])  # (n, 2d, d)
+        dxi3 = np.einsum("ikl,ik->il", g[:, :d, :], xp[s]) + np.einsum("ikl,ik->il", g[:, d:, :], A[s])
+        dxi5 = kaxp[:, d:]
+        dxi6 = np.zeros((n, d))
+        for dim in range(d):
+            Wd = xm[s][:, dim:dim + 1] * A[s]
+            dxi6 += _kernel(d, width, P[s], Wd).ConvolveGradient(P[s], dim)
+        dxi6 -= np.einsum("ikl,il->ik", g[:, :d, :], xm[s])
+        dxi4 = _kernel(d, width, P[s], A[s]).ConvolveSpecialHessian(xm[s])
+        return dxi1 + dxi3 + dxi4, dxi2 + dxi5 + dxi6
+
+    for t in range(T - 1, 0, -1):  # AEI.cxx:438-462
+        dpos, dmom = update(t)
+        xp[t - 1] = xp[t] + dpos * dt
+        xm[t - 1] = xm[t] + dmom * dt
+        if improved_euler:
+            dpos2, dmom2 = update(t - 1)
+            xp[t - 1] = xp[t] + (dpos + dpos2) * (dt * 0.5)
+            xm[t - 1] = xm[t] + (dmom + dmom2) * (dt * 0.5)
+    return xp[0], xm[0], theta[0]
+
+
+def read_dlm(path):
+    """Deformetrica's DLM text matrices (src/io/MatrixDLM.cxx): whitespace-separated rows;
+    the multiple-matrix form starts with "count rows cols"."""
+    rows = [line.split() for line in open(path) if line.strip()]
+    first = rows[0]
+    if len(first) == 3 and len(rows) > 1 and len(rows[1]) != 3 and float(first[0]).is_integer():
+        cnt, r, c = (int(float(v)) for v in first)
+        data = np.array([[float(v) for v in row] for row in rows[1:1 + r]])
+        assert data.shape == (r, c)
+        return data
+    return np.array([[float(v) for v in row] for row in rows])

@@ -1,0 +1,231 @@
+"""GPU integrators: device-resident shooting and flow against the oracle restatement of
+Diffeos (oracle/deformations.py), the reference's golden parallel-transport vectors, and
+the hand-written discrete adjoint against central finite differences (through a
+torch.autograd.Function used only here) and against the reference's continuous adjoint."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "parallel_transport")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _setup(seed, n, m, d, scale=0.1):
+    rng = np.random.default_rng(seed)
+    p0 = rng.uniform(size=(n, d))
+    a0 = rng.normal(scale=scale, size=(n, d))
+    x0 = rng.uniform(size=(m, d))
+    return p0, a0, x0
+
+
+def _diffeo(d, width, T, p0, a0, x0, prec="fp64", scheme="euler", improved=True):
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    df = HIPDiffeos(d, prec)
+    df.SetKernelWidth(width)
+    df.SetNumberOfTimePoints(T)
+    df.SetShootingScheme(scheme)
+    df.SetStartPositions(p0)
+    df.SetStartMomentas(a0)
+    df.SetLandmarkPoints(x0)
+    if not improved:
+        df.UseStandardEuler()
+    return df
+
+
+@pytest.mark.parametrize("scheme", ["euler", "heun"])
+@pytest.mark.parametrize("improved", [True, False])
+def test_shoot_and_flow_match_oracle(oracle_mod, scheme, improved):
+    from oracle import deformations as dfo
+    p0, a0, x0 = _setup(1, 150, 400, 3)
+    width, T = 0.3, 11
+    df = _diffeo(3, width, T, p0, a0, x0, scheme=scheme, improved=improved)
+    df.Update()
+    P, A = dfo.shoot(p0, a0, width, T, scheme=scheme)
+    X = dfo.flow(P, A, x0, width, T, improved_euler=improved)
+    for t in range(T):
+        assert rel(df.GetTrajectoryPositions()[t], P[t]) < 1e-13
+        assert rel(df.GetTrajectoryMomentas()[t], A[t]) < 1e-12
+        assert rel(df.GetLandmarkPointsTrajectory()[t], X[t]) < 1e-13
+
+
+def test_shoot_fp32c_within_1e6(oracle_mod):
+    from oracle import deformations as dfo
+    p0, a0, x0 = _setup(2, 1000, 2000, 3, scale=0.01)   # landmark-shooting workload shape
+    width, T = 0.2, 11
+    df = _diffeo(3, width, T, p0, a0, x0, prec="fp32c")
+    df.Update()
+    P, A = dfo.shoot(p0, a0, width, T)
+    X = dfo.flow(P, A, x0, width, T)
+    assert rel(df.GetTrajectoryPositions()[-1] - p0, P[-1] - p0) < 1e-6
+    assert rel(df.GetTrajectoryMomentas()[-1], A[-1]) < 1e-6
+    assert rel(df.GetLandmarkPointsTrajectory()[-1] - x0, X[-1] - x0) < 1e-6
+
+
+def test_zero_momenta_leave_points_in_place():
+    p0, _, x0 = _setup(3, 20, 30, 2)
+    df = _diffeo(2, 0.5, 5, p0, np.full((20, 2), 1e-22), x0)
+    df.Update()
+    for p in df.GetTrajectoryPositions():
+        np.testing.assert_array_equal(p, p0)
+    for a in df.GetTrajectoryMomentas()[1:]:
+        np.testing.assert_array_equal(a, np.zeros_like(a))
+    for x in df.GetLandmarkPointsTrajectory():
+        np.testing.assert_array_equal(x, x0)
+
+
+def test_gpu_shoot_reproduces_reference_parallel_transport():
+    """GPU shooting of the reference's regression geodesic, transported by the oracle
+    restatement of Diffeos::ParallelTransport: GroundTruthMom.txt to 1e-4
+    (TestParallelTransport.cxx:77-81)."""
+    from oracle import deformations as dfo
+    r = lambda f: dfo.read_dlm(os.path.join(GOLD, f))
+    cp, mom = r("CPRegression.txt"), r("MomRegression.txt")
+    df = _diffeo(2, 15.0, 30, cp, mom, cp[:1])
+    df.Update()
+    P, A = df.GetTrajectoryPositions(), df.GetTrajectoryMomentas()
+    out = dfo.parallel_transport(P, A, 15.0, 30, r("MomToTransport.txt"), r("CPToTransport.txt"), 0.0,
+                                 [i / 9.0 for i in range(10)])
+    assert np.max(np.abs(out[-1] - r("GroundTruthMom.txt"))) <= 1e-4
+
+
+def test_shoot_vjp_matches_jacobian_differences():
+    """J_F^T (mu_p, mu_a) against central differences of the shooting field F."""
+    from deformetrica_code_amd import HIPExactKernel
+    from deformetrica_code_amd.diffeos import shoot_vjp
+    rng = np.random.default_rng(4)
+    n, d, width = 25, 3, 0.4
+    p = rng.uniform(size=(n, d))
+    a = rng.normal(size=(n, d))
+    mp = rng.normal(size=(n, d))
+    ma = rng.normal(size=(n, d))
+
+    def F(pp, aa):
+        k = HIPExactKernel(d, width, "fp64")
+        k.SetSources(pp)
+        k.SetWeights(aa)
+        c, g = k.ConvolveAndGradient(pp, aa)
+        return c, -g
+
+    def objective(pp, aa):
+        fp, fa = F(pp, aa)
+        return float(np.sum(mp * fp) + np.sum(ma * fa))
+
+    jp, ja = shoot_vjp(p, a, mp, ma, width)
+    eps = 1e-6
+    fdp, fda = np.zeros_like(p), np.zeros_like(a)
+    for i in range(n):
+        for k in range(d):
+            for arr, out in ((p, fdp), (a, fda)):
+                plus, minus = arr.copy(), arr.copy()
+                plus[i, k] += eps
+                minus[i, k] -= eps
+                if arr is p:
+                    out[i, k] = (objective(plus, a) - objective(minus, a)) / (2 * eps)
+                else:
+                    out[i, k] = (objective(p, plus) - objective(p, minus)) / (2 * eps)
+    assert rel(jp, fdp) < 1e-7
+    assert rel(ja, fda) < 1e-7
+
+
+class _TerminalCost:
+    """E(p0, a0, x0) = 1/2 |x_final - target|^2 through the GPU shoot + flow, with the
+    backward from the hand-written adjoint, as a torch.autograd.Function (test harness)."""
+
+    @staticmethod
+    def make(d, width, T, target, scheme, improved, k, prec="fp64"):
+        import torch
+
+        class Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, p0, a0, x0):
+                df = _diffeo(d, width, T, p0.numpy(), a0.numpy(), x0.numpy(), prec, scheme, improved)
+                df.Update()
+                xT = df.GetDeformedLandmarkPoints()
+                ctx.save_for_backward(p0, a0, x0)
+                ctx.theta = xT - target
+                return torch.tensor(0.5 * np.sum((xT - target) ** 2), dtype=torch.float64)
+
+            @staticmethod
+            def backward(ctx, g):
+                p0, a0, x0 = ctx.saved_tensors
+                df = _diffeo(d, width, T, p0.numpy(), a0.numpy(), x0.numpy(), prec, scheme, improved)
+                gp, ga, gx = df.GradientOfTerminalCost(ctx.theta, checkpoint_every=k)
+                s = float(g)
+                return (torch.from_numpy(gp * s), torch.from_numpy(ga * s), torch.from_numpy(gx * s))
+
+        return Fn
+
+
+@pytest.mark.parametrize("scheme,improved,T,k", [("euler", True, 20, 5), ("heun", True, 12, 3),
+                                                 ("euler", False, 9, 4), ("heun", False, 7, 1)])
+def test_adjoint_matches_central_differences(scheme, improved, T, k):
+    """The discrete adjoint is the exact gradient of the discrete forward: central
+    differences (eps = 1e-4) agree to 1e-5 relative."""
+    from deformetrica_code_amd import _lib
+    _lib.load()          # the library's HIP runtime first; torch stays on the CPU here
+    import torch
+    d, width = 3, 0.3
+    p0, a0, x0 = _setup(5 + T, 30, 60, d, scale=0.3)
+    target = x0 + np.random.default_rng(9).normal(scale=0.05, size=x0.shape)
+    Fn = _TerminalCost.make(d, width, T, target, scheme, improved, k)
+    tp, ta, tx = (torch.tensor(v, requires_grad=True) for v in (p0, a0, x0))
+    E = Fn.apply(tp, ta, tx)
+    gp, ga, gx = torch.autograd.grad(E, (tp, ta, tx))
+    eps = 1e-4
+
+    def E_at(pp, aa, xx):
+        return float(Fn.apply(torch.from_numpy(pp), torch.from_numpy(aa), torch.from_numpy(xx)))
+
+    rng = np.random.default_rng(11)
+    for name, base, g in (("p0", p0, gp), ("a0", a0, ga), ("x0", x0, gx)):
+        fd = np.zeros_like(base)
+        idx = [(i, kk) for i in rng.choice(base.shape[0], 6, replace=False) for kk in range(d)]
+        for i, kk in idx:
+            plus, minus = base.copy(), base.copy()
+            plus[i, kk] += eps
+            minus[i, kk] -= eps
+            args_p = {"p0": (plus, a0, x0), "a0": (p0, plus, x0), "x0": (p0, a0, plus)}[name]
+            args_m = {"p0": (minus, a0, x0), "a0": (p0, minus, x0), "x0": (p0, a0, minus)}[name]
+            fd[i, kk] = (E_at(*args_p) - E_at(*args_m)) / (2 * eps)
+        sel = tuple(np.array(idx).T)
+        assert rel(g.numpy()[sel], fd[sel]) < 1e-5, (name, rel(g.numpy()[sel], fd[sel]))
+
+
+def test_adjoint_checkpoint_interval_does_not_change_the_gradient():
+    p0, a0, x0 = _setup(21, 40, 50, 2, scale=0.2)
+    target = x0 + 0.02
+    grads = []
+    for k in (1, 2, 5, 16, 100):
+        df = _diffeo(2, 0.35, 17, p0, a0, x0, scheme="heun")
+        df.Update()
+        grads.append(df.GradientOfTerminalCost(df.GetDeformedLandmarkPoints() - target, checkpoint_every=k))
+    for g in grads[1:]:
+        for u, v in zip(g, grads[0]):
+            np.testing.assert_array_equal(u, v)
+
+
+def test_discrete_adjoint_agrees_with_reference_adjoint_to_first_order():
+    """The reference's continuous adjoint (AdjointEquationsIntegrator, restated in the
+    oracle) and the discrete adjoint differ by O(dt)."""
+    from oracle import deformations as dfo
+    p0, a0, x0 = _setup(31, 20, 30, 2, scale=0.3)
+    width = 0.5
+    target = x0 + 0.05
+    diffs = []
+    for T in (11, 41):
+        df = _diffeo(2, width, T, p0, a0, x0)
+        df.Update()
+        theta = df.GetDeformedLandmarkPoints() - target
+        gp, ga, _ = df.GradientOfTerminalCost(theta, checkpoint_every=5)
+        P, A = dfo.shoot(p0, a0, width, T)
+        X = dfo.flow(P, A, x0, width, T)
+        rp, ra, _ = dfo.adjoint_reference(P, A, X, X[-1] - target, width, T)
+        diffs.append((rel(gp, rp), rel(ga, ra)))
+    assert diffs[1][0] < diffs[0][0] / 2.5 and diffs[1][1] < diffs[0][1] / 2.5
+    assert diffs[1][0] < 0.05 and diffs[1][1] < 0.05
