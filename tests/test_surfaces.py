@@ -1,0 +1,61 @@
+"""Pins the oracle's surface data terms (oracle/surfaces.py) on the CPU."""
+import os
+
+import numpy as np
+import pytest
+
+MESH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes")
+
+
+def _meshes(seed=0):
+    from oracle import surfaces as so
+    V, F = so.read_vtk_polydata(os.path.join(MESH, "sphere80.vtk"))
+    rng = np.random.default_rng(seed)
+    Vs = V * 1.1 + rng.normal(scale=0.02, size=V.shape)
+    Vt = V + rng.normal(scale=0.03, size=V.shape) + 0.05
+    return Vs, F, Vt, F
+
+
+def test_read_reference_icosphere():
+    from oracle import surfaces as so
+    V, F = so.read_vtk_polydata(os.path.join(MESH, "sphere80.vtk"))
+    assert V.shape == (42, 3) and F.shape == (80, 3)
+    np.testing.assert_allclose(np.linalg.norm(V, axis=1), 0.5, atol=1e-5)
+    V2, F2 = so.read_vtk_polydata(os.path.join(MESH, "sphere1280.vtk"))
+    assert V2.shape == (642, 3) and F2.shape == (1280, 3)
+    # closed surface: total oriented area vanishes; the inscribed polyhedron has a little
+    # less area than the sphere (4 pi r^2)
+    from oracle.surfaces import current_geometry
+    _, n = current_geometry(V2, F2)
+    assert np.linalg.norm(n.sum(0)) < 1e-6
+    area = np.linalg.norm(n, axis=1).sum()
+    assert 0.99 * np.pi < area < np.pi
+
+
+@pytest.mark.parametrize("kind", ["current", "varifold"])
+def test_match_equals_brute_force_double_sum(kind):
+    from oracle import surfaces as so
+    Vs, Fs, Vt, Ft = _meshes()
+    w = 0.3
+    m = getattr(so, kind + "_match")(Vs, Fs, Vt, Ft, w)
+    b = getattr(so, "brute_force_" + kind)(Vs, Fs, Vt, Ft, w)
+    assert abs(m - b) <= 1e-12 * max(1.0, abs(b))
+    assert m > 0
+
+
+@pytest.mark.parametrize("kind", ["current", "varifold"])
+def test_gradient_matches_central_differences(kind):
+    from oracle import surfaces as so
+    Vs, Fs, Vt, Ft = _meshes(1)
+    w = 0.3
+    g = getattr(so, kind + "_match_gradient")(Vs, Fs, Vt, Ft, w)
+    match = getattr(so, kind + "_match")
+    eps = 1e-6
+    fd = np.zeros_like(Vs)
+    for i in range(Vs.shape[0]):
+        for k in range(3):
+            p, m = Vs.copy(), Vs.copy()
+            p[i, k] += eps
+            m[i, k] -= eps
+            fd[i, k] = (match(p, Fs, Vt, Ft, w) - match(m, Fs, Vt, Ft, w)) / (2 * eps)
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-6
