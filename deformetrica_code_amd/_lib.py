@@ -15,6 +15,7 @@ DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, 
 FP64, FP32C, FP32X = 0, 1, 2
 OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP = range(7)
 EULER, HEUN = 0, 1
+CURRENT, VARIFOLD = 0, 1
 
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X}
 
@@ -69,6 +70,15 @@ def load():
     L.dfm_shoot_flow_adjoint.argtypes = [vp, i, d, i, i, i, vp, vp, l, vp, l, i, d, d, i, vp, vp,
                                          vp, vp, vp, sz]
     for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint"):
+        getattr(L, name).restype = i
+    # surface data terms
+    L.dfm_mesh_incidence.argtypes = [vp, l, l, vp, vp]
+    L.dfm_surface_workspace.argtypes = [i, l, l, i, i]
+    L.dfm_surface_workspace.restype = sz
+    L.dfm_surface_match.argtypes = [vp, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
+    L.dfm_surface_match_gradient.argtypes = [vp, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l, vp,
+                                             vp, sz]
+    for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient"):
         getattr(L, name).restype = i
     L.dfm_profile_enable.argtypes = [i]
     L.dfm_profile_enable.restype = i

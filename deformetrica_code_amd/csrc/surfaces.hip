@@ -1,0 +1,373 @@
+// surfaces.hip — triangle-surface data terms on the GPU: currents (OrientedSurfaceMesh)
+// and varifolds (NonOrientedSurfaceMesh), their match values and their gradients with
+// respect to the deformed template's vertices.
+//
+// Every kernel sum is one pair-sum call; the rest is per-face and per-vertex work:
+//   k_face_geometry   centers, kernel weights W_f and per-face coefficients A_f
+//                     current:  W = A = N = (p2 - p0) x (p1 - p0) / 2
+//                               (OrientedSurfaceMesh.cxx:298-326)
+//                     varifold: N /= sqrt(|N| + 1e-20); W = upper triangle of N N^T,
+//                               A = the quadratic-form coefficients
+//                               (N0^2, 2N0N1, 2N0N2, N1^2, 2N1N2, N2^2) so that
+//                               N^T S(m) N = A . m (NonOrientedSurfaceMesh.cxx:269-308,
+//                               special_product :364-379)
+//   match             |S|^2 + |T|^2 - 2 <S, T> with <X, Y> = sum_f A^Y_f . (K W^X)(c^Y_f):
+//                     three Convolve calls and deterministic two-stage FP64 reductions
+//                     (OrientedSurfaceMesh.cxx:125-163, NonOrientedSurfaceMesh.cxx:124-165)
+//   gradient          (K W)(c_S) and ConvolveGradient(c_S, A) from the template and from
+//                     the target (two fused Convolve + Grad1 calls; for the varifold the
+//                     reference's contraction of the varlin list with N N^T
+//                     (NonOrientedSurfaceMesh.cxx:243-249) is exactly Grad1 with alpha = A),
+//                     then per-face corner contributions (k_face_contrib) and a
+//                     deterministic per-vertex gather over a vertex->corner incidence list
+//                     (k_vertex_gather), in place of the reference's serial scatter
+//                     (OrientedSurfaceMesh.cxx:205-234, NonOrientedSurfaceMesh.cxx:213-256).
+// Faces with an index outside [0, nv) contribute nothing and set a flag that the
+// entry points report (device-side bounds checks: no out-of-range access).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dfm_hip.h"
+#include "dfm_internal.h"
+
+namespace dfm {
+namespace {
+
+constexpr int kRedBlocks = 256;
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// geometry of each face; bad indices flag the face (weights zero)
+__global__ __launch_bounds__(256) void k_face_geometry(int kind, const double *V, long nv,
+                                                       const int *F, long nf, double *C,
+                                                       double *W, double *A, double *Nout,
+                                                       double *bad) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  int i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+  if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) {
+    for (int k = 0; k < 3; k++) C[3 * f + k] = 0.0, Nout[3 * f + k] = 0.0;
+    for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
+    *bad = 1.0;
+    return;
+  }
+  double p0[3], p1[3], p2[3], u[3], v[3], n[3];
+  for (int k = 0; k < 3; k++) {
+    p0[k] = V[3 * (long)i0 + k];
+    p1[k] = V[3 * (long)i1 + k];
+    p2[k] = V[3 * (long)i2 + k];
+    C[3 * f + k] = (p0[k] + p1[k] + p2[k]) / 3.0;
+    u[k] = p2[k] - p0[k];
+    v[k] = p1[k] - p0[k];
+  }
+  cross3(u, v, n);
+  for (int k = 0; k < 3; k++) n[k] /= 2;
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < 3; k++) W[3 * f + k] = A[3 * f + k] = Nout[3 * f + k] = n[k];
+    return;
+  }
+  const double s = sqrt(sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) + 1e-20);
+  for (int k = 0; k < 3; k++) {
+    n[k] /= s;
+    Nout[3 * f + k] = n[k];
+  }
+  int idx = 0;
+  for (int p = 0; p < 3; p++)
+    for (int q = p; q < 3; q++, idx++) {
+      W[6 * f + idx] = n[p] * n[q];
+      A[6 * f + idx] = (p == q ? 1.0 : 2.0) * n[p] * n[q];
+    }
+}
+
+// partial[b] = sum over the block's faces of dot(X_f, Y_f) (dv columns)
+__global__ __launch_bounds__(256) void k_dot_partial(const double *X, const double *Y, long nf,
+                                                     int dv, double *partial) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+       f += (long)gridDim.x * blockDim.x)
+    for (int k = 0; k < dv; k++) s += X[dv * f + k] * Y[dv * f + k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out = c0 * sum(p0) + c1 * sum(p1) + c2 * sum(p2) over kRedBlocks partials each
+__global__ __launch_bounds__(256) void k_combine(const double *p0, double c0, const double *p1,
+                                                 double c1, const double *p2, double c2,
+                                                 double *out) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  sh[t] = c0 * p0[t] + c1 * p1[t] + c2 * p2[t];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) sh[t] += sh[t + st];
+    __syncthreads();
+  }
+  if (t == 0) *out = sh[0];
+}
+
+__device__ __forceinline__ void sym_times(const double *m, const double *x, double *y) {
+  // symmetric matrix from (00, 01, 02, 11, 12, 22) times x
+  y[0] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+  y[1] = m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
+  y[2] = m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
+}
+
+// contrib[f][c] (3 corners x 3) of the match gradient
+__global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V, long nv,
+                                                      const int *F, long nf, const double *N,
+                                                      const double *ktS, const double *gS,
+                                                      const double *ktT, const double *gT,
+                                                      double *contrib) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double *out = contrib + 9 * f;
+  for (int k = 0; k < 9; k++) out[k] = 0.0;
+  const int i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+  if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) return;
+  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  double e[3][3], kt[3], gk[3];
+  for (int k = 0; k < 3; k++) {
+    e[0][k] = V[3 * (long)i2 + k] - V[3 * (long)i1 + k];
+    e[1][k] = V[3 * (long)i0 + k] - V[3 * (long)i2 + k];
+    e[2][k] = V[3 * (long)i1 + k] - V[3 * (long)i0 + k];
+    gk[k] = (gS[3 * f + k] - gT[3 * f + k]) * 2.0 / 3.0;
+  }
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < 3; k++) kt[k] = ktS[3 * f + k] - ktT[3 * f + k];
+  } else {
+    const double *n = N + 3 * f;
+    const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (!(n2 > 1e-20)) return;
+    double dm[6], t[3];
+    for (int k = 0; k < 6; k++) dm[k] = ktS[dv * f + k] - ktT[dv * f + k];
+    sym_times(dm, n, t);
+    for (int k = 0; k < 3; k++) t[k] *= 2.0;
+    const double nt = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];  // S(N N^T) t = N (N . t)
+    const double r = 1.0 / sqrt(n2);
+    for (int k = 0; k < 3; k++) kt[k] = (t[k] - n[k] * nt / (2 * n2)) * r;
+  }
+  for (int c = 0; c < 3; c++) {
+    double x[3];
+    cross3(e[c], kt, x);
+    for (int k = 0; k < 3; k++) out[3 * c + k] = x[k] + gk[k];
+  }
+}
+
+// grad[v] = sum of the corner contributions incident to v, in incidence order
+__global__ __launch_bounds__(256) void k_vertex_gather(const int *offsets, const int *inc,
+                                                       long nv, long ncorner,
+                                                       const double *contrib, double *grad) {
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double g[3] = {0.0, 0.0, 0.0};
+  const long b = max(offsets[v], 0), e = min((long)offsets[v + 1], ncorner);
+  for (long i = b; i < e; i++) {
+    const int c = inc[i];
+    if (c < 0 || c >= ncorner) continue;
+    for (int k = 0; k < 3; k++) g[k] += contrib[3 * (long)c + k];
+  }
+  for (int k = 0; k < 3; k++) grad[3 * v + k] = g[k];
+}
+
+size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+  double *cS, *wS, *aS, *nS, *cT, *wT, *aT, *nT;
+  double *k1, *k2, *k3, *g1, *g2;  // conv / grad outputs
+  double *p1, *p2, *p3;            // reduction partials
+  double *contrib;
+  double *bad;
+  void *pw;
+  size_t pw_bytes, total;
+  Layout(char *base, int kind, int prec, long nfs, long nft, bool gradient) {
+    const int dv = kind == DFM_CURRENT ? 3 : 6;
+    size_t off = 0;
+    auto take = [&](size_t count) {
+      double *p = base ? (double *)(base + off) : nullptr;
+      off += a256(count * sizeof(double));
+      return p;
+    };
+    cS = take(3 * nfs); wS = take(dv * nfs); aS = take(dv * nfs); nS = take(3 * nfs);
+    cT = take(3 * nft); wT = take(dv * nft); aT = take(dv * nft); nT = take(3 * nft);
+    const long nmax = std::max(nfs, nft);
+    k1 = take(dv * nmax); k2 = take(dv * nmax); k3 = take(dv * nmax);
+    g1 = take(3 * nfs); g2 = take(3 * nfs);
+    p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
+    contrib = gradient ? take(9 * nfs) : nullptr;
+    bad = take(1);
+    pw_bytes = 0;
+    if (gradient) {
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONVGRAD1, prec, 3, dv, nfs, nfs));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONVGRAD1, prec, 3, dv, nfs, nft));
+    } else {
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nfs, nfs));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nft, nft));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nft, nfs));
+    }
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
+
+int launched(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+unsigned blocks_for(long n) { return (unsigned)std::max(1L, (n + 255) / 256); }
+
+KParams blank() {
+  KParams p;
+  std::memset(&p, 0, sizeof(p));
+  return p;
+}
+
+int check_args(int kind, int prec, double sigma, long nvs, long nfs, long nvt, long nft) {
+  if (kind != DFM_CURRENT && kind != DFM_VARIFOLD)
+    return set_error(DFM_ERR_ARG, "unknown data term %d", kind);
+  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+    return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
+  if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
+  if (nvs < 0 || nfs < 0 || nvt < 0 || nft < 0 || nfs > 0x7fffffffL || nft > 0x7fffffffL ||
+      nvs > 0x7fffffffL || nvt > 0x7fffffffL)
+    return set_error(DFM_ERR_ARG, "counts must be in [0, 2^31)");
+  return DFM_OK;
+}
+
+// geometry of both meshes and the reset of the bad-face flag
+int geometry(hipStream_t st, int kind, const Layout &L, const double *vs, long nvs, const int *fs,
+             long nfs, const double *vt, long nvt, const int *ft, long nft) {
+  int rc = fill(st, L.bad, 1, 0.0);  // a fill kernel, so that callers may capture the call
+  if (rc) return rc;
+  if (nfs > 0)
+    hipLaunchKernelGGL(k_face_geometry, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
+                       nfs, L.cS, L.wS, L.aS, L.nS, L.bad);
+  if (nft > 0)
+    hipLaunchKernelGGL(k_face_geometry, dim3(blocks_for(nft)), dim3(256), 0, st, kind, vt, nvt, ft,
+                       nft, L.cT, L.wT, L.aT, L.nT, L.bad);
+  return launched("k_face_geometry");
+}
+
+int report_bad(hipStream_t st, const Layout &L) {
+  double bad = 0.0;
+  hipError_t e = hipMemcpyAsync(&bad, L.bad, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return set_error(DFM_ERR_HIP, "flag read: %s", hipGetErrorString(e));
+  return bad != 0.0 ? set_error(DFM_ERR_ARG, "a face references a vertex index outside [0, nv)") : DFM_OK;
+}
+
+}  // namespace
+}  // namespace dfm
+
+using namespace dfm;
+
+extern "C" {
+
+int dfm_mesh_incidence(const int *faces, long nf, long nv, int *offsets, int *incidences) {
+  clear_error();
+  if (nf < 0 || nv < 0 || nf > 0x2aaaaaaaL) return set_error(DFM_ERR_ARG, "bad counts");
+  if ((nf > 0 && (!faces || !incidences)) || !offsets) return set_error(DFM_ERR_ARG, "NULL argument");
+  std::vector<int> cnt((size_t)nv + 1, 0);
+  for (long i = 0; i < 3 * nf; i++) {
+    const int v = faces[i];
+    if (v < 0 || v >= nv) return set_error(DFM_ERR_ARG, "face %ld references vertex %d outside [0, %ld)", i / 3, v, nv);
+    cnt[v + 1]++;
+  }
+  for (long v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
+  for (long v = 0; v <= nv; v++) offsets[v] = cnt[v];
+  std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+  for (long i = 0; i < 3 * nf; i++) incidences[pos[faces[i]]++] = (int)i;  // 3 * face + corner
+  return DFM_OK;
+}
+
+size_t dfm_surface_workspace(int kind, long nfs, long nft, int precision, int gradient) {
+  if (check_args(kind, precision, 1.0, 0, nfs, 0, nft)) return 0;
+  return Layout(nullptr, kind, precision, nfs, nft, gradient != 0).total;
+}
+
+int dfm_surface_match(void *stream, int kind, double sigma, int prec, const double *vs, long nvs,
+                      const int *fs, long nfs, const double *vt, long nvt, const int *ft, long nft,
+                      double *match, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_args(kind, prec, sigma, nvs, nfs, nvt, nft);
+  if (rc) return rc;
+  if (!match || (nfs > 0 && (!vs || !fs)) || (nft > 0 && (!vt || !ft)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  Layout need(nullptr, kind, prec, nfs, nft, false);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  Layout L((char *)work, kind, prec, nfs, nft, false);
+  hipStream_t st = (hipStream_t)stream;
+  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  if ((rc = geometry(st, kind, L, vs, nvs, fs, nfs, vt, nvt, ft, nft))) return rc;
+  KParams p = blank();
+  // |S|^2: (K W_S)(c_S) . A_S
+  p.x = L.cS; p.nx = nfs; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k1;
+  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nfs * dv, 0, L.pw, L.pw_bytes))) return rc;
+  // |T|^2
+  p.x = L.cT; p.nx = nft; p.y = L.cT; p.w = L.wT; p.ny = nft; p.out0 = L.k2;
+  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
+  // <S, T>: (K W_S)(c_T) . A_T
+  p.x = L.cT; p.nx = nft; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k3;
+  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, nfs, dv, L.p1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aT, nft, dv, L.p2);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, L.aT, nft, dv, L.p3);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, st, L.p2, 1.0, L.p1, 1.0, L.p3, -2.0, match);
+  if ((rc = launched("match reduction"))) return rc;
+  return report_bad(st, L);
+}
+
+int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, const double *vs,
+                               long nvs, const int *fs, long nfs, const int *offsets,
+                               const int *incidences, const double *vt, long nvt, const int *ft,
+                               long nft, double *grad, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_args(kind, prec, sigma, nvs, nfs, nvt, nft);
+  if (rc) return rc;
+  if (!grad || !offsets || (nfs > 0 && (!vs || !fs || !incidences)) || (nft > 0 && (!vt || !ft)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  Layout need(nullptr, kind, prec, nfs, nft, true);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  Layout L((char *)work, kind, prec, nfs, nft, true);
+  hipStream_t st = (hipStream_t)stream;
+  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  if ((rc = geometry(st, kind, L, vs, nvs, fs, nfs, vt, nvt, ft, nft))) return rc;
+  KParams p = blank();
+  // template on template: (K W_S)(c_S) and ConvolveGradient(c_S, A_S)
+  p.x = L.cS; p.ta = L.aS; p.nx = nfs; p.y = L.cS; p.w = L.wS; p.ny = nfs;
+  p.out0 = L.k1; p.out1 = L.g1;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
+    return rc;
+  // target on template
+  p.y = L.cT; p.w = L.wT; p.ny = nft; p.out0 = L.k2; p.out1 = L.g2;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
+    return rc;
+  if (nfs > 0)
+    hipLaunchKernelGGL(k_face_contrib, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
+                       nfs, L.nS, L.k1, L.g1, L.k2, L.g2, L.contrib);
+  if (nvs > 0)
+    hipLaunchKernelGGL(k_vertex_gather, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
+                       incidences, nvs, 3 * nfs, L.contrib, grad);
+  if ((rc = launched("gradient assembly"))) return rc;
+  return report_bad(st, L);
+}
+
+}  // extern "C"

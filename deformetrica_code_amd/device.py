@@ -16,11 +16,12 @@ from . import _lib
 
 
 class DeviceArray:
-    """A C-contiguous float64 array in device memory."""
+    """A C-contiguous array in device memory (float64, or int32 for mesh indices)."""
 
-    def __init__(self, shape):
+    def __init__(self, shape, dtype=np.float64):
         self.shape = tuple(int(s) for s in shape)
-        self.nbytes = int(np.prod(self.shape)) * 8
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         self._ptr = ctypes.c_void_p()
         _lib.check(_lib.load().dfm_malloc(ctypes.byref(self._ptr), self.nbytes))
 
@@ -29,21 +30,21 @@ class DeviceArray:
         return self._ptr.value
 
     @classmethod
-    def from_numpy(cls, a, stream=None):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        d = cls(a.shape)
+    def from_numpy(cls, a, stream=None, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, dtype)
         d.upload(a, stream)
         return d
 
     def upload(self, a, stream=None):
-        a = np.ascontiguousarray(a, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=self.dtype)
         if a.shape != self.shape:
             raise ValueError("shape mismatch %s vs %s" % (a.shape, self.shape))
         _lib.check(_lib.load().dfm_memcpy(self._ptr, a.ctypes.data_as(ctypes.c_void_p),
                                           self.nbytes, 0, _sp(stream)))
 
     def numpy(self, stream=None) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
+        out = np.empty(self.shape, dtype=self.dtype)
         _lib.check(_lib.load().dfm_memcpy(out.ctypes.data_as(ctypes.c_void_p), self._ptr,
                                           self.nbytes, 1, _sp(stream)))
         return out

@@ -224,6 +224,41 @@ int dfm_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precisi
                            int checkpoint_every, const double *theta_T, double *grad_p0,
                            double *grad_a0, double *grad_x0, void *work, size_t work_bytes);
 
+/* ---- surface data terms (device pointers) ----------------------------------------------
+ * Meshes are vertices (nv x 3, FP64, row-major) and triangles (nf x 3, int32 indices).
+ * DFM_CURRENT is OrientedSurfaceMesh (area-weighted normals as currents), DFM_VARIFOLD is
+ * NonOrientedSurfaceMesh (normals scaled by |N|^-1/2, upper-triangular N N^T weights). */
+enum dfm_data_term { DFM_CURRENT = 0, DFM_VARIFOLD = 1 };
+
+/* Vertex -> corner incidence list of a mesh (host pointers, built once per topology):
+ * offsets (nv + 1), incidences (3 nf) holding 3 * face + corner, grouped by vertex in
+ * increasing corner order.  Used by dfm_surface_match_gradient's deterministic gather, in
+ * place of the reference's serial scatter over faces (OrientedSurfaceMesh.cxx:205-234,
+ * NonOrientedSurfaceMesh.cxx:213-256).  DFM_ERR_ARG for an index outside [0, nv). */
+int dfm_mesh_incidence(const int *faces, long nf, long nv, int *offsets, int *incidences);
+
+/* Workspace bytes for a match (gradient = 0) or a match gradient (gradient = 1) between a
+ * template of nfs faces and a target of nft faces. */
+size_t dfm_surface_workspace(int kind, long nfs, long nft, int precision, int gradient);
+
+/* *match (device, one double) = |S|^2 + |T|^2 - 2 <S, T> of the template S (vs, fs) and the
+ * target T (vt, ft) at kernel width sigma.  Replaces OrientedSurfaceMesh::ComputeMatch
+ * (OrientedSurfaceMesh.cxx:125-163) and NonOrientedSurfaceMesh::ComputeMatch
+ * (NonOrientedSurfaceMesh.cxx:124-165).  DFM_ERR_ARG for a face index outside [0, nv). */
+int dfm_surface_match(void *stream, int kind, double sigma, int precision, const double *vs,
+                      long nvs, const int *fs, long nfs, const double *vt, long nvt,
+                      const int *ft, long nft, double *match, void *work, size_t work_bytes);
+
+/* grad (device, nvs x 3) = d match / d vs, with the template's incidence list from
+ * dfm_mesh_incidence (device copies of offsets and incidences).  Replaces
+ * OrientedSurfaceMesh::ComputeMatchGradient (OrientedSurfaceMesh.cxx:166-237) and
+ * NonOrientedSurfaceMesh::ComputeMatchGradient (NonOrientedSurfaceMesh.cxx:169-260). */
+int dfm_surface_match_gradient(void *stream, int kind, double sigma, int precision,
+                               const double *vs, long nvs, const int *fs, long nfs,
+                               const int *offsets, const int *incidences, const double *vt,
+                               long nvt, const int *ft, long nft, double *grad, void *work,
+                               size_t work_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,3 +77,21 @@ def test_python_mirror_errors():
         k.ConvolveGradient(np.zeros((2, 3)), 3)
     with pytest.raises(ValueError):
         HIPExactKernel(4)
+
+
+def test_mesh_incidence_is_host_only_and_checks_indices():
+    """dfm_mesh_incidence groups the 3 nf corners by vertex (CSR), in corner order."""
+    import numpy as np
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.surfaces import mesh_incidence
+    F = np.array([[0, 1, 2], [2, 1, 3], [3, 0, 2]], dtype=np.int32)
+    off, inc = mesh_incidence(F, 5)
+    np.testing.assert_array_equal(off, [0, 2, 4, 7, 9, 9])
+    np.testing.assert_array_equal(inc, [0, 7, 1, 4, 2, 3, 8, 5, 6])
+    flat = F.reshape(-1)
+    for v in range(5):
+        assert all(flat[c] == v for c in inc[off[v]:off[v + 1]])
+    with pytest.raises(_lib.DfmError, match="outside"):
+        mesh_incidence(np.array([[0, 1, 5]], dtype=np.int32), 5)
+    with pytest.raises(_lib.DfmError, match="outside"):
+        mesh_incidence(np.array([[0, -1, 2]], dtype=np.int32), 5)
