@@ -71,6 +71,28 @@ def load():
                                          vp, vp, vp, sz]
     for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint"):
         getattr(L, name).restype = i
+    # ragged batches and graphs
+    L.dfm_batch_workspace.argtypes = [i, i, i, i, vp, vp, i]
+    L.dfm_batch_workspace.restype = sz
+    L.dfm_batch_conv_grad1.argtypes = [vp, i, i, d, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz]
+    L.dfm_batch_shoot_workspace.argtypes = [i, i, vp, i]
+    L.dfm_batch_shoot_workspace.restype = sz
+    L.dfm_batch_shoot.argtypes = [vp, i, d, i, i, i, vp, vp, vp, i, d, d, vp, vp, vp, sz]
+    L.dfm_batch_flow_workspace.argtypes = [i, i, vp, vp, i]
+    L.dfm_batch_flow_workspace.restype = sz
+    L.dfm_batch_flow.argtypes = [vp, i, d, i, i, i, vp, vp, vp, vp, vp, i, d, d, vp, vp, sz]
+    L.dfm_batch_adjoint_workspace.argtypes = [i, i, vp, vp, i, i, i]
+    L.dfm_batch_adjoint_workspace.restype = sz
+    L.dfm_batch_shoot_flow_adjoint.argtypes = [vp, i, d, i, i, i, i, vp, vp, vp, vp, vp, i, d, d,
+                                               i, vp, vp, vp, vp, vp, sz]
+    L.dfm_graph_begin_capture.argtypes = [vp]
+    L.dfm_graph_end_capture.argtypes = [vp, ctypes.POINTER(vp)]
+    L.dfm_graph_launch.argtypes = [vp, vp]
+    L.dfm_graph_destroy.argtypes = [vp]
+    for name in ("dfm_batch_conv_grad1", "dfm_batch_shoot", "dfm_batch_flow",
+                 "dfm_batch_shoot_flow_adjoint", "dfm_graph_begin_capture",
+                 "dfm_graph_end_capture", "dfm_graph_launch", "dfm_graph_destroy"):
+        getattr(L, name).restype = i
     # surface data terms
     L.dfm_mesh_incidence.argtypes = [vp, l, l, vp, vp]
     L.dfm_surface_workspace.argtypes = [i, l, l, i, i]

@@ -140,6 +140,42 @@ int dfm_stream_synchronize(void *stream) {
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
 }
 
+int dfm_graph_begin_capture(void *stream) {
+  clear_error();
+  if (!stream) return set_error(DFM_ERR_ARG, "capture needs a created stream, not the default one");
+  hipError_t e = hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+}
+
+int dfm_graph_end_capture(void *stream, void **graph_exec) {
+  clear_error();
+  if (!stream || !graph_exec) return set_error(DFM_ERR_ARG, "NULL argument");
+  *graph_exec = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e != hipSuccess) return set_error(DFM_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_error(DFM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  *graph_exec = (void *)ex;
+  return DFM_OK;
+}
+
+int dfm_graph_launch(void *graph_exec, void *stream) {
+  clear_error();
+  if (!graph_exec) return set_error(DFM_ERR_ARG, "graph is NULL");
+  hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+}
+
+int dfm_graph_destroy(void *graph_exec) {
+  clear_error();
+  if (!graph_exec) return DFM_OK;
+  hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipGraphExecDestroy: %s", hipGetErrorString(e));
+}
+
 int dfm_profile_enable(int on) {
   profile_enable(on != 0);
   return DFM_OK;
@@ -426,6 +462,87 @@ int dfm_gauss_conv_grad1(int D, int DV, double sigma, const double *x, const dou
   if (conv) TRY(download(conv, dc, (size_t)nx * DV));
   if (grad) TRY(download(grad, dg, (size_t)nx * D));
   return finish();
+}
+
+
+}  // extern "C"
+
+// ---- ragged batches ---------------------------------------------------------------------
+
+namespace {
+
+struct BatchLayout {
+  long *xoff, *yoff;
+  void *pw;
+  size_t pw_bytes, total;
+  BatchLayout(char *base, int op, int prec, int D, int DV, const Seg &xs, const Seg &ys) {
+    auto a256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off = 0;
+    xoff = base ? (long *)(base + off) : nullptr;
+    off += a256(sizeof(long) * (xs.n + 1));
+    yoff = base ? (long *)(base + off) : nullptr;
+    off += a256(sizeof(long) * (ys.n + 1));
+    pw_bytes = pairsum_batch_workspace(op, prec, D, DV, xs, ys);
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
+
+int batch_op(int op_flags_conv, int op_flags_grad) {
+  return op_flags_conv && op_flags_grad ? OP_CONVGRAD1 : op_flags_conv ? OP_CONV : OP_GRAD1;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dfm_batch_workspace(int op, int dimpoint, int dimvect, int nseg, const long *x_offsets,
+                           const long *y_offsets, int precision) {
+  if (op != DFM_OP_CONV && op != DFM_OP_GRAD1 && op != DFM_OP_CONV_GRAD1) return 0;
+  Seg xs, ys;
+  if (seg_from_host(xs, nseg, x_offsets, nullptr) || seg_from_host(ys, nseg, y_offsets, nullptr))
+    return 0;
+  if (check_common(dimpoint, dimvect, precision, 1.0, xs.total, ys.total)) return 0;
+  return BatchLayout(nullptr, op, precision, dimpoint, dimvect, xs, ys).total;
+}
+
+int dfm_batch_conv_grad1(void *stream, int D, int DV, double sigma, int prec, int nseg,
+                         const long *x_offsets, const long *y_offsets, const double *x,
+                         const double *alpha, const double *y, const double *beta, double *conv,
+                         double *grad, void *work, size_t work_bytes) {
+  clear_error();
+  Seg xs, ys;
+  int rc = seg_from_host(xs, nseg, x_offsets, nullptr);
+  if (!rc) rc = seg_from_host(ys, nseg, y_offsets, nullptr);
+  if (!rc) rc = check_common(D, DV, prec, sigma, xs.total, ys.total);
+  if (!rc && xs.total > 0) rc = need(x, "x");
+  if (!rc && xs.total > 0 && grad) rc = need(alpha, "alpha");
+  if (!rc && xs.total > 0 && ys.total > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  if (!conv && !grad) return DFM_OK;
+  const int op = batch_op(conv != nullptr, grad != nullptr);
+  BatchLayout need_(nullptr, op, prec, D, DV, xs, ys);
+  if (!work || work_bytes < need_.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need_.total);
+  BatchLayout L((char *)work, op, prec, D, DV, xs, ys);
+  hipStream_t st = (hipStream_t)stream;
+  if (nseg > 1) {
+    if ((rc = put_longs(st, L.xoff, x_offsets, nseg + 1))) return rc;
+    if ((rc = put_longs(st, L.yoff, y_offsets, nseg + 1))) return rc;
+    xs.d = L.xoff;
+    ys.d = L.yoff;
+  }
+  KParams p = blank();
+  p.x = x; p.ta = alpha; p.y = y; p.w = beta;
+  if (op == OP_GRAD1) {
+    p.out0 = grad;
+    return pairsum_batch(st, op, prec, D, DV, sigma, p, xs, ys, xs.total * D, 0, L.pw, L.pw_bytes);
+  }
+  p.out0 = conv;
+  p.out1 = grad;
+  return pairsum_batch(st, op, prec, D, DV, sigma, p, xs, ys, xs.total * DV,
+                       grad ? xs.total * D : 0, L.pw, L.pw_bytes);
 }
 
 }  // extern "C"

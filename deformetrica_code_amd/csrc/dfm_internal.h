@@ -43,6 +43,10 @@ struct KParams {
   const double *se;  // per-source extra: eta_j (GRADDIFF), mu_p_j (SHOOTVJP)
   const double *se2; // per-source extra: mu_a_j (SHOOTVJP)
   const uint32_t *perm;
+  // ragged batch (blockIdx.z = segment): segment s owns targets [xoff[s], xoff[s + 1]) and
+  // sources [yoff[s], yoff[s + 1]) of the concatenated arrays; null for a single segment
+  const long *xoff;
+  const long *yoff;
   double *partial;
   double *out0;
   double *out1;
@@ -61,7 +65,9 @@ struct KParams {
 // Work decomposition and workspace layout of one call.
 struct Plan {
   int R;            // targets per lane
-  long chunks;      // workgroups along the targets
+  int nseg;         // segments of a ragged batch (grid.z)
+  bool sort;        // Morton-order the targets first (single-segment FP32 calls)
+  long chunks;      // workgroups along the targets (of the largest segment)
   int nsplit;       // source splits
   long split_len;   // sources per split (multiple of the tile)
   size_t off_bbox, off_keys_in, off_keys_out, off_vals_in, off_perm, off_sort_tmp, off_partial;
@@ -74,6 +80,10 @@ const char *last_error();
 void clear_error();
 
 Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
+// A ragged batch of nseg segments: nx_total targets in all, at most max_nx targets and
+// max_ny sources in one segment.  No Morton sort (batched calls are capture-safe).
+Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
+                     long max_ny);
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
                    char *work);
 int fill(hipStream_t st, double *dst, long n, double v);
@@ -83,6 +93,33 @@ int fill(hipStream_t st, double *dst, long n, double v);
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
             long out0_n, long out1_n, void *work, size_t work_bytes);
 size_t pairsum_workspace(int op, int prec, int D, int DV, long nx, long ny);
+
+// Segments of a ragged batch: host offsets (n + 1 entries, from 0, non-decreasing) and
+// their device copy.  n == 1 with null offsets is one plain segment of `total` points.
+struct Seg {
+  int n = 1;
+  const long *h = nullptr;
+  const long *d = nullptr;
+  long total = 0;
+  long max = 0;
+  static Seg single(long count) {
+    Seg s;
+    s.total = s.max = count;
+    return s;
+  }
+  long count(int i) const { return h ? h[i + 1] - h[i] : total; }
+};
+// Checks host offsets (first 0, non-decreasing) and fills total and max.
+int seg_from_host(Seg &s, int n, const long *h, const long *d);
+// Writes n longs from host memory into device memory with kernel launches only (no
+// memcpy node, so a captured graph replays it).
+int put_longs(hipStream_t st, long *dst, const long *src, int n);
+// One pair-sum call over a ragged batch (targets segmented by xs, sources by ys, same
+// segment count); p.nx / p.ny are the concatenated totals.
+int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
+                  const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
+                  size_t work_bytes);
+size_t pairsum_batch_workspace(int op, int prec, int D, int DV, const Seg &xs, const Seg &ys);
 int copy_d2d(hipStream_t st, void *dst, const void *src, size_t bytes);
 
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)

@@ -1,5 +1,6 @@
 // integrators.hip — device-resident geodesic shooting, landmark flow, and the discrete
-// adjoint (backward pass) of the shoot + flow scheme, recomputed from checkpoints.
+// adjoint (backward pass) of the shoot + flow scheme, recomputed from checkpoints; for one
+// subject or for a ragged batch of subjects in the same launches.
 //
 // Forward (Diffeos.cxx:589-710), all state in HBM, every kernel sum one pair-sum launch:
 //   shoot  z = (p, a):  F(z) = (K(p,p) a, -g(p, a)),  g_i = sum_j (a_i.a_j) grad_1 K(p_i, p_j)
@@ -18,6 +19,15 @@
 //     one fused OP_CONVGRAD1 pair sum
 // The forward stores (p, a, x) every k-th step; the backward recomputes each segment of
 // at most k steps from its checkpoint and walks it in reverse.
+//
+// Batches (the per-subject loop of AbstractAtlas::ComputeDataTermGradient,
+// AbstractAtlas.cxx:490-560): subject s owns control points [cp[s], cp[s + 1]) and
+// landmarks [lm[s], lm[s + 1]) of concatenated arrays; trajectories are time-major,
+// [T][all points][d].  Every pair sum of a step covers all subjects in one launch
+// (grid.z = subject), every elementwise update is one launch over all subjects, and the
+// zero-momenta exit is decided per subject.  Batched calls launch kernels only (no
+// allocation, memset, host copy or synchronisation), so a caller may capture them in a
+// HIP graph and replay it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,15 +42,22 @@ namespace {
 // |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
 constexpr double kZeroMomenta2 = 1e-40;
 
-// out = ca a + cb b + cc c (b, c may be null).  With a gate (squared norm of the initial
-// momenta) below kZeroMomenta2, out = a, or 0 when gate_zero is set.
+// out = ca a + cb b + cc c (b, c may be null) over segment blockIdx.y of off (in points of
+// D coordinates; the flat range [0, n) without offsets).  With a gate (squared norm of the
+// segment's initial momenta) below kZeroMomenta2, out = a, or 0 when gate_zero is set.
 __global__ __launch_bounds__(256) void k_lin(double *out, const double *a, double ca,
                                              const double *b, double cb, const double *c,
                                              double cc, long n, const double *gate,
-                                             int gate_zero) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (gate && *gate < kZeroMomenta2) {
+                                             int gate_zero, const long *off, int D) {
+  const int seg = blockIdx.y;
+  long lo = 0, hi = n;
+  if (off) {
+    lo = off[seg] * D;
+    hi = off[seg + 1] * D;
+  }
+  const long i = lo + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  if (gate && gate[seg] < kZeroMomenta2) {
     out[i] = gate_zero ? 0.0 : a[i];
     return;
   }
@@ -50,18 +67,25 @@ __global__ __launch_bounds__(256) void k_lin(double *out, const double *a, doubl
   out[i] = v;
 }
 
-// *out = sum a_i^2 (one workgroup)
-__global__ __launch_bounds__(256) void k_sumsq(const double *a, long n, double *out) {
+// out[s] = sum of a_i^2 over segment s (one workgroup per segment)
+__global__ __launch_bounds__(256) void k_sumsq(const double *a, long n, const long *off, int D,
+                                               double *out) {
   __shared__ double sh[256];
+  const int seg = blockIdx.x;
+  long lo = 0, hi = n;
+  if (off) {
+    lo = off[seg] * D;
+    hi = off[seg + 1] * D;
+  }
   double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += 256) s += a[i] * a[i];
+  for (long i = lo + threadIdx.x; i < hi; i += 256) s += a[i] * a[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sh[0];
+  if (threadIdx.x == 0) out[seg] = sh[0];
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -74,6 +98,11 @@ struct Arena {
   double *take(size_t count) {
     double *p = base ? (double *)(base + off) : nullptr;
     off += align256(count * sizeof(double));
+    return p;
+  }
+  long *take_longs(size_t count) {
+    long *p = base ? (long *)(base + off) : nullptr;
+    off += align256(count * sizeof(long));
     return p;
   }
   void *take_bytes(size_t bytes) {
@@ -90,6 +119,7 @@ struct Ctx {
   int prec;
   void *pw;      // pair-sum workspace
   size_t pw_bytes;
+  bool batch;    // capture-safe batched pair sums (no Morton sort)
 };
 
 int check_launch(const char *what) {
@@ -97,16 +127,36 @@ int check_launch(const char *what) {
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// flat out = ca a + cb b + cc c over n doubles
 int lin(const Ctx &cx, double *out, const double *a, double ca, const double *b, double cb,
-        const double *c, double cc, long n, const double *gate = nullptr, int gate_zero = 0) {
+        const double *c, double cc, long n) {
   if (n <= 0) return DFM_OK;
   hipLaunchKernelGGL(k_lin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.st, out, a, ca, b,
-                     cb, c, cc, n, gate, gate_zero);
+                     cb, c, cc, n, nullptr, 0, nullptr, cx.D);
+  return check_launch("k_lin");
+}
+
+// the same, gated per segment of sg by the segment's initial momenta
+int glin(const Ctx &cx, const Seg &sg, double *out, const double *a, double ca, const double *b,
+         double cb, const double *c, double cc, const double *gate, int gate_zero) {
+  if (sg.total <= 0) return DFM_OK;
+  const bool segs = sg.n > 1;
+  const long len = (segs ? sg.max : sg.total) * cx.D;
+  if (len <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_lin, dim3((unsigned)((len + 255) / 256), (unsigned)sg.n), dim3(256), 0,
+                     cx.st, out, a, ca, b, cb, c, cc, sg.total * cx.D, gate, gate_zero,
+                     segs ? sg.d : nullptr, cx.D);
   return check_launch("k_lin");
 }
 
 int copy(const Ctx &cx, double *out, const double *a, long n) {
   return lin(cx, out, a, 1.0, nullptr, 0.0, nullptr, 0.0, n);
+}
+
+int norm2(const Ctx &cx, const double *a, const Seg &sg, double *out) {
+  hipLaunchKernelGGL(k_sumsq, dim3((unsigned)sg.n), dim3(256), 0, cx.st, a, sg.total * cx.D,
+                     sg.n > 1 ? sg.d : nullptr, cx.D, out);
+  return check_launch("k_sumsq");
 }
 
 KParams blank() {
@@ -115,38 +165,55 @@ KParams blank() {
   return p;
 }
 
+int ps(const Ctx &cx, int op, KParams p, const Seg &xs, const Seg &ys, long o0, long o1) {
+  if (cx.batch)
+    return pairsum_batch(cx.st, op, cx.prec, cx.D, cx.D, cx.sigma, p, xs, ys, o0, o1, cx.pw,
+                         cx.pw_bytes);
+  p.nx = xs.total;
+  p.ny = ys.total;
+  return pairsum(cx.st, op, cx.prec, cx.D, cx.D, cx.sigma, p, o0, o1, cx.pw, cx.pw_bytes);
+}
+
 // conv (nx x D) and/or grad (nx x D) of sum_j K(x_i, y_j) w_j with alpha (nx x D)
-int conv_grad1(const Ctx &cx, const double *x, const double *alpha, long nx, const double *y,
-               const double *w, long ny, double *conv, double *grad) {
+int conv_grad1(const Ctx &cx, const double *x, const double *alpha, const Seg &xs,
+               const double *y, const double *w, const Seg &ys, double *conv, double *grad) {
   KParams p = blank();
-  p.x = x; p.ta = alpha; p.y = y; p.w = w; p.nx = nx; p.ny = ny;
+  p.x = x; p.ta = alpha; p.y = y; p.w = w;
+  const long n = xs.total * cx.D;
   if (conv && grad) {
     p.out0 = conv; p.out1 = grad;
-    return pairsum(cx.st, OP_CONVGRAD1, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, nx * cx.D,
-                   cx.pw, cx.pw_bytes);
+    return ps(cx, OP_CONVGRAD1, p, xs, ys, n, n);
   }
   if (conv) {
     p.out0 = conv;
-    return pairsum(cx.st, OP_CONV, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, 0, cx.pw,
-                   cx.pw_bytes);
+    return ps(cx, OP_CONV, p, xs, ys, n, 0);
   }
   p.out0 = grad;
-  return pairsum(cx.st, OP_GRAD1, cx.prec, cx.D, cx.D, cx.sigma, p, nx * cx.D, 0, cx.pw,
-                 cx.pw_bytes);
+  return ps(cx, OP_GRAD1, p, xs, ys, n, 0);
 }
 
 int shoot_vjp(const Ctx &cx, const double *p_, const double *a, const double *mp,
-              const double *ma, long n, double *jp, double *ja) {
+              const double *ma, const Seg &cs, double *jp, double *ja) {
   KParams p = blank();
   p.x = p_; p.ta = a; p.tb = mp; p.tc = ma;
   p.y = p_; p.w = a; p.se = mp; p.se2 = ma;
-  p.out0 = jp; p.out1 = ja; p.nx = n; p.ny = n;
-  return pairsum(cx.st, OP_SHOOTVJP, cx.prec, cx.D, cx.D, cx.sigma, p, n * cx.D, n * cx.D, cx.pw,
-                 cx.pw_bytes);
+  p.out0 = jp; p.out1 = ja;
+  return ps(cx, OP_SHOOTVJP, p, cs, cs, cs.total * cx.D, cs.total * cx.D);
 }
 
-size_t pair_ws(int prec, int D, long n, long m) {
+size_t pair_ws(int prec, int D, const Seg &cs, const Seg &ls, bool batch) {
   size_t w = 0;
+  if (batch) {
+    w = std::max(w, pairsum_batch_workspace(OP_CONVGRAD1, prec, D, D, cs, cs));
+    w = std::max(w, pairsum_batch_workspace(OP_SHOOTVJP, prec, D, D, cs, cs));
+    if (ls.total > 0) {
+      w = std::max(w, pairsum_batch_workspace(OP_CONV, prec, D, D, ls, cs));
+      w = std::max(w, pairsum_batch_workspace(OP_GRAD1, prec, D, D, ls, cs));
+      w = std::max(w, pairsum_batch_workspace(OP_CONVGRAD1, prec, D, D, cs, ls));
+    }
+    return w;
+  }
+  const long n = cs.total, m = ls.total;
   w = std::max(w, pairsum_workspace(OP_CONVGRAD1, prec, D, D, n, n));
   w = std::max(w, pairsum_workspace(OP_SHOOTVJP, prec, D, D, n, n));
   if (m > 0) {
@@ -168,20 +235,20 @@ struct ShootTmp {
 };
 
 // (p, a) -> (p1, a1) by one Euler or Heun step.
-int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, long n, double dt,
-               double *p1, double *a1, const ShootTmp &t, const double *gate) {
-  const long nd = n * cx.D;
-  int rc = conv_grad1(cx, p, a, n, p, a, n, t.dpos, t.dmom);
+int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, const Seg &cs,
+               double dt, double *p1, double *a1, const ShootTmp &t, const double *gate) {
+  const long nd = cs.total * cx.D;
+  int rc = conv_grad1(cx, p, a, cs, p, a, cs, t.dpos, t.dmom);
   if (rc) return rc;
   if (scheme == DFM_EULER) {
-    if ((rc = lin(cx, p1, p, 1.0, t.dpos, dt, nullptr, 0.0, nd, gate, 0))) return rc;
-    return lin(cx, a1, a, 1.0, t.dmom, -dt, nullptr, 0.0, nd, gate, 1);
+    if ((rc = glin(cx, cs, p1, p, 1.0, t.dpos, dt, nullptr, 0.0, gate, 0))) return rc;
+    return glin(cx, cs, a1, a, 1.0, t.dmom, -dt, nullptr, 0.0, gate, 1);
   }
   if ((rc = lin(cx, t.ps, p, 1.0, t.dpos, dt, nullptr, 0.0, nd))) return rc;
   if ((rc = lin(cx, t.as, a, 1.0, t.dmom, -dt, nullptr, 0.0, nd))) return rc;
-  if ((rc = conv_grad1(cx, t.ps, t.as, n, t.ps, t.as, n, t.dpos2, t.dmom2))) return rc;
-  if ((rc = lin(cx, p1, p, 1.0, t.dpos, 0.5 * dt, t.dpos2, 0.5 * dt, nd, gate, 0))) return rc;
-  return lin(cx, a1, a, 1.0, t.dmom, -0.5 * dt, t.dmom2, -0.5 * dt, nd, gate, 1);
+  if ((rc = conv_grad1(cx, t.ps, t.as, cs, t.ps, t.as, cs, t.dpos2, t.dmom2))) return rc;
+  if ((rc = glin(cx, cs, p1, p, 1.0, t.dpos, 0.5 * dt, t.dpos2, 0.5 * dt, gate, 0))) return rc;
+  return glin(cx, cs, a1, a, 1.0, t.dmom, -0.5 * dt, t.dmom2, -0.5 * dt, gate, 1);
 }
 
 struct FlowTmp {
@@ -191,20 +258,15 @@ struct FlowTmp {
 
 // x -> x1 in the flow from (p, a) to (p1, a1).
 int flow_step(const Ctx &cx, int improved, const double *p, const double *a, const double *p1,
-              const double *a1, long n, const double *x, long m, double dt, double *x1,
-              const FlowTmp &t, const double *gate) {
-  const long md = m * cx.D;
-  int rc = conv_grad1(cx, x, nullptr, m, p, a, n, t.v1, nullptr);
+              const double *a1, const Seg &cs, const double *x, const Seg &ls, double dt,
+              double *x1, const FlowTmp &t, const double *gate) {
+  const long md = ls.total * cx.D;
+  int rc = conv_grad1(cx, x, nullptr, ls, p, a, cs, t.v1, nullptr);
   if (rc) return rc;
-  if (!improved) return lin(cx, x1, x, 1.0, t.v1, dt, nullptr, 0.0, md, gate, 0);
+  if (!improved) return glin(cx, ls, x1, x, 1.0, t.v1, dt, nullptr, 0.0, gate, 0);
   if ((rc = lin(cx, t.xs, x, 1.0, t.v1, dt, nullptr, 0.0, md))) return rc;
-  if ((rc = conv_grad1(cx, t.xs, nullptr, m, p1, a1, n, t.v2, nullptr))) return rc;
-  return lin(cx, x1, x, 1.0, t.v1, 0.5 * dt, t.v2, 0.5 * dt, md, gate, 0);
-}
-
-int norm2(const Ctx &cx, const double *a, long n, double *out) {
-  hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, cx.st, a, n, out);
-  return check_launch("k_sumsq");
+  if ((rc = conv_grad1(cx, t.xs, nullptr, ls, p1, a1, cs, t.v2, nullptr))) return rc;
+  return glin(cx, ls, x1, x, 1.0, t.v1, 0.5 * dt, t.v2, 0.5 * dt, gate, 0);
 }
 
 int check_common(int D, int prec, double sigma, long n, long m, int T) {
@@ -220,33 +282,44 @@ int check_common(int D, int prec, double sigma, long n, long m, int T) {
 
 // ---- layouts (shared by the workspace queries and the drivers) ----------------------
 
-struct ShootLayout {
+// device copies of the segment offsets and the per-segment zero-momenta gates
+struct SegArrays {
+  long *cp, *lm;
   double *gate;
+  void take(Arena &ar, int nseg) {
+    cp = ar.take_longs(nseg + 1);
+    lm = ar.take_longs(nseg + 1);
+    gate = ar.take(nseg);
+  }
+};
+
+struct ShootLayout {
+  SegArrays sa;
   ShootTmp t;
   void *pw;
   size_t pw_bytes;
   size_t total;
-  ShootLayout(void *base, int prec, int D, long n) {
+  ShootLayout(void *base, int prec, int D, const Seg &cs, bool batch) {
     Arena ar(base);
-    gate = ar.take(1);
-    t.take(ar, n * D);
-    pw_bytes = pair_ws(prec, D, n, 0);
+    sa.take(ar, cs.n);
+    t.take(ar, cs.total * D);
+    pw_bytes = pair_ws(prec, D, cs, Seg::single(0), batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
   }
 };
 
 struct FlowLayout {
-  double *gate;
+  SegArrays sa;
   FlowTmp t;
   void *pw;
   size_t pw_bytes;
   size_t total;
-  FlowLayout(void *base, int prec, int D, long n, long m) {
+  FlowLayout(void *base, int prec, int D, const Seg &cs, const Seg &ls, bool batch) {
     Arena ar(base);
-    gate = ar.take(1);
-    t.take(ar, m * D);
-    pw_bytes = pair_ws(prec, D, n, m);
+    sa.take(ar, cs.n);
+    t.take(ar, ls.total * D);
+    pw_bytes = pair_ws(prec, D, cs, ls, batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
   }
@@ -254,7 +327,7 @@ struct FlowLayout {
 
 struct AdjointLayout {
   int segs, k;
-  double *gate;
+  SegArrays sa;
   double *ckpt;   // segs x (2n + m) x D
   double *seg;    // (k + 1) x (2n + m) x D
   ShootTmp st;
@@ -264,12 +337,14 @@ struct AdjointLayout {
   void *pw;
   size_t pw_bytes;
   size_t total;
-  AdjointLayout(void *base, int prec, int D, long n, long m, int T, int kk) {
+  AdjointLayout(void *base, int prec, int D, const Seg &cs, const Seg &ls, int T, int kk,
+                bool batch) {
     Arena ar(base);
+    const long n = cs.total, m = ls.total;
     k = std::max(1, kk);
     segs = T > 1 ? (T - 1 + k - 1) / k : 0;
     const long rec = (2 * n + m) * D;
-    gate = ar.take(1);
+    sa.take(ar, cs.n);
     ckpt = ar.take((size_t)std::max(segs, 1) * rec);
     seg = ar.take((size_t)(std::min(k, std::max(T - 1, 0)) + 1) * rec);
     st.take(ar, n * D);
@@ -278,127 +353,140 @@ struct AdjointLayout {
     for (double **v : vn) *v = ar.take(n * D);
     double **vm[] = {&th, &thn, &ths, &mu};
     for (double **v : vm) *v = ar.take(m * D);
-    pw_bytes = pair_ws(prec, D, n, m);
+    pw_bytes = pair_ws(prec, D, cs, ls, batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
   }
 };
 
-}  // namespace
-}  // namespace dfm
-
-using namespace dfm;
-
-extern "C" {
-
-size_t dfm_shoot_workspace(int dimpoint, long n, int precision) {
-  if (check_common(dimpoint, precision, 1.0, n, 0, 1)) return 0;
-  return ShootLayout(nullptr, precision, dimpoint, n).total;
+// segment descriptors of a call (nseg == 0: one subject with plain counts n, m)
+int segs_of(int nseg, const long *cp_h, const long *lm_h, long n, long m, Seg &cs, Seg &ls) {
+  if (nseg == 0) {
+    cs = Seg::single(n);
+    ls = Seg::single(m);
+    return DFM_OK;
+  }
+  int rc = seg_from_host(cs, nseg, cp_h, nullptr);
+  if (rc) return rc;
+  if (lm_h) return seg_from_host(ls, nseg, lm_h, nullptr);
+  ls = Seg::single(0);
+  ls.n = nseg;
+  return DFM_OK;
 }
 
-int dfm_shoot(void *stream, int D, double sigma, int prec, int scheme, const double *p0,
-              const double *a0, long n, int T, double t0, double tN, double *P, double *A,
-              void *work, size_t work_bytes) {
-  clear_error();
-  int rc = check_common(D, prec, sigma, n, 0, T);
+// device copies of the offsets, written into the workspace by kernels
+int upload_segs(const Ctx &cx, Seg &cs, Seg &ls, const SegArrays &sa) {
+  if (cs.n <= 1) return DFM_OK;
+  int rc = put_longs(cx.st, sa.cp, cs.h, cs.n + 1);
   if (rc) return rc;
+  cs.d = sa.cp;
+  if (ls.h) {
+    if ((rc = put_longs(cx.st, sa.lm, ls.h, ls.n + 1))) return rc;
+    ls.d = sa.lm;
+  }
+  return DFM_OK;
+}
+
+// ---- drivers (nseg == 0: one subject through the sorted single-call pair sums) ---------
+
+int shoot_impl(void *stream, int D, double sigma, int prec, int scheme, int nseg,
+               const long *cp_off, const double *p0, const double *a0, long n, int T,
+               double t0, double tN, double *P, double *A, void *work, size_t work_bytes) {
+  clear_error();
+  Seg cs, ls;
+  int rc = segs_of(nseg, cp_off, nullptr, n, 0, cs, ls);
+  if (rc) return rc;
+  if ((rc = check_common(D, prec, sigma, cs.total, 0, T))) return rc;
   if (scheme != DFM_EULER && scheme != DFM_HEUN) return set_error(DFM_ERR_ARG, "unknown scheme %d", scheme);
-  if (n == 0) return DFM_OK;
+  if (cs.total == 0) return DFM_OK;
   if (!p0 || !a0 || !P || !A) return set_error(DFM_ERR_ARG, "NULL argument");
-  ShootLayout need(nullptr, prec, D, n);
+  const bool batch = nseg > 0;
+  ShootLayout need(nullptr, prec, D, cs, batch);
   if (!work || work_bytes < need.total)
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
-  ShootLayout L(work, prec, D, n);
-  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
-  const long nd = n * D;
-  if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, nd, L.gate)))
+  ShootLayout L(work, prec, D, cs, batch);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
+  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
+  const long nd = cs.total * D;
+  if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, cs, L.sa.gate)))
     return rc;
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   for (int t = 0; t + 1 < T; t++)
-    if ((rc = shoot_step(cx, scheme, P + t * nd, A + t * nd, n, dt, P + (t + 1) * nd,
-                         A + (t + 1) * nd, L.t, L.gate)))
+    if ((rc = shoot_step(cx, scheme, P + t * nd, A + t * nd, cs, dt, P + (t + 1) * nd,
+                         A + (t + 1) * nd, L.t, L.sa.gate)))
       return rc;
   return DFM_OK;
 }
 
-size_t dfm_flow_workspace(int dimpoint, long n, long m, int precision) {
-  if (check_common(dimpoint, precision, 1.0, n, m, 1)) return 0;
-  return FlowLayout(nullptr, precision, dimpoint, n, m).total;
-}
-
-int dfm_flow(void *stream, int D, double sigma, int prec, int improved_euler, const double *P,
-             const double *A, long n, const double *x0, long m, int T, double t0, double tN,
-             double *X, void *work, size_t work_bytes) {
+int flow_impl(void *stream, int D, double sigma, int prec, int improved_euler, int nseg,
+              const long *cp_off, const long *lm_off, const double *P, const double *A, long n,
+              const double *x0, long m, int T, double t0, double tN, double *X, void *work,
+              size_t work_bytes) {
   clear_error();
-  int rc = check_common(D, prec, sigma, n, m, T);
+  if (nseg > 0 && !lm_off) return set_error(DFM_ERR_ARG, "landmark offsets are NULL");
+  Seg cs, ls;
+  int rc = segs_of(nseg, cp_off, lm_off, n, m, cs, ls);
   if (rc) return rc;
-  if (m == 0) return DFM_OK;
-  if (!x0 || !X || (n > 0 && (!P || !A))) return set_error(DFM_ERR_ARG, "NULL argument");
-  FlowLayout need(nullptr, prec, D, n, m);
+  if ((rc = check_common(D, prec, sigma, cs.total, ls.total, T))) return rc;
+  if (ls.total == 0) return DFM_OK;
+  if (!x0 || !X || (cs.total > 0 && (!P || !A))) return set_error(DFM_ERR_ARG, "NULL argument");
+  const bool batch = nseg > 0;
+  FlowLayout need(nullptr, prec, D, cs, ls, batch);
   if (!work || work_bytes < need.total)
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
-  FlowLayout L(work, prec, D, n, m);
-  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
-  const long nd = n * D, md = m * D;
+  FlowLayout L(work, prec, D, cs, ls, batch);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
+  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
+  const long nd = cs.total * D, md = ls.total * D;
   if ((rc = copy(cx, X, x0, md))) return rc;
-  if (n == 0) {
+  if (cs.total == 0) {
     for (int t = 1; t < T; t++)
       if ((rc = copy(cx, X + t * md, x0, md))) return rc;
     return DFM_OK;
   }
-  if ((rc = norm2(cx, A, nd, L.gate))) return rc;
+  if ((rc = norm2(cx, A, cs, L.sa.gate))) return rc;
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   for (int t = 0; t + 1 < T; t++)
     if ((rc = flow_step(cx, improved_euler, P + t * nd, A + t * nd, P + (t + 1) * nd,
-                        A + (t + 1) * nd, n, X + t * md, m, dt, X + (t + 1) * md, L.t, L.gate)))
+                        A + (t + 1) * nd, cs, X + t * md, ls, dt, X + (t + 1) * md, L.t,
+                        L.sa.gate)))
       return rc;
   return DFM_OK;
 }
 
-size_t dfm_adjoint_workspace(int dimpoint, long n, long m, int T, int checkpoint_every,
-                             int precision) {
-  if (check_common(dimpoint, precision, 1.0, n, m, T) || checkpoint_every < 1) return 0;
-  return AdjointLayout(nullptr, precision, dimpoint, n, m, T, checkpoint_every).total;
-}
-
-int dfm_shoot_vjp(void *stream, int D, double sigma, int prec, const double *p, const double *a,
-                  const double *mu_p, const double *mu_a, long n, double *jp, double *ja,
-                  void *work, size_t work_bytes) {
+int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int improved_euler,
+                 int nseg, const long *cp_off, const long *lm_off, const double *p0,
+                 const double *a0, long n_, const double *x0, long m_, int T, double t0,
+                 double tN, int checkpoint_every, const double *theta_T, double *grad_p0,
+                 double *grad_a0, double *grad_x0, void *work, size_t work_bytes) {
   clear_error();
-  int rc = check_common(D, prec, sigma, n, 0, 1);
+  if (nseg > 0 && !lm_off) return set_error(DFM_ERR_ARG, "landmark offsets are NULL");
+  Seg cs, ls;
+  int rc = segs_of(nseg, cp_off, lm_off, n_, m_, cs, ls);
   if (rc) return rc;
-  if (n == 0) return DFM_OK;
-  if (!p || !a || !mu_p || !mu_a || !jp || !ja) return set_error(DFM_ERR_ARG, "NULL argument");
-  Ctx cx{(hipStream_t)stream, D, sigma, prec, work, work_bytes};
-  return shoot_vjp(cx, p, a, mu_p, mu_a, n, jp, ja);
-}
-
-int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int scheme,
-                           int improved_euler, const double *p0, const double *a0, long n,
-                           const double *x0, long m, int T, double t0, double tN,
-                           int checkpoint_every, const double *theta_T, double *grad_p0,
-                           double *grad_a0, double *grad_x0, void *work, size_t work_bytes) {
-  clear_error();
-  int rc = check_common(D, prec, sigma, n, m, T);
-  if (rc) return rc;
+  if ((rc = check_common(D, prec, sigma, cs.total, ls.total, T))) return rc;
   if (scheme != DFM_EULER && scheme != DFM_HEUN) return set_error(DFM_ERR_ARG, "unknown scheme %d", scheme);
   if (checkpoint_every < 1) return set_error(DFM_ERR_ARG, "checkpoint interval must be >= 1");
+  const long n = cs.total, m = ls.total;
   if (n == 0) return DFM_OK;
   if (!p0 || !a0 || !grad_p0 || !grad_a0 || (m > 0 && (!x0 || !theta_T)))
     return set_error(DFM_ERR_ARG, "NULL argument");
-  AdjointLayout need(nullptr, prec, D, n, m, T, checkpoint_every);
+  const bool batch = nseg > 0;
+  AdjointLayout need(nullptr, prec, D, cs, ls, T, checkpoint_every, batch);
   if (!work || work_bytes < need.total)
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
-  AdjointLayout L(work, prec, D, n, m, T, checkpoint_every);
-  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes};
+  AdjointLayout L(work, prec, D, cs, ls, T, checkpoint_every, batch);
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
+  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
   const long nd = n * D, md = m * D, rec = 2 * nd + md;
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
+  double *gate = L.sa.gate;
   // state record layout: [p (nd) | a (nd) | x (md)]
   auto P_ = [&](double *r) { return r; };
   auto A_ = [&](double *r) { return r + nd; };
   auto X_ = [&](double *r) { return r + 2 * nd; };
 
-  if ((rc = norm2(cx, a0, nd, L.gate))) return rc;
+  if ((rc = norm2(cx, a0, cs, gate))) return rc;
   // terminal cotangents
   if ((rc = lin(cx, L.lp, p0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
   if ((rc = lin(cx, L.la, a0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
@@ -410,10 +498,10 @@ int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int sche
   if (m > 0 && (rc = copy(cx, X_(cur), x0, md))) return rc;
   for (int t = 0; t + 1 < T; t++) {
     if (t % L.k == 0 && (rc = copy(cx, L.ckpt + (size_t)(t / L.k) * rec, cur, rec))) return rc;
-    if ((rc = shoot_step(cx, scheme, P_(cur), A_(cur), n, dt, P_(nxt), A_(nxt), L.st, L.gate)))
+    if ((rc = shoot_step(cx, scheme, P_(cur), A_(cur), cs, dt, P_(nxt), A_(nxt), L.st, gate)))
       return rc;
-    if (m > 0 && (rc = flow_step(cx, improved_euler, P_(cur), A_(cur), P_(nxt), A_(nxt), n, X_(cur),
-                                 m, dt, X_(nxt), L.ft, L.gate)))
+    if (m > 0 && (rc = flow_step(cx, improved_euler, P_(cur), A_(cur), P_(nxt), A_(nxt), cs,
+                                 X_(cur), ls, dt, X_(nxt), L.ft, gate)))
       return rc;
     std::swap(cur, nxt);
   }
@@ -425,24 +513,24 @@ int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int sche
     if ((rc = copy(cx, S, L.ckpt + (size_t)s * rec, rec))) return rc;
     for (int t = tb; t < te; t++) {
       double *a_ = S + (size_t)(t - tb) * rec, *b_ = S + (size_t)(t - tb + 1) * rec;
-      if ((rc = shoot_step(cx, scheme, P_(a_), A_(a_), n, dt, P_(b_), A_(b_), L.st, L.gate)))
+      if ((rc = shoot_step(cx, scheme, P_(a_), A_(a_), cs, dt, P_(b_), A_(b_), L.st, gate)))
         return rc;
-      if (m > 0 && (rc = flow_step(cx, improved_euler, P_(a_), A_(a_), P_(b_), A_(b_), n, X_(a_), m,
-                                   dt, X_(b_), L.ft, L.gate)))
+      if (m > 0 && (rc = flow_step(cx, improved_euler, P_(a_), A_(a_), P_(b_), A_(b_), cs, X_(a_),
+                                   ls, dt, X_(b_), L.ft, gate)))
         return rc;
     }
     for (int t = te - 1; t >= tb; t--) {
       double *zt = S + (size_t)(t - tb) * rec, *z1 = S + (size_t)(t - tb + 1) * rec;
       // flow step t: cotangent th of x_{t+1} -> thn of x_t, plus its pull onto z_t, z_{t+1}
       if (m > 0) {
-        if ((rc = conv_grad1(cx, X_(zt), nullptr, m, P_(zt), A_(zt), n, L.ft.v1, nullptr))) return rc;
+        if ((rc = conv_grad1(cx, X_(zt), nullptr, ls, P_(zt), A_(zt), cs, L.ft.v1, nullptr))) return rc;
         if (improved_euler) {
           if ((rc = lin(cx, L.ft.xs, X_(zt), 1.0, L.ft.v1, dt, nullptr, 0.0, md))) return rc;
           // ths = dt/2 J_x v(xs; z_{t+1})^T th
-          if ((rc = conv_grad1(cx, L.ft.xs, L.th, m, P_(z1), A_(z1), n, nullptr, L.ths))) return rc;
+          if ((rc = conv_grad1(cx, L.ft.xs, L.th, ls, P_(z1), A_(z1), cs, nullptr, L.ths))) return rc;
           if ((rc = lin(cx, L.ths, L.ths, 0.5 * dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
           // lambda(z_{t+1}) += dt/2 J_z v(xs; z_{t+1})^T th
-          if ((rc = conv_grad1(cx, P_(z1), A_(z1), n, L.ft.xs, L.th, m, L.laf, L.lpf))) return rc;
+          if ((rc = conv_grad1(cx, P_(z1), A_(z1), cs, L.ft.xs, L.th, ls, L.laf, L.lpf))) return rc;
           if ((rc = lin(cx, L.lp, L.lp, 1.0, L.lpf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
           if ((rc = lin(cx, L.la, L.la, 1.0, L.laf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
           // mu = dt/2 th + dt ths
@@ -451,28 +539,28 @@ int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int sche
           if ((rc = lin(cx, L.mu, L.th, dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
         }
         // thn = th (+ ths) + J_x v(x_t; z_t)^T mu
-        if ((rc = conv_grad1(cx, X_(zt), L.mu, m, P_(zt), A_(zt), n, nullptr, L.thn))) return rc;
+        if ((rc = conv_grad1(cx, X_(zt), L.mu, ls, P_(zt), A_(zt), cs, nullptr, L.thn))) return rc;
         if ((rc = lin(cx, L.thn, L.thn, 1.0, L.th, 1.0, improved_euler ? L.ths : nullptr, 1.0, md)))
           return rc;
         // flow pull onto z_t: J_z v(x_t; z_t)^T mu
-        if ((rc = conv_grad1(cx, P_(zt), A_(zt), n, X_(zt), L.mu, m, L.laf, L.lpf))) return rc;
+        if ((rc = conv_grad1(cx, P_(zt), A_(zt), cs, X_(zt), L.mu, ls, L.laf, L.lpf))) return rc;
       }
       // shoot step t: cotangent (lp, la) of z_{t+1} -> (lpn, lan) of z_t
       if (scheme == DFM_EULER) {
-        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.lp, L.la, n, L.jp, L.ja))) return rc;
+        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.lp, L.la, cs, L.jp, L.ja))) return rc;
         if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jp, dt, nullptr, 0.0, nd))) return rc;
         if ((rc = lin(cx, L.lan, L.la, 1.0, L.ja, dt, nullptr, 0.0, nd))) return rc;
       } else {
         // z* = z_t + dt F(z_t)
-        if ((rc = conv_grad1(cx, P_(zt), A_(zt), n, P_(zt), A_(zt), n, L.st.dpos, L.st.dmom))) return rc;
+        if ((rc = conv_grad1(cx, P_(zt), A_(zt), cs, P_(zt), A_(zt), cs, L.st.dpos, L.st.dmom))) return rc;
         if ((rc = lin(cx, L.st.ps, P_(zt), 1.0, L.st.dpos, dt, nullptr, 0.0, nd))) return rc;
         if ((rc = lin(cx, L.st.as, A_(zt), 1.0, L.st.dmom, -dt, nullptr, 0.0, nd))) return rc;
         // l* = dt/2 J_F(z*)^T l
-        if ((rc = shoot_vjp(cx, L.st.ps, L.st.as, L.lp, L.la, n, L.jsp, L.jsa))) return rc;
+        if ((rc = shoot_vjp(cx, L.st.ps, L.st.as, L.lp, L.la, cs, L.jsp, L.jsa))) return rc;
         // nu = dt/2 l + dt l* = dt/2 l + dt^2/2 J_F(z*)^T l
         if ((rc = lin(cx, L.nup, L.lp, 0.5 * dt, L.jsp, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
         if ((rc = lin(cx, L.nua, L.la, 0.5 * dt, L.jsa, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
-        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.nup, L.nua, n, L.jp, L.ja))) return rc;
+        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.nup, L.nua, cs, L.jp, L.ja))) return rc;
         // lpn = l + l* + J_F(z_t)^T nu
         if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jsp, 0.5 * dt, L.jp, 1.0, nd))) return rc;
         if ((rc = lin(cx, L.lan, L.la, 1.0, L.jsa, 0.5 * dt, L.ja, 1.0, nd))) return rc;
@@ -490,6 +578,126 @@ int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int sche
   if ((rc = copy(cx, grad_p0, L.lp, nd)) || (rc = copy(cx, grad_a0, L.la, nd))) return rc;
   if (grad_x0 && m > 0 && (rc = copy(cx, grad_x0, L.th, md))) return rc;
   return DFM_OK;
+}
+
+// which: 0 shoot, 1 flow, 2 adjoint (checkpoint interval k)
+size_t workspace_or_zero(int nseg, const long *cp_off, const long *lm_off, long n, long m, int D,
+                         int prec, int T, int which, int k) {
+  Seg cs, ls;
+  if (segs_of(nseg, cp_off, lm_off, n, m, cs, ls)) return 0;
+  if (check_common(D, prec, 1.0, cs.total, ls.total, T)) return 0;
+  const bool batch = nseg > 0;
+  if (which == 0) return ShootLayout(nullptr, prec, D, cs, batch).total;
+  if (which == 1) return FlowLayout(nullptr, prec, D, cs, ls, batch).total;
+  if (k < 1) return 0;
+  return AdjointLayout(nullptr, prec, D, cs, ls, T, k, batch).total;
+}
+
+}  // namespace
+}  // namespace dfm
+
+using namespace dfm;
+
+extern "C" {
+
+size_t dfm_shoot_workspace(int dimpoint, long n, int precision) {
+  return workspace_or_zero(0, nullptr, nullptr, n, 0, dimpoint, precision, 1, 0, 1);
+}
+
+int dfm_shoot(void *stream, int D, double sigma, int prec, int scheme, const double *p0,
+              const double *a0, long n, int T, double t0, double tN, double *P, double *A,
+              void *work, size_t work_bytes) {
+  return shoot_impl(stream, D, sigma, prec, scheme, 0, nullptr, p0, a0, n, T, t0, tN, P, A, work,
+                    work_bytes);
+}
+
+size_t dfm_flow_workspace(int dimpoint, long n, long m, int precision) {
+  return workspace_or_zero(0, nullptr, nullptr, n, m, dimpoint, precision, 1, 1, 1);
+}
+
+int dfm_flow(void *stream, int D, double sigma, int prec, int improved_euler, const double *P,
+             const double *A, long n, const double *x0, long m, int T, double t0, double tN,
+             double *X, void *work, size_t work_bytes) {
+  return flow_impl(stream, D, sigma, prec, improved_euler, 0, nullptr, nullptr, P, A, n, x0, m, T,
+                   t0, tN, X, work, work_bytes);
+}
+
+size_t dfm_adjoint_workspace(int dimpoint, long n, long m, int T, int checkpoint_every,
+                             int precision) {
+  return workspace_or_zero(0, nullptr, nullptr, n, m, dimpoint, precision, T, 2, checkpoint_every);
+}
+
+int dfm_shoot_vjp(void *stream, int D, double sigma, int prec, const double *p, const double *a,
+                  const double *mu_p, const double *mu_a, long n, double *jp, double *ja,
+                  void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_common(D, prec, sigma, n, 0, 1);
+  if (rc) return rc;
+  if (n == 0) return DFM_OK;
+  if (!p || !a || !mu_p || !mu_a || !jp || !ja) return set_error(DFM_ERR_ARG, "NULL argument");
+  Ctx cx{(hipStream_t)stream, D, sigma, prec, work, work_bytes, false};
+  return shoot_vjp(cx, p, a, mu_p, mu_a, Seg::single(n), jp, ja);
+}
+
+int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int scheme,
+                           int improved_euler, const double *p0, const double *a0, long n,
+                           const double *x0, long m, int T, double t0, double tN,
+                           int checkpoint_every, const double *theta_T, double *grad_p0,
+                           double *grad_a0, double *grad_x0, void *work, size_t work_bytes) {
+  return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, 0, nullptr, nullptr, p0, a0,
+                      n, x0, m, T, t0, tN, checkpoint_every, theta_T, grad_p0, grad_a0, grad_x0,
+                      work, work_bytes);
+}
+
+// ---- batched subjects -----------------------------------------------------------------
+
+size_t dfm_batch_shoot_workspace(int dimpoint, int nseg, const long *cp_offsets, int precision) {
+  if (nseg < 1) return 0;
+  return workspace_or_zero(nseg, cp_offsets, nullptr, 0, 0, dimpoint, precision, 1, 0, 1);
+}
+
+int dfm_batch_shoot(void *stream, int D, double sigma, int prec, int scheme, int nseg,
+                    const long *cp_offsets, const double *p0, const double *a0, int T, double t0,
+                    double tN, double *P, double *A, void *work, size_t work_bytes) {
+  if (nseg < 1) return set_error(DFM_ERR_ARG, "segment count must be >= 1");
+  return shoot_impl(stream, D, sigma, prec, scheme, nseg, cp_offsets, p0, a0, 0, T, t0, tN, P, A,
+                    work, work_bytes);
+}
+
+size_t dfm_batch_flow_workspace(int dimpoint, int nseg, const long *cp_offsets,
+                                const long *lm_offsets, int precision) {
+  if (nseg < 1 || !lm_offsets) return 0;
+  return workspace_or_zero(nseg, cp_offsets, lm_offsets, 0, 0, dimpoint, precision, 1, 1, 1);
+}
+
+int dfm_batch_flow(void *stream, int D, double sigma, int prec, int improved_euler, int nseg,
+                   const long *cp_offsets, const long *lm_offsets, const double *P,
+                   const double *A, const double *x0, int T, double t0, double tN, double *X,
+                   void *work, size_t work_bytes) {
+  if (nseg < 1) return set_error(DFM_ERR_ARG, "segment count must be >= 1");
+  return flow_impl(stream, D, sigma, prec, improved_euler, nseg, cp_offsets, lm_offsets, P, A, 0,
+                   x0, 0, T, t0, tN, X, work, work_bytes);
+}
+
+size_t dfm_batch_adjoint_workspace(int dimpoint, int nseg, const long *cp_offsets,
+                                   const long *lm_offsets, int T, int checkpoint_every,
+                                   int precision) {
+  if (nseg < 1 || !lm_offsets) return 0;
+  return workspace_or_zero(nseg, cp_offsets, lm_offsets, 0, 0, dimpoint, precision, T, 2,
+                           checkpoint_every);
+}
+
+int dfm_batch_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int scheme,
+                                 int improved_euler, int nseg, const long *cp_offsets,
+                                 const long *lm_offsets, const double *p0, const double *a0,
+                                 const double *x0, int T, double t0, double tN,
+                                 int checkpoint_every, const double *theta_T, double *grad_p0,
+                                 double *grad_a0, double *grad_x0, void *work,
+                                 size_t work_bytes) {
+  if (nseg < 1) return set_error(DFM_ERR_ARG, "segment count must be >= 1");
+  return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, nseg, cp_offsets,
+                      lm_offsets, p0, a0, 0, x0, 0, T, t0, tN, checkpoint_every, theta_T, grad_p0,
+                      grad_a0, grad_x0, work, work_bytes);
 }
 
 }  // extern "C"

@@ -161,6 +161,14 @@ __global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, long
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
+struct LongChunk {
+  long v[32];
+};
+
+__global__ __launch_bounds__(64) void k_put_longs(long *dst, LongChunk c, int n) {
+  if ((int)threadIdx.x < n) dst[threadIdx.x] = c.v[threadIdx.x];
+}
+
 __global__ __launch_bounds__(256) void k_fill(double *dst, long n, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -195,22 +203,46 @@ static int nacc_of(int op, int D, int DV) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
-  Plan pl{};
+// Work split shared by single calls and ragged batches: chunks of the largest segment's
+// targets, and source splits of the largest segment's sources chosen so that the grid
+// holds >= 4096 wavefronts (4 per SIMD on 256 CUs) with >= 4 source tiles per split.
+static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long max_nx,
+                       long max_ny) {
   pl.R = targets_per_lane(op, prec, D, DV);
-  pl.chunks = (nx + 64L * pl.R - 1) / (64L * pl.R);
+  pl.nseg = nseg;
+  pl.chunks = (max_nx + 64L * pl.R - 1) / (64L * pl.R);
   const int TS = tile_of(prec);
-  const long tiles = (ny + TS - 1) / TS;
-  // Aim for >= 4096 wavefronts (4 per SIMD on 256 CUs) and >= 4 source tiles per split.
-  long want = pl.chunks > 0 ? (4096 + pl.chunks - 1) / pl.chunks : 1;
+  const long tiles = (max_ny + TS - 1) / TS;
+  const long blocks = pl.chunks * nseg;
+  long want = blocks > 0 ? (4096 + blocks - 1) / blocks : 1;
   long maxsplit = std::max(1L, tiles / 4);
   long ns = std::max(1L, std::min(want, maxsplit));
   ns = std::min(ns, 65535L);
   const long per = (tiles + ns - 1) / ns;
   pl.split_len = std::max(1L, per) * TS;
-  pl.nsplit = (int)std::max(1L, (ny + pl.split_len - 1) / pl.split_len);
+  pl.nsplit = (int)std::max(1L, (max_ny + pl.split_len - 1) / pl.split_len);
+}
+
+Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
+                     long max_ny) {
+  Plan pl{};
+  split_plan(pl, op, prec, D, DV, nseg, max_nx, max_ny);
+  pl.sort = false;
   size_t off = 0;
-  if (prec != DFM_FP64) {
+  if (pl.nsplit > 1) {
+    pl.off_partial = off;
+    off = align256(off + sizeof(double) * (size_t)pl.nsplit * nx_total * nacc_of(op, D, DV));
+  }
+  pl.work_bytes = off;
+  return pl;
+}
+
+Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
+  Plan pl{};
+  split_plan(pl, op, prec, D, DV, 1, nx, ny);
+  pl.sort = prec != DFM_FP64;
+  size_t off = 0;
+  if (pl.sort) {
     pl.off_bbox = off;
     off = align256(off + sizeof(double) * 6 * kBBoxParts);
     pl.off_keys_in = off;
@@ -279,6 +311,36 @@ int fill(hipStream_t st, double *dst, long n, double v) {
   return DFM_OK;
 }
 
+int put_longs(hipStream_t st, long *dst, const long *src, int n) {
+  for (int i = 0; i < n; i += 32) {
+    LongChunk c{};
+    const int k = std::min(32, n - i);
+    for (int j = 0; j < k; j++) c.v[j] = src[i + j];
+    hipLaunchKernelGGL(k_put_longs, dim3(1), dim3(64), 0, st, dst + i, c, k);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "offsets: %s", hipGetErrorString(e));
+}
+
+int seg_from_host(Seg &s, int n, const long *h, const long *d) {
+  if (n < 1 || n > 65535) return set_error(DFM_ERR_ARG, "segment count must be in [1, 65535]");
+  if (!h) return set_error(DFM_ERR_ARG, "segment offsets are NULL");
+  if (h[0] != 0) return set_error(DFM_ERR_ARG, "segment offsets must start at 0");
+  long mx = 0;
+  for (int i = 0; i < n; i++) {
+    const long c = h[i + 1] - h[i];
+    if (c < 0) return set_error(DFM_ERR_ARG, "segment offsets must be non-decreasing");
+    mx = std::max(mx, c);
+  }
+  if (h[n] > 0x7fffffffL) return set_error(DFM_ERR_ARG, "point counts must be in [0, 2^31)");
+  s.n = n;
+  s.h = h;
+  s.d = d;
+  s.total = h[n];
+  s.max = mx;
+  return DFM_OK;
+}
+
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
                    const Plan &pl, char *work) {
   const double sigma = p.sigma;
@@ -294,7 +356,7 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
   p.split_len = pl.split_len;
   p.perm = nullptr;
   p.partial = pl.nsplit > 1 ? (double *)(work + pl.off_partial) : nullptr;
-  if (prec != DFM_FP64) {
+  if (pl.sort) {
     int rc = sort_targets(st, p.x, p.nx, D, work, pl);
     if (rc != DFM_OK) return rc;
     p.perm = (const uint32_t *)(work + pl.off_perm);
@@ -324,6 +386,35 @@ static double clamp_width(double h) { return h < 1e-10 ? 1e-10 : h; }
 size_t pairsum_workspace(int op, int prec, int D, int DV, long nx, long ny) {
   if (nx <= 0 || ny <= 0) return 0;
   return make_plan(op, prec, D, DV, nx, ny).work_bytes;
+}
+
+size_t pairsum_batch_workspace(int op, int prec, int D, int DV, const Seg &xs, const Seg &ys) {
+  if (xs.total <= 0 || ys.max <= 0) return 0;
+  return make_batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max).work_bytes;
+}
+
+int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
+                  const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
+                  size_t work_bytes) {
+  if (xs.n != ys.n) return set_error(DFM_ERR_ARG, "target and source segment counts differ");
+  p.nx = xs.total;
+  p.ny = ys.total;
+  p.xoff = xs.n > 1 ? xs.d : nullptr;
+  p.yoff = ys.n > 1 ? ys.d : nullptr;
+  if (xs.n > 1 && (!xs.d || !ys.d)) return set_error(DFM_ERR_ARG, "device segment offsets missing");
+  if (p.nx == 0) return DFM_OK;
+  if (ys.max == 0) {  // every segment has an empty source set
+    fill(st, p.out0, out0_n, 0.0);
+    fill(st, p.out1, out1_n, 0.0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
+  }
+  Plan pl = make_batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max);
+  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
+                     work ? work_bytes : (size_t)0);
+  p.sigma = clamp_width(sigma);
+  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
 }
 
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,

@@ -28,6 +28,9 @@
 // units (sparse targets, narrow kernels) instead forms each difference from FP32 hi and
 // lo parts, (xh - yh) + (xl - yl); FP32X takes that path in every wavefront.
 //
+// Ragged batches: grid.z = segment; segment s pairs targets [xoff[s], xoff[s + 1]) with
+// sources [yoff[s], yoff[s + 1]) of the concatenated arrays.
+//
 // Work split: grid.x = target chunks of R*64, grid.y = source splits.  With one split the
 // kernel writes the scaled results straight to the caller's output (through the sort
 // permutation); with several it writes FP64 partials that k_reduce_splits sums in split
@@ -353,8 +356,19 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   __shared__ __attribute__((aligned(16))) T sm[TS * (Prec<T>::kCenter ? S::NSPX : S::NSP)];
 
   const int lane = threadIdx.x;
-  const long base = (long)blockIdx.x * (64 * R);
   const int split = blockIdx.y;
+  // segment of a ragged batch (one segment: the whole target and source sets)
+  long xb = 0, xe = p.nx, yb = 0, ye = p.ny;
+  if (p.xoff) {
+    xb = p.xoff[blockIdx.z];
+    xe = p.xoff[blockIdx.z + 1];
+  }
+  if (p.yoff) {
+    yb = p.yoff[blockIdx.z];
+    ye = p.yoff[blockIdx.z + 1];
+  }
+  const long base = xb + (long)blockIdx.x * (64 * R);
+  if (base >= xe) return;  // the whole wavefront lies past its segment
 
   // ---- targets -> registers
   long pos[R];
@@ -364,7 +378,7 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     pos[r] = base + r * 64 + lane;
-    valid[r] = pos[r] < p.nx;
+    valid[r] = pos[r] < xe;
     orig[r] = valid[r] ? (p.perm ? (long)p.perm[pos[r]] : pos[r]) : 0;
 #pragma unroll
     for (int k = 0; k < D; k++) xd[r][k] = valid[r] ? p.x[orig[r] * D + k] : 0.0;
@@ -427,8 +441,8 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
 #pragma unroll
     for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
 
-  const long js = (long)split * p.split_len;
-  const long je = std::min(p.ny, js + p.split_len);
+  const long js = yb + (long)split * p.split_len;
+  const long je = std::min(ye, js + p.split_len);
   if constexpr (Prec<T>::kCenter) {
     // wavefront-uniform choice: coordinates re-centred on a box wider than a few kernel
     // widths lose too much to FP32 rounding, so such wavefronts form exact differences
@@ -477,7 +491,7 @@ __global__ __launch_bounds__(256) void k_reduce_splits(KParams p) {
 
 template <typename T, int MODE, int OP, int D, int DV, int AUX>
 static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
-  dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit);
+  dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit, (unsigned)pl.nseg);
   switch (pl.R) {
     case 1: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE>), grid, dim3(64), 0, st, p); break;
     case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE>), grid, dim3(64), 0, st, p); break;

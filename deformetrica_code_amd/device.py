@@ -197,3 +197,27 @@ def graddiff(x, beta, eta, sigma, precision="fp32c", stream=None, workspace=None
     _lib.check(L.dfm_graddiff(_sp(stream), d, dv, float(sigma), prec, x.ptr, beta.ptr, eta.ptr,
                               n, out.ptr, wp, wb))
     return out
+
+
+def batch_conv_grad1(x, alpha, y, w, x_offsets, y_offsets, sigma, precision="fp32c", conv=True,
+                     grad=True, stream=None, workspace=None):
+    """conv_grad1 over a ragged batch: segment s pairs targets x[x_offsets[s]:x_offsets[s+1]]
+    with sources y[y_offsets[s]:y_offsets[s+1]] (host int64 offset arrays), one launch."""
+    L = _lib.load()
+    prec = _lib.precision_code(precision)
+    n, d = x.shape
+    dv = w.shape[1]
+    xo = np.ascontiguousarray(x_offsets, dtype=np.int64)
+    yo = np.ascontiguousarray(y_offsets, dtype=np.int64)
+    S = len(xo) - 1
+    cout = DeviceArray((n, dv)) if conv is True else (conv or None)
+    gout = DeviceArray((n, d)) if grad is True else (grad or None)
+    op = _lib.OP_CONV_GRAD1 if (cout is not None and gout is not None) else (
+        _lib.OP_CONV if cout is not None else _lib.OP_GRAD1)
+    ws = workspace or _default_ws
+    xp, yp = xo.ctypes.data_as(ctypes.c_void_p), yo.ctypes.data_as(ctypes.c_void_p)
+    wp, wb = ws.get(L.dfm_batch_workspace(op, d, dv, S, xp, yp, prec))
+    _lib.check(L.dfm_batch_conv_grad1(_sp(stream), d, dv, float(sigma), prec, S, xp, yp, x.ptr,
+                                      _p(alpha) if gout is not None else None, y.ptr, w.ptr,
+                                      _p(cout), _p(gout), wp, wb))
+    return cout, gout

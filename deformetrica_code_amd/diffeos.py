@@ -140,3 +140,94 @@ def shoot_vjp(p, a, mu_p, mu_a, sigma, precision="fp64"):
     _lib.check(L.dfm_shoot_vjp(st.handle, d, float(sigma), prec, dp.ptr, da.ptr, dmp.ptr, dma.ptr,
                                n, jp.ptr, ja.ptr, wp, wb))
     return jp.numpy(st), ja.numpy(st)
+
+
+def _offsets(counts):
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    return off
+
+
+class HIPBatchDiffeos(HIPDiffeos):
+    """Many subjects (each its own control points, momenta and landmarks, with uneven
+    counts) shot, flowed and back-propagated together: every kernel sum of a time step is
+    one launch over all subjects (dfm_batch_shoot / dfm_batch_flow /
+    dfm_batch_shoot_flow_adjoint).  Per-subject setters take lists of arrays; getters
+    return lists.  Replaces the per-subject Diffeos clones of the reference's atlas loop
+    (AbstractAtlas.cxx:490-560)."""
+
+    def SetStartPositions(self, p0_list):
+        self.p0_list = [np.ascontiguousarray(p, dtype=np.float64) for p in p0_list]
+
+    def SetStartMomentas(self, a0_list):
+        self.a0_list = [np.ascontiguousarray(a, dtype=np.float64) for a in a0_list]
+
+    def SetLandmarkPoints(self, x0_list):
+        self.x0_list = [np.ascontiguousarray(x, dtype=np.float64) for x in x0_list]
+
+    def _segments(self):
+        d = self.dim
+        if len(self.p0_list) != len(self.a0_list) or any(
+                p.shape != a.shape for p, a in zip(self.p0_list, self.a0_list)):
+            raise RuntimeError("start positions and momenta differ in shape")
+        self.cp_off = _offsets([p.shape[0] for p in self.p0_list])
+        self.lm_off = _offsets([x.shape[0] for x in self.x0_list])
+        cat = lambda lst: (np.concatenate(lst).reshape(-1, d) if lst else np.zeros((0, d)))
+        return cat(self.p0_list), cat(self.a0_list), cat(self.x0_list)
+
+    @staticmethod
+    def _cp(off):
+        return off.ctypes.data_as(ctypes.c_void_p)
+
+    def Update(self):
+        L = _lib.load()
+        d, T, S = self.dim, self.T, len(self.p0_list)
+        p0, a0, x0 = self._segments()
+        st = self._stream
+        dp0, da0, dx0 = (DeviceArray.from_numpy(v, st) for v in (p0, a0, x0))
+        n, m = p0.shape[0], x0.shape[0]
+        self._P, self._A, self._X = DeviceArray((T, n, d)), DeviceArray((T, n, d)), DeviceArray((T, m, d))
+        cp, lm = self._cp(self.cp_off), self._cp(self.lm_off)
+        wp, wb = self._ws.get(L.dfm_batch_shoot_workspace(d, S, cp, self.precision))
+        _lib.check(L.dfm_batch_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme, S,
+                                     cp, dp0.ptr, da0.ptr, T, self.t0, self.tN, self._P.ptr,
+                                     self._A.ptr, wp, wb))
+        wp, wb = self._ws.get(L.dfm_batch_flow_workspace(d, S, cp, lm, self.precision))
+        _lib.check(L.dfm_batch_flow(st.handle, d, self.kernel_width, self.precision,
+                                    int(self.improved_euler), S, cp, lm, self._P.ptr, self._A.ptr,
+                                    dx0.ptr, T, self.t0, self.tN, self._X.ptr, wp, wb))
+        st.synchronize()
+
+    def _split(self, arr, off):
+        return [arr[..., off[s]:off[s + 1], :] for s in range(len(off) - 1)]
+
+    def GetTrajectoryPositions(self):
+        return self._split(self._P.numpy(self._stream), self.cp_off)
+
+    def GetTrajectoryMomentas(self):
+        return self._split(self._A.numpy(self._stream), self.cp_off)
+
+    def GetLandmarkPointsTrajectory(self):
+        return self._split(self._X.numpy(self._stream), self.lm_off)
+
+    def GetDeformedLandmarkPoints(self):
+        return self._split(self._X.numpy(self._stream)[-1], self.lm_off)
+
+    def GradientOfTerminalCost(self, theta_list, checkpoint_every: int = 5):
+        """Per-subject lists (dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_list[s]."""
+        L = _lib.load()
+        d, T, S = self.dim, self.T, len(self.p0_list)
+        p0, a0, x0 = self._segments()
+        th = np.concatenate([np.asarray(t, dtype=np.float64) for t in theta_list]).reshape(-1, d)
+        st = self._stream
+        dp0, da0, dx0, dth = (DeviceArray.from_numpy(v, st) for v in (p0, a0, x0, th))
+        gp, ga, gx = DeviceArray(p0.shape), DeviceArray(p0.shape), DeviceArray(x0.shape)
+        cp, lm = self._cp(self.cp_off), self._cp(self.lm_off)
+        wp, wb = self._ws.get(L.dfm_batch_adjoint_workspace(d, S, cp, lm, T, int(checkpoint_every),
+                                                            self.precision))
+        _lib.check(L.dfm_batch_shoot_flow_adjoint(
+            st.handle, d, self.kernel_width, self.precision, self.scheme, int(self.improved_euler),
+            S, cp, lm, dp0.ptr, da0.ptr, dx0.ptr, T, self.t0, self.tN, int(checkpoint_every),
+            dth.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
+        return (self._split(gp.numpy(st), self.cp_off), self._split(ga.numpy(st), self.cp_off),
+                self._split(gx.numpy(st), self.lm_off))

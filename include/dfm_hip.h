@@ -224,6 +224,59 @@ int dfm_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precisi
                            int checkpoint_every, const double *theta_T, double *grad_p0,
                            double *grad_a0, double *grad_x0, void *work, size_t work_bytes);
 
+/* ---- ragged batches of subjects (device pointers, host offset arrays) -------------------
+ * A batch of nseg subjects (1 <= nseg <= 65535) stores each per-point array concatenated
+ * over subjects: subject s owns rows [off[s], off[s + 1]) where off is a host array of
+ * nseg + 1 entries, off[0] = 0, non-decreasing.  The library writes device copies of the
+ * offsets into the workspace with kernel launches, and batched calls make no allocation,
+ * memset, host copy or synchronisation: they can be captured in a HIP graph (stream
+ * capture) and replayed.  Each pair sum covers every subject in one launch (grid.z =
+ * subject).  FP32 paths skip the Morton ordering in batches and rely on the per-wavefront
+ * exact-difference switch for accuracy.
+ * No single reference counterpart: the reference runs subjects one after another, or on
+ * threads, through per-subject kernel objects (AbstractAtlas.cxx:490-560, 826-865). */
+
+/* HIP graphs of stream-ordered calls: begin capture on a created stream, issue batched
+ * calls, end capture to get an instantiated graph, replay it with dfm_graph_launch. */
+int dfm_graph_begin_capture(void *stream);
+int dfm_graph_end_capture(void *stream, void **graph_exec);
+int dfm_graph_launch(void *graph_exec, void *stream);
+int dfm_graph_destroy(void *graph_exec);
+
+/* dfm_conv_grad1 for every subject at once: targets x (rows by x_offsets), sources y and
+ * weights beta (rows by y_offsets); conv and/or grad may be NULL.  op for the workspace
+ * query: DFM_OP_CONV, DFM_OP_GRAD1 or DFM_OP_CONV_GRAD1 (what the call computes). */
+size_t dfm_batch_workspace(int op, int dimpoint, int dimvect, int nseg, const long *x_offsets,
+                           const long *y_offsets, int precision);
+int dfm_batch_conv_grad1(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                         int nseg, const long *x_offsets, const long *y_offsets, const double *x,
+                         const double *alpha, const double *y, const double *beta, double *conv,
+                         double *grad, void *work, size_t work_bytes);
+
+/* dfm_shoot / dfm_flow / dfm_shoot_flow_adjoint for every subject at once: control points
+ * by cp_offsets, landmarks by lm_offsets; trajectories time-major, P, A: T x (all control
+ * points) x d and X: T x (all landmarks) x d.  The zero-momenta exit is per subject. */
+size_t dfm_batch_shoot_workspace(int dimpoint, int nseg, const long *cp_offsets, int precision);
+int dfm_batch_shoot(void *stream, int dimpoint, double sigma, int precision, int scheme, int nseg,
+                    const long *cp_offsets, const double *p0, const double *a0, int T, double t0,
+                    double tN, double *P, double *A, void *work, size_t work_bytes);
+size_t dfm_batch_flow_workspace(int dimpoint, int nseg, const long *cp_offsets,
+                                const long *lm_offsets, int precision);
+int dfm_batch_flow(void *stream, int dimpoint, double sigma, int precision, int improved_euler,
+                   int nseg, const long *cp_offsets, const long *lm_offsets, const double *P,
+                   const double *A, const double *x0, int T, double t0, double tN, double *X,
+                   void *work, size_t work_bytes);
+size_t dfm_batch_adjoint_workspace(int dimpoint, int nseg, const long *cp_offsets,
+                                   const long *lm_offsets, int T, int checkpoint_every,
+                                   int precision);
+int dfm_batch_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precision,
+                                 int scheme, int improved_euler, int nseg,
+                                 const long *cp_offsets, const long *lm_offsets,
+                                 const double *p0, const double *a0, const double *x0, int T,
+                                 double t0, double tN, int checkpoint_every,
+                                 const double *theta_T, double *grad_p0, double *grad_a0,
+                                 double *grad_x0, void *work, size_t work_bytes);
+
 /* ---- surface data terms (device pointers) ----------------------------------------------
  * Meshes are vertices (nv x 3, FP64, row-major) and triangles (nf x 3, int32 indices).
  * DFM_CURRENT is OrientedSurfaceMesh (area-weighted normals as currents), DFM_VARIFOLD is

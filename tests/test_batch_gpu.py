@@ -1,0 +1,185 @@
+"""Ragged batches of subjects: one launch per kernel sum for all subjects, against the
+single-subject entry points (themselves pinned against the oracle), per-subject
+zero-momenta exits, and HIP-graph capture and replay of batched shooting and flow."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _offsets(counts):
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    return off
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c", "fp32x"])
+def test_batch_conv_grad1_matches_per_segment_calls(prec):
+    from deformetrica_code_amd import device
+    rng = np.random.default_rng(1)
+    nx = [300, 0, 1000, 77, 5000, 1]
+    ny = [400, 200, 0, 1500, 3000, 9]
+    xo, yo = _offsets(nx), _offsets(ny)
+    X = rng.uniform(size=(xo[-1], 3))
+    A = rng.normal(size=(xo[-1], 3))
+    Y = rng.uniform(size=(yo[-1], 3))
+    W = rng.normal(size=(yo[-1], 3))
+    sigma = 0.1
+    dX, dA, dY, dW = (device.DeviceArray.from_numpy(v) for v in (X, A, Y, W))
+    conv, grad = device.batch_conv_grad1(dX, dA, dY, dW, xo, yo, sigma, prec)
+    conv, grad = conv.numpy(), grad.numpy()
+    for s in range(len(nx)):
+        xs, ys = slice(xo[s], xo[s + 1]), slice(yo[s], yo[s + 1])
+        if nx[s] == 0:
+            continue
+        if ny[s] == 0:
+            np.testing.assert_array_equal(conv[xs], 0.0)
+            np.testing.assert_array_equal(grad[xs], 0.0)
+            continue
+        c1, g1 = device.conv_grad1(device.DeviceArray.from_numpy(X[xs]),
+                                   device.DeviceArray.from_numpy(A[xs]),
+                                   device.DeviceArray.from_numpy(Y[ys]),
+                                   device.DeviceArray.from_numpy(W[ys]), sigma, "fp64")
+        tol = 1e-12 if prec == "fp64" else 1e-6
+        assert rel(conv[xs], c1.numpy()) < tol, s
+        assert rel(grad[xs], g1.numpy()) < tol, s
+
+
+def _subjects(seed, counts_cp, counts_lm, d=3, scale=0.2):
+    rng = np.random.default_rng(seed)
+    p0 = [rng.uniform(size=(n, d)) for n in counts_cp]
+    a0 = [rng.normal(scale=scale, size=(n, d)) for n in counts_cp]
+    x0 = [rng.uniform(size=(m, d)) for m in counts_lm]
+    return p0, a0, x0
+
+
+def _single(d, width, T, p0, a0, x0, prec, scheme, improved):
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    df = HIPDiffeos(d, prec)
+    df.SetKernelWidth(width)
+    df.SetNumberOfTimePoints(T)
+    df.SetShootingScheme(scheme)
+    if not improved:
+        df.UseStandardEuler()
+    df.SetStartPositions(p0)
+    df.SetStartMomentas(a0)
+    df.SetLandmarkPoints(x0)
+    return df
+
+
+def _batch(d, width, T, p0, a0, x0, prec, scheme, improved):
+    from deformetrica_code_amd.diffeos import HIPBatchDiffeos
+    df = HIPBatchDiffeos(d, prec)
+    df.SetKernelWidth(width)
+    df.SetNumberOfTimePoints(T)
+    df.SetShootingScheme(scheme)
+    if not improved:
+        df.UseStandardEuler()
+    df.SetStartPositions(p0)
+    df.SetStartMomentas(a0)
+    df.SetLandmarkPoints(x0)
+    return df
+
+
+@pytest.mark.parametrize("scheme,improved", [("euler", True), ("heun", False)])
+def test_batch_shoot_flow_and_adjoint_match_single_subject_calls(scheme, improved):
+    d, width, T = 3, 0.3, 9
+    cps, lms = [40, 130, 7, 64], [200, 55, 300, 1]
+    p0, a0, x0 = _subjects(2, cps, lms)
+    B = _batch(d, width, T, p0, a0, x0, "fp64", scheme, improved)
+    B.Update()
+    rng = np.random.default_rng(3)
+    thetas = [rng.normal(size=x.shape) for x in x0]
+    gps, gas, gxs = B.GradientOfTerminalCost(thetas, checkpoint_every=3)
+    Ps, As, Xs = B.GetTrajectoryPositions(), B.GetTrajectoryMomentas(), B.GetLandmarkPointsTrajectory()
+    for s in range(len(cps)):
+        S = _single(d, width, T, p0[s], a0[s], x0[s], "fp64", scheme, improved)
+        S.Update()
+        assert rel(Ps[s], np.array(S.GetTrajectoryPositions())) < 1e-13
+        assert rel(As[s], np.array(S.GetTrajectoryMomentas())) < 1e-12
+        assert rel(Xs[s], np.array(S.GetLandmarkPointsTrajectory())) < 1e-13
+        gp, ga, gx = S.GradientOfTerminalCost(thetas[s], checkpoint_every=3)
+        assert rel(gps[s], gp) < 1e-11 and rel(gas[s], ga) < 1e-11 and rel(gxs[s], gx) < 1e-11
+
+
+def test_batch_fp32c_within_1e6_of_fp64():
+    d, width, T = 3, 0.2, 11
+    p0, a0, x0 = _subjects(4, [500, 900], [1000, 300], scale=0.01)
+    out = {}
+    for prec in ("fp64", "fp32c"):
+        B = _batch(d, width, T, p0, a0, x0, prec, "euler", True)
+        B.Update()
+        out[prec] = (B.GetTrajectoryPositions(), B.GetLandmarkPointsTrajectory())
+    for s in range(2):
+        assert rel(out["fp32c"][0][s][-1] - p0[s], out["fp64"][0][s][-1] - p0[s]) < 1e-6
+        assert rel(out["fp32c"][1][s][-1] - x0[s], out["fp64"][1][s][-1] - x0[s]) < 1e-6
+
+
+def test_zero_momenta_exit_is_per_subject():
+    p0, a0, x0 = _subjects(5, [30, 30, 30], [40, 40, 40])
+    a0[1] = np.full_like(a0[1], 1e-22)
+    B = _batch(3, 0.4, 6, p0, a0, x0, "fp64", "euler", True)
+    B.Update()
+    P, X = B.GetTrajectoryPositions(), B.GetLandmarkPointsTrajectory()
+    for t in range(6):
+        np.testing.assert_array_equal(P[1][t], p0[1])
+        np.testing.assert_array_equal(X[1][t], x0[1])
+    assert np.abs(P[0][-1] - p0[0]).max() > 1e-3
+    assert np.abs(X[2][-1] - x0[2]).max() > 1e-3
+
+
+def test_graph_capture_replays_batched_shoot_and_flow():
+    """Capture dfm_batch_shoot + dfm_batch_flow once, replay it after changing the momenta
+    in place, with eager work in between; every replay equals the eager result bitwise."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.device import DeviceArray, Stream
+    L = _lib.load()
+    d, T, width, prec = 3, 7, 0.3, _lib.FP32C
+    cps, lms = [64, 200, 33], [500, 120, 260]
+    p0, a0, x0 = _subjects(6, cps, lms)
+    cp, lm = _offsets(cps), _offsets(lms)
+    cpp, lmp = cp.ctypes.data_as(ctypes.c_void_p), lm.ctypes.data_as(ctypes.c_void_p)
+    P0, A0, X0 = (np.concatenate(v) for v in (p0, a0, x0))
+    n, m, S = P0.shape[0], X0.shape[0], len(cps)
+    st = Stream()
+    dp0, da0, dx0 = (DeviceArray.from_numpy(v, st) for v in (P0, A0, X0))
+    P, A, X = DeviceArray((T, n, d)), DeviceArray((T, n, d)), DeviceArray((T, m, d))
+    w1 = DeviceArray(((L.dfm_batch_shoot_workspace(d, S, cpp, prec) + 7) // 8,))
+    w2 = DeviceArray(((L.dfm_batch_flow_workspace(d, S, cpp, lmp, prec) + 7) // 8,))
+
+    def issue():
+        _lib.check(L.dfm_batch_shoot(st.handle, d, width, prec, _lib.EULER, S, cpp, dp0.ptr, da0.ptr,
+                                     T, 0.0, 1.0, P.ptr, A.ptr, w1.ptr, w1.nbytes))
+        _lib.check(L.dfm_batch_flow(st.handle, d, width, prec, 1, S, cpp, lmp, P.ptr, A.ptr, dx0.ptr,
+                                    T, 0.0, 1.0, X.ptr, w2.ptr, w2.nbytes))
+
+    def eager():
+        issue()
+        st.synchronize()
+        return P.numpy(st), X.numpy(st)
+
+    ref = eager()
+    _lib.check(L.dfm_graph_begin_capture(st.handle))
+    issue()
+    g = ctypes.c_void_p()
+    _lib.check(L.dfm_graph_end_capture(st.handle, ctypes.byref(g)))
+    try:
+        rng = np.random.default_rng(7)
+        for r in range(4):
+            if r > 0:
+                da0.upload(A0 * (1.0 + 0.1 * r) + rng.normal(scale=1e-3, size=A0.shape), st)
+                ref = eager()
+                P.upload(np.zeros((T, n, d)), st)
+                X.upload(np.zeros((T, m, d)), st)
+            _lib.check(L.dfm_graph_launch(g, st.handle))
+            st.synchronize()
+            np.testing.assert_array_equal(P.numpy(st), ref[0])
+            np.testing.assert_array_equal(X.numpy(st), ref[1])
+    finally:
+        _lib.check(L.dfm_graph_destroy(g))
