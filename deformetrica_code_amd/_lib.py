@@ -93,6 +93,14 @@ def load():
                  "dfm_batch_shoot_flow_adjoint", "dfm_graph_begin_capture",
                  "dfm_graph_end_capture", "dfm_graph_launch", "dfm_graph_destroy"):
         getattr(L, name).restype = i
+    # deterministic atlas
+    L.dfm_atlas_create.argtypes = [vp, ctypes.POINTER(vp)]
+    L.dfm_atlas_destroy.argtypes = [vp]
+    L.dfm_atlas_device_bytes.argtypes = [vp]
+    L.dfm_atlas_device_bytes.restype = sz
+    L.dfm_atlas_gradient.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    for name in ("dfm_atlas_create", "dfm_atlas_destroy", "dfm_atlas_gradient"):
+        getattr(L, name).restype = i
     # surface data terms
     L.dfm_mesh_incidence.argtypes = [vp, l, l, vp, vp]
     L.dfm_surface_workspace.argtypes = [i, l, l, i, i]

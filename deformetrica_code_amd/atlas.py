@@ -1,0 +1,155 @@
+"""Deterministic atlas of surfaces on the GPU: the log-likelihood and gradient of
+Deformetrica's DeterministicAtlas (DeterministicAtlas.cxx:100-281, AbstractAtlas.cxx:490-751)
+for many subjects at once, through libdfm_hip.so's dfm_atlas_* entry points.
+
+``HIPDeterministicAtlas`` is created once per (template topology, targets, parameters);
+``ComputeLogLikelihood`` / ``ComputeCompleteLogLikelihoodGradient`` then run on
+device-resident control points, momenta and template vertices.  ``capture()`` records one
+gradient evaluation in a HIP graph for replay.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, Stream
+
+
+class AtlasDesc(ctypes.Structure):
+    """Mirror of dfm_atlas_desc (include/dfm_hip.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int), ("precision", ctypes.c_int), ("scheme", ctypes.c_int),
+        ("improved_euler", ctypes.c_int), ("number_of_time_points", ctypes.c_int),
+        ("checkpoint_every", ctypes.c_int), ("t0", ctypes.c_double), ("tN", ctypes.c_double),
+        ("deformation_width", ctypes.c_double), ("attachment_width", ctypes.c_double),
+        ("data_sigma_squared", ctypes.c_double), ("smoothing_width", ctypes.c_double),
+        ("n_control_points", ctypes.c_long), ("n_template_vertices", ctypes.c_long),
+        ("n_template_faces", ctypes.c_long), ("template_faces", ctypes.c_void_p),
+        ("nsubjects", ctypes.c_int), ("target_vertex_offsets", ctypes.c_void_p),
+        ("target_face_offsets", ctypes.c_void_p), ("target_vertices", ctypes.c_void_p),
+        ("target_faces", ctypes.c_void_p),
+    ]
+
+
+def _offsets(counts):
+    off = np.zeros(len(counts) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts)
+    return off
+
+
+class HIPDeterministicAtlas:
+    """targets: list of (vertices (v_s x 3), faces (f_s x 3)) per subject."""
+
+    def __init__(self, template_faces, n_template_vertices, n_control_points, targets, *,
+                 kind="varifold", precision="fp64", deformation_width=1.0,
+                 attachment_width=1.0, data_sigma_squared=1.0, smoothing_width=0.0,
+                 number_of_time_points=11, t0=0.0, tN=1.0, scheme="euler", improved_euler=True,
+                 checkpoint_every=5):
+        L = _lib.load()
+        tf = np.ascontiguousarray(template_faces, dtype=np.int32)
+        tv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1, 3)
+                                                  for v, _ in targets]))
+        tfa = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32).reshape(-1, 3)
+                                                   for _, f in targets]))
+        vo = _offsets([len(v) for v, _ in targets])
+        fo = _offsets([len(f) for _, f in targets])
+        self._keep = (tf, tv, tfa, vo, fo)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        d = AtlasDesc()
+        d.kind = {"current": _lib.CURRENT, "varifold": _lib.VARIFOLD}[kind]
+        d.precision = _lib.precision_code(precision)
+        d.scheme = {"euler": _lib.EULER, "heun": _lib.HEUN, "rk2": _lib.HEUN}[scheme]
+        d.improved_euler = int(improved_euler)
+        d.number_of_time_points = int(number_of_time_points)
+        d.checkpoint_every = int(checkpoint_every)
+        d.t0, d.tN = float(t0), float(tN)
+        d.deformation_width = float(deformation_width)
+        d.attachment_width = float(attachment_width)
+        d.data_sigma_squared = float(data_sigma_squared)
+        d.smoothing_width = float(smoothing_width)
+        d.n_control_points = int(n_control_points)
+        d.n_template_vertices = int(n_template_vertices)
+        d.n_template_faces = tf.shape[0]
+        d.template_faces = vp(tf)
+        d.nsubjects = len(targets)
+        d.target_vertex_offsets, d.target_face_offsets = vp(vo), vp(fo)
+        d.target_vertices, d.target_faces = vp(tv), vp(tfa)
+        self.desc = d
+        self.S, self.n, self.m = len(targets), int(n_control_points), int(n_template_vertices)
+        self._h = ctypes.c_void_p()
+        _lib.check(L.dfm_atlas_create(ctypes.byref(d), ctypes.byref(self._h)))
+        self.stream = Stream()
+        self.loglik = DeviceArray((2,))
+        self.residuals = DeviceArray((self.S,))
+        self.grad_cp = DeviceArray((self.n, 3))
+        self.grad_momenta = DeviceArray((self.S, self.n, 3))
+        self.grad_template = DeviceArray((self.m, 3))
+
+    def device_bytes(self):
+        return int(_lib.load().dfm_atlas_device_bytes(self._h))
+
+    def gradient_device(self, cp, momenta, template, with_gradient=True):
+        """One evaluation on DeviceArrays, stream-ordered (no synchronisation)."""
+        L = _lib.load()
+        g = (lambda a: a.ptr) if with_gradient else (lambda a: None)
+        _lib.check(L.dfm_atlas_gradient(self._h, self.stream.handle, cp.ptr, momenta.ptr,
+                                        template.ptr, self.loglik.ptr, self.residuals.ptr,
+                                        g(self.grad_cp), g(self.grad_momenta),
+                                        g(self.grad_template)))
+
+    def _upload(self, cp, momenta, template):
+        st = self.stream
+        return (DeviceArray.from_numpy(cp, st), DeviceArray.from_numpy(momenta, st),
+                DeviceArray.from_numpy(template, st))
+
+    def ComputeLogLikelihood(self, cp, momenta, template):
+        """(data term, regularity term) and the per-subject residuals."""
+        args = self._upload(cp, momenta, template)
+        self.gradient_device(*args, with_gradient=False)
+        self.stream.synchronize()
+        return self.loglik.numpy(self.stream), self.residuals.numpy(self.stream)
+
+    def ComputeCompleteLogLikelihoodGradient(self, cp, momenta, template):
+        """loglik terms, d/d cp, d/d momenta (S x n x 3), d/d template vertices."""
+        args = self._upload(cp, momenta, template)
+        self.gradient_device(*args)
+        self.stream.synchronize()
+        st = self.stream
+        return (self.loglik.numpy(st), self.grad_cp.numpy(st), self.grad_momenta.numpy(st),
+                self.grad_template.numpy(st))
+
+    def capture(self, cp, momenta, template):
+        """A HIP graph of one gradient evaluation on the given DeviceArrays."""
+        L = _lib.load()
+        _lib.check(L.dfm_graph_begin_capture(self.stream.handle))
+        try:
+            self.gradient_device(cp, momenta, template)
+        finally:
+            g = ctypes.c_void_p()
+            rc = L.dfm_graph_end_capture(self.stream.handle, ctypes.byref(g))
+        _lib.check(rc)
+        return Graph(g, self.stream)
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                _lib.load().dfm_atlas_destroy(self._h)
+        except Exception:
+            pass
+
+
+class Graph:
+    def __init__(self, handle, stream):
+        self._g, self.stream = handle, stream
+
+    def launch(self):
+        _lib.check(_lib.load().dfm_graph_launch(self._g, self.stream.handle))
+
+    def __del__(self):
+        try:
+            if self._g.value:
+                _lib.load().dfm_graph_destroy(self._g)
+        except Exception:
+            pass

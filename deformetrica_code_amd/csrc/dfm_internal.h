@@ -122,6 +122,27 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
 size_t pairsum_batch_workspace(int op, int prec, int D, int DV, const Seg &xs, const Seg &ys);
 int copy_d2d(hipStream_t st, void *dst, const void *src, size_t bytes);
 
+// surface data-term launchers (surfaces.hip); nsub meshes share one triangle list
+int surf_geometry(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
+                  int nsub, double *C, double *W, double *A, double *N, double *bad);
+int surf_contrib(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
+                 int nsub, const double *N, const double *ktS, const double *gS,
+                 const double *ktT, const double *gT, double *contrib);
+int surf_gather(hipStream_t st, const int *offsets, const int *inc, long nv, long nf, int nsub,
+                const double *contrib, double scale, double *grad);
+int seg_dot(hipStream_t st, const double *X, const double *Y, int dv, int nsub, long seg_len,
+            const long *off, double *out);
+
+// batched shoot + flow adjoint in phases (integrators.hip): phase 1 forward (checkpoints
+// and *x_final in the workspace), phase 2 backward from them, phase 3 both
+int batch_adjoint_phase(hipStream_t st, int D, double sigma, int prec, int scheme, int improved,
+                        int nseg, const long *cp_off, const long *lm_off, const double *p0,
+                        const double *a0, const double *x0, int T, double t0, double tN, int k,
+                        const double *theta_T, double *gp, double *ga, double *gx, void *work,
+                        size_t work_bytes, int phase, const double **x_final);
+size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *lm_off, int T,
+                               int k, int prec);
+
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)
 int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
 int launch_op_f32c(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);

@@ -454,11 +454,15 @@ int flow_impl(void *stream, int D, double sigma, int prec, int improved_euler, i
   return DFM_OK;
 }
 
+// phase 1: forward pass only (checkpoints and the final landmarks, *x_final, stay in the
+// workspace); phase 2: backward pass from the checkpoints of an earlier phase 1 with the
+// same arguments; phase 3: both.
 int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int improved_euler,
                  int nseg, const long *cp_off, const long *lm_off, const double *p0,
                  const double *a0, long n_, const double *x0, long m_, int T, double t0,
                  double tN, int checkpoint_every, const double *theta_T, double *grad_p0,
-                 double *grad_a0, double *grad_x0, void *work, size_t work_bytes) {
+                 double *grad_a0, double *grad_x0, void *work, size_t work_bytes, int phase = 3,
+                 const double **x_final = nullptr) {
   clear_error();
   if (nseg > 0 && !lm_off) return set_error(DFM_ERR_ARG, "landmark offsets are NULL");
   Seg cs, ls;
@@ -469,7 +473,8 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
   if (checkpoint_every < 1) return set_error(DFM_ERR_ARG, "checkpoint interval must be >= 1");
   const long n = cs.total, m = ls.total;
   if (n == 0) return DFM_OK;
-  if (!p0 || !a0 || !grad_p0 || !grad_a0 || (m > 0 && (!x0 || !theta_T)))
+  if (!p0 || !a0 || (m > 0 && !x0)) return set_error(DFM_ERR_ARG, "NULL argument");
+  if ((phase & 2) && (!grad_p0 || !grad_a0 || (m > 0 && !theta_T)))
     return set_error(DFM_ERR_ARG, "NULL argument");
   const bool batch = nseg > 0;
   AdjointLayout need(nullptr, prec, D, cs, ls, T, checkpoint_every, batch);
@@ -486,25 +491,28 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
   auto A_ = [&](double *r) { return r + nd; };
   auto X_ = [&](double *r) { return r + 2 * nd; };
 
-  if ((rc = norm2(cx, a0, cs, gate))) return rc;
+  if (phase & 1) {
+    if ((rc = norm2(cx, a0, cs, gate))) return rc;
+    // ---- forward: checkpoints at steps 0, k, 2k, ...
+    double *cur = L.seg, *nxt = L.seg + rec;
+    if ((rc = copy(cx, P_(cur), p0, nd)) || (rc = copy(cx, A_(cur), a0, nd))) return rc;
+    if (m > 0 && (rc = copy(cx, X_(cur), x0, md))) return rc;
+    for (int t = 0; t + 1 < T; t++) {
+      if (t % L.k == 0 && (rc = copy(cx, L.ckpt + (size_t)(t / L.k) * rec, cur, rec))) return rc;
+      if ((rc = shoot_step(cx, scheme, P_(cur), A_(cur), cs, dt, P_(nxt), A_(nxt), L.st, gate)))
+        return rc;
+      if (m > 0 && (rc = flow_step(cx, improved_euler, P_(cur), A_(cur), P_(nxt), A_(nxt), cs,
+                                   X_(cur), ls, dt, X_(nxt), L.ft, gate)))
+        return rc;
+      std::swap(cur, nxt);
+    }
+    if (x_final) *x_final = X_(cur);
+  }
+  if (!(phase & 2)) return DFM_OK;
   // terminal cotangents
   if ((rc = lin(cx, L.lp, p0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
   if ((rc = lin(cx, L.la, a0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
   if (m > 0 && (rc = copy(cx, L.th, theta_T, md))) return rc;
-
-  // ---- forward: checkpoints at steps 0, k, 2k, ...
-  double *cur = L.seg, *nxt = L.seg + rec;
-  if ((rc = copy(cx, P_(cur), p0, nd)) || (rc = copy(cx, A_(cur), a0, nd))) return rc;
-  if (m > 0 && (rc = copy(cx, X_(cur), x0, md))) return rc;
-  for (int t = 0; t + 1 < T; t++) {
-    if (t % L.k == 0 && (rc = copy(cx, L.ckpt + (size_t)(t / L.k) * rec, cur, rec))) return rc;
-    if ((rc = shoot_step(cx, scheme, P_(cur), A_(cur), cs, dt, P_(nxt), A_(nxt), L.st, gate)))
-      return rc;
-    if (m > 0 && (rc = flow_step(cx, improved_euler, P_(cur), A_(cur), P_(nxt), A_(nxt), cs,
-                                 X_(cur), ls, dt, X_(nxt), L.ft, gate)))
-      return rc;
-    std::swap(cur, nxt);
-  }
 
   // ---- backward, one checkpoint segment at a time
   for (int s = L.segs - 1; s >= 0; s--) {
@@ -594,6 +602,21 @@ size_t workspace_or_zero(int nseg, const long *cp_off, const long *lm_off, long 
 }
 
 }  // namespace
+
+int batch_adjoint_phase(hipStream_t st, int D, double sigma, int prec, int scheme, int improved,
+                        int nseg, const long *cp_off, const long *lm_off, const double *p0,
+                        const double *a0, const double *x0, int T, double t0, double tN, int k,
+                        const double *theta_T, double *gp, double *ga, double *gx, void *work,
+                        size_t work_bytes, int phase, const double **x_final) {
+  return adjoint_impl(st, D, sigma, prec, scheme, improved, nseg, cp_off, lm_off, p0, a0, 0, x0,
+                      0, T, t0, tN, k, theta_T, gp, ga, gx, work, work_bytes, phase, x_final);
+}
+
+size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *lm_off, int T,
+                               int k, int prec) {
+  return workspace_or_zero(nseg, cp_off, lm_off, 0, 0, D, prec, T, 2, k);
+}
+
 }  // namespace dfm
 
 using namespace dfm;

@@ -45,15 +45,19 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// geometry of each face; bad indices flag the face (weights zero)
+// Geometry of each face; bad indices flag the face (weights zero).  nsub meshes share
+// one triangle list F (nf faces over nv vertices): face f of the output belongs to mesh
+// f / nf, whose vertices start at row (f / nf) * nv of V.
 __global__ __launch_bounds__(256) void k_face_geometry(int kind, const double *V, long nv,
-                                                       const int *F, long nf, double *C,
-                                                       double *W, double *A, double *Nout,
-                                                       double *bad) {
+                                                       const int *F, long nf, int nsub,
+                                                       double *C, double *W, double *A,
+                                                       double *Nout, double *bad) {
   const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf * nsub) return;
+  const long sub = f / nf, fl = f - sub * nf;
+  V += sub * nv * 3;
   const int dv = kind == DFM_CURRENT ? 3 : 6;
-  int i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+  int i0 = F[3 * fl], i1 = F[3 * fl + 1], i2 = F[3 * fl + 2];
   if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) {
     for (int k = 0; k < 3; k++) C[3 * f + k] = 0.0, Nout[3 * f + k] = 0.0;
     for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
@@ -127,17 +131,19 @@ __device__ __forceinline__ void sym_times(const double *m, const double *x, doub
   y[2] = m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
 }
 
-// contrib[f][c] (3 corners x 3) of the match gradient
+// contrib[f][c] (3 corners x 3) of the match gradient (nsub meshes as in k_face_geometry)
 __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V, long nv,
-                                                      const int *F, long nf, const double *N,
-                                                      const double *ktS, const double *gS,
-                                                      const double *ktT, const double *gT,
-                                                      double *contrib) {
+                                                      const int *F, long nf, int nsub,
+                                                      const double *N, const double *ktS,
+                                                      const double *gS, const double *ktT,
+                                                      const double *gT, double *contrib) {
   const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf * nsub) return;
+  const long sub = f / nf, fl = f - sub * nf;
+  V += sub * nv * 3;
   double *out = contrib + 9 * f;
   for (int k = 0; k < 9; k++) out[k] = 0.0;
-  const int i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+  const int i0 = F[3 * fl], i1 = F[3 * fl + 1], i2 = F[3 * fl + 2];
   if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) return;
   const int dv = kind == DFM_CURRENT ? 3 : 6;
   double e[3][3], kt[3], gk[3];
@@ -168,20 +174,44 @@ __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V,
   }
 }
 
-// grad[v] = sum of the corner contributions incident to v, in incidence order
+// grad[v] = scale * sum of the corner contributions incident to v, in incidence order
+// (nsub meshes sharing one incidence list of ncorner corners per mesh)
 __global__ __launch_bounds__(256) void k_vertex_gather(const int *offsets, const int *inc,
-                                                       long nv, long ncorner,
-                                                       const double *contrib, double *grad) {
+                                                       long nv, long ncorner, int nsub,
+                                                       const double *contrib, double scale,
+                                                       double *grad) {
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
+  if (v >= nv * nsub) return;
+  const long sub = v / nv, vl = v - sub * nv;
+  contrib += sub * ncorner * 3;
   double g[3] = {0.0, 0.0, 0.0};
-  const long b = max(offsets[v], 0), e = min((long)offsets[v + 1], ncorner);
+  const long b = max(offsets[vl], 0), e = min((long)offsets[vl + 1], ncorner);
   for (long i = b; i < e; i++) {
     const int c = inc[i];
     if (c < 0 || c >= ncorner) continue;
     for (int k = 0; k < 3; k++) g[k] += contrib[3 * (long)c + k];
   }
-  for (int k = 0; k < 3; k++) grad[3 * v + k] = g[k];
+  for (int k = 0; k < 3; k++) grad[3 * v + k] = scale * g[k];
+}
+
+// out[s] = sum over the rows of segment s of dot(X_row, Y_row) (dv columns); segments of
+// seg_len rows, or [off[s], off[s + 1]) when off is given.  One workgroup per segment.
+__global__ __launch_bounds__(256) void k_seg_dot(const double *X, const double *Y, int dv,
+                                                 long seg_len, const long *off, double *out) {
+  __shared__ double sh[256];
+  const int sub = blockIdx.x;
+  const long lo = off ? off[sub] : sub * seg_len;
+  const long hi = off ? off[sub + 1] : lo + seg_len;
+  double s = 0.0;
+  for (long f = lo + threadIdx.x; f < hi; f += 256)
+    for (int k = 0; k < dv; k++) s += X[dv * f + k] * Y[dv * f + k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[sub] = sh[0];
 }
 
 size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -257,10 +287,10 @@ int geometry(hipStream_t st, int kind, const Layout &L, const double *vs, long n
   if (rc) return rc;
   if (nfs > 0)
     hipLaunchKernelGGL(k_face_geometry, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
-                       nfs, L.cS, L.wS, L.aS, L.nS, L.bad);
+                       nfs, 1, L.cS, L.wS, L.aS, L.nS, L.bad);
   if (nft > 0)
     hipLaunchKernelGGL(k_face_geometry, dim3(blocks_for(nft)), dim3(256), 0, st, kind, vt, nvt, ft,
-                       nft, L.cT, L.wT, L.aT, L.nT, L.bad);
+                       nft, 1, L.cT, L.wT, L.aT, L.nT, L.bad);
   return launched("k_face_geometry");
 }
 
@@ -273,6 +303,40 @@ int report_bad(hipStream_t st, const Layout &L) {
 }
 
 }  // namespace
+
+int surf_geometry(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
+                  int nsub, double *C, double *W, double *A, double *N, double *bad) {
+  if (nf * nsub <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_face_geometry, dim3(blocks_for(nf * nsub)), dim3(256), 0, st, kind, V, nv,
+                     F, nf, nsub, C, W, A, N, bad);
+  return launched("k_face_geometry");
+}
+
+int surf_contrib(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
+                 int nsub, const double *N, const double *ktS, const double *gS,
+                 const double *ktT, const double *gT, double *contrib) {
+  if (nf * nsub <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_face_contrib, dim3(blocks_for(nf * nsub)), dim3(256), 0, st, kind, V, nv,
+                     F, nf, nsub, N, ktS, gS, ktT, gT, contrib);
+  return launched("k_face_contrib");
+}
+
+int surf_gather(hipStream_t st, const int *offsets, const int *inc, long nv, long nf, int nsub,
+                const double *contrib, double scale, double *grad) {
+  if (nv * nsub <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_vertex_gather, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets, inc,
+                     nv, 3 * nf, nsub, contrib, scale, grad);
+  return launched("k_vertex_gather");
+}
+
+int seg_dot(hipStream_t st, const double *X, const double *Y, int dv, int nsub, long seg_len,
+            const long *off, double *out) {
+  if (nsub <= 0) return DFM_OK;
+  hipLaunchKernelGGL(k_seg_dot, dim3((unsigned)nsub), dim3(256), 0, st, X, Y, dv, seg_len, off,
+                     out);
+  return launched("k_seg_dot");
+}
+
 }  // namespace dfm
 
 using namespace dfm;
@@ -362,10 +426,10 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, c
     return rc;
   if (nfs > 0)
     hipLaunchKernelGGL(k_face_contrib, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
-                       nfs, L.nS, L.k1, L.g1, L.k2, L.g2, L.contrib);
+                       nfs, 1, L.nS, L.k1, L.g1, L.k2, L.g2, L.contrib);
   if (nvs > 0)
     hipLaunchKernelGGL(k_vertex_gather, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
-                       incidences, nvs, 3 * nfs, L.contrib, grad);
+                       incidences, nvs, 3 * nfs, 1, L.contrib, 1.0, grad);
   if ((rc = launched("gradient assembly"))) return rc;
   return report_bad(st, L);
 }

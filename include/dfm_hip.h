@@ -312,6 +312,58 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int precisi
                                long nvt, const int *ft, long nft, double *grad, void *work,
                                size_t work_bytes);
 
+/* ---- deterministic atlas of surfaces ---------------------------------------------------
+ * Log-likelihood and gradient of DeterministicAtlas for surface subjects
+ * (DeterministicAtlas::UpdateFixedEffectsAndComputeCompleteLogLikelihood and
+ * ::ComputeCompleteLogLikelihoodGradient, DeterministicAtlas.cxx:100-151, 159-281, with
+ * AbstractAtlas::ComputeDataTermGradient(Subject) and ::ConvolveGradTemplate,
+ * AbstractAtlas.cxx:490-620, 727-751; RKHS regularity):
+ *   loglik[0] = -1/2 sum_s r_s / data_sigma_squared, r_s = match(phi_s(template), target_s)
+ *   loglik[1] = -1/2 sum_s a_s^T K(cp, cp) a_s
+ * and the gradients of loglik[0] + loglik[1] with respect to the momenta a_s, the shared
+ * control points cp and the template vertices (optionally smoothed).  The adjoint is the
+ * exact discrete adjoint of the shoot + flow scheme (the reference's continuous adjoint
+ * agrees with it to O(dt)).  Every subject runs in the same launches.
+ * dfm_atlas_create allocates and uploads everything (template topology, target meshes and
+ * their squared norms, the working memory; dfm_atlas_device_bytes reports the latter);
+ * dfm_atlas_gradient then allocates nothing and launches kernels only, so it can be
+ * captured in a HIP graph.  Calls on one atlas must be ordered on one stream. */
+typedef struct dfm_atlas dfm_atlas;
+
+typedef struct dfm_atlas_desc {
+  int kind;                     /* DFM_CURRENT or DFM_VARIFOLD */
+  int precision;                /* DFM_FP64, DFM_FP32C or DFM_FP32X */
+  int scheme;                   /* DFM_EULER or DFM_HEUN shooting */
+  int improved_euler;           /* Heun (1) or Euler (0) landmark flow */
+  int number_of_time_points;    /* T >= 1 */
+  int checkpoint_every;         /* adjoint checkpoint interval >= 1 */
+  double t0, tN;
+  double deformation_width;     /* kernel width of the deformation */
+  double attachment_width;      /* kernel width of the current / varifold norm */
+  double data_sigma_squared;    /* noise variance of the data term */
+  double smoothing_width;       /* template gradient smoothing width; <= 1e-20: none */
+  long n_control_points;
+  long n_template_vertices;
+  long n_template_faces;
+  const int *template_faces;    /* host, n_template_faces x 3 */
+  int nsubjects;                /* 1 .. 65535 */
+  const long *target_vertex_offsets;  /* host, nsubjects + 1 */
+  const long *target_face_offsets;    /* host, nsubjects + 1 */
+  const double *target_vertices;      /* host, concatenated, x 3 */
+  const int *target_faces;            /* host, concatenated, indices local to each target */
+} dfm_atlas_desc;
+
+int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **atlas);
+int dfm_atlas_destroy(dfm_atlas *atlas);
+size_t dfm_atlas_device_bytes(const dfm_atlas *atlas);
+/* Device pointers: cp (n x 3), momenta (nsubjects x n x 3), template_vertices (nv x 3);
+ * outputs loglik (2 doubles), residuals (nsubjects, may be NULL), grad_cp (n x 3, may be
+ * NULL), grad_momenta (nsubjects x n x 3, may be NULL), grad_template (nv x 3, may be
+ * NULL). */
+int dfm_atlas_gradient(dfm_atlas *atlas, void *stream, const double *cp, const double *momenta,
+                       const double *template_vertices, double *loglik, double *residuals,
+                       double *grad_cp, double *grad_momenta, double *grad_template);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,3 +95,18 @@ def test_mesh_incidence_is_host_only_and_checks_indices():
         mesh_incidence(np.array([[0, 1, 5]], dtype=np.int32), 5)
     with pytest.raises(_lib.DfmError, match="outside"):
         mesh_incidence(np.array([[0, -1, 2]], dtype=np.int32), 5)
+
+
+def test_atlas_rejects_bad_meshes_before_touching_the_gpu():
+    """dfm_atlas_create validates the template and target topology on the host."""
+    import numpy as np
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.atlas import HIPDeterministicAtlas
+    F = np.array([[0, 1, 2], [0, 2, 3]])
+    V = np.zeros((4, 3))
+    with pytest.raises(_lib.DfmError, match="outside"):
+        HIPDeterministicAtlas(F, 4, 5, [(V, np.array([[0, 1, 4]]))])
+    with pytest.raises(_lib.DfmError, match="outside"):
+        HIPDeterministicAtlas(np.array([[0, 1, 7]]), 4, 5, [(V, F)])
+    with pytest.raises(_lib.DfmError, match="positive"):
+        HIPDeterministicAtlas(F, 4, 5, [(V, F)], deformation_width=0.0)
