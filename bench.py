@@ -18,6 +18,18 @@ Beside the headline the run reports:
     give the FP32-compensated path's error against the FP64 oracle on those rows.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--m M]
+
+Other workloads (one JSON line each, their own metric; the default is the headline):
+    --workload shooting      landmark geodesic shooting, 1,000 control points, sigma 0.2,
+                             10 steps, Euler and RK2, GPU FP64/FP32C vs the CPU oracle
+    --workload registration  varifold registration gradient: icosphere template with
+                             20,480 triangles (radius 1), bump-displaced target, 4,096
+                             control points on a 16^3 grid, 20 steps, checkpoints every 5,
+                             adjoint gradient vs central differences (eps 1e-4)
+    --workload atlas         batched atlas gradient: 64 subjects of 2,000-40,000 triangles,
+                             2,048 shared control points, eager and HIP-graph replay
+    --workload varifold      varifold match + gradient between two sets of 1,000,000
+                             oriented sphere samples, sigma 0.02, FP32C vs FP64
 """
 from __future__ import annotations
 
@@ -46,6 +58,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="kernel",
+                    choices=["kernel", "shooting", "registration", "atlas", "varifold"])
+    ap.add_argument("--subjects", type=int, default=64)
     return ap.parse_args()
 
 
@@ -63,6 +78,9 @@ def main():
     if device.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload != "kernel":
+        print(json.dumps(WORKLOADS[args.workload](args)), flush=True)
+        return
     L = _lib.load()
     N, M, sigma = args.n, args.m, args.sigma
 
@@ -218,6 +236,232 @@ def main():
         "error_vs_fp64_oracle": err_oracle,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------------------
+# other workloads
+
+
+def _line(metric, value, unit, hib, ms, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "higher_is_better": hib,
+           "ms_per_step": ms, "vs_baseline": None, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def _nrel(ref, got):
+    ref, got = np.asarray(ref), np.asarray(got)
+    return float(np.linalg.norm(ref - got) / max(np.linalg.norm(ref), 1e-300))
+
+
+def _profiled(L, fn, reps):
+    """Wall seconds per call and pair-sum kernel seconds per call (HIP events)."""
+    import ctypes
+    L.dfm_profile_read(None, None)
+    tot, cnt = ctypes.c_double(), ctypes.c_long()
+    L.dfm_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    t1 = time.perf_counter()
+    L.dfm_profile_enable(0)
+    L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt))
+    return (t1 - t0) / reps, tot.value / 1e3 / reps, cnt.value // reps
+
+
+def workload_shooting(args):
+    """Landmark geodesic shooting: 1,000 control points U[0,1]^3, momenta N(0, 0.01^2),
+    sigma 0.2, 10 steps (11 time points); Euler (Diffeos::Shoot) and RK2 (Heun)."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    from oracle import deformations as dfo
+    rng = np.random.default_rng(20261205)
+    n, sigma, T = 1000, 0.2, 11
+    p0 = rng.uniform(size=(n, 3))
+    a0 = rng.normal(scale=0.01, size=(n, 3))
+    L = _lib.load()
+    res = {}
+    for scheme in ("euler", "heun"):
+        ta = time.perf_counter()
+        P, A = dfo.shoot(p0, a0, sigma, T, scheme=scheme)
+        cpu_s = time.perf_counter() - ta
+        row = {"cpu_oracle_ms": cpu_s * 1e3}
+        for prec in ("fp64", "fp32c"):
+            df = HIPDiffeos(3, prec)
+            df.SetKernelWidth(sigma)
+            df.SetNumberOfTimePoints(T)
+            df.SetShootingScheme(scheme)
+            df.SetStartPositions(p0)
+            df.SetStartMomentas(a0)
+            df.Update()
+            wall, kern, launches = _profiled(L, df.Update, 5)
+            row[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3,
+                         "pairsum_launches": launches,
+                         "rel_err_final_positions_vs_oracle":
+                             _nrel(P[-1] - p0, df.GetTrajectoryPositions()[-1] - p0),
+                         "rel_err_final_momenta_vs_oracle": _nrel(A[-1], df.GetTrajectoryMomentas()[-1])}
+        res[scheme] = row
+    e = res["euler"]
+    return _line("geodesic shooting time (1,000 control points, 10 steps)", e["fp32c"]["ms"], "ms",
+                 False, e["fp32c"]["ms"],
+                 {"workload": "landmark shooting: 1,000 control points U[0,1]^3, momenta "
+                              "N(0, 0.01^2), sigma 0.2, 10 steps, host arrays in and out"},
+                 dtype="f32", schemes=res,
+                 cpu_baseline={"value": e["cpu_oracle_ms"], "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "whole workload, Euler, FP64 oracle (one thread)"})
+
+
+def _grid(lo, hi, counts):
+    axes = [np.linspace(lo, hi, c) for c in counts]
+    return np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+
+
+def workload_registration(args):
+    """Varifold registration gradient at the scale of one subject."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.atlas import HIPDeterministicAtlas
+    from deformetrica_code_amd.device import DeviceArray
+    from deformetrica_code_amd.meshes import bump_field, icosphere
+    rng = np.random.default_rng(20261206)
+    V, F = icosphere(5, 1.0)
+    target = V + bump_field(V, 8, 0.1, 0.3, rng)
+    cp = _grid(-1.5, 1.5, (16, 16, 16))
+    mom = rng.normal(scale=0.01, size=(1, cp.shape[0], 3))
+    L = _lib.load()
+    kw = dict(kind="varifold", deformation_width=0.3, attachment_width=0.1, data_sigma_squared=1.0,
+              number_of_time_points=21, checkpoint_every=5)
+    out = {}
+    for prec in ("fp32c", "fp64"):
+        A = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], [(target, F)], precision=prec, **kw)
+        dcp, dmom, dV = (DeviceArray.from_numpy(x, A.stream) for x in (cp, mom, V))
+        run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
+        run()
+        wall, kern, launches = _profiled(L, run, 3 if prec == "fp32c" else 1)
+        out[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "pairsum_launches": launches,
+                     "grads": [A.grad_momenta.numpy(A.stream), A.grad_cp.numpy(A.stream),
+                               A.grad_template.numpy(A.stream)],
+                     "atlas": A}
+    # central differences of the FP64 log-likelihood along random directions (eps = 1e-4)
+    A64 = out["fp64"]["atlas"]
+    f = lambda c, m, v: float(np.sum(A64.ComputeLogLikelihood(c, m, v)[0]))
+    fd_err = {}
+    drng = np.random.default_rng(5)
+    for name, g in zip(("momenta", "control_points", "template"), out["fp64"]["grads"]):
+        D = drng.normal(size=g.shape)
+        D /= np.linalg.norm(D)
+        e = 1e-4
+
+        def at(h):
+            if name == "momenta":
+                return cp, mom + h * D, V
+            if name == "control_points":
+                return cp + h * D, mom, V
+            return cp, mom, V + h * D
+
+        fd = (f(*at(e)) - f(*at(-e))) / (2 * e)
+        fd_err[name] = abs(float(np.sum(g * D)) - fd) / max(abs(fd), 1e-300)
+    errs = {k: _nrel(a, b) for k, a, b in zip(("momenta", "control_points", "template"),
+                                              out["fp64"]["grads"], out["fp32c"]["grads"])}
+    r = out["fp32c"]
+    return _line("registration gradient time (varifold, 20,480 triangles, 4,096 control points, 20 steps)",
+                 r["ms"], "ms", False, r["ms"],
+                 {"workload": "icosphere template 20,480 triangles radius 1, target displaced by 8 "
+                              "Gaussian bumps (amplitude 0.1, width 0.3), 16^3 control grid over "
+                              "[-1.5,1.5]^3, deformation sigma 0.3, varifold sigma 0.1, 21 time "
+                              "points, checkpoints every 5"},
+                 dtype="f32", pairsum_kernel_ms=r["pairsum_kernel_ms"],
+                 pairsum_launches=r["pairsum_launches"], fp64_ms=out["fp64"]["ms"],
+                 fp32c_gradient_rel_err_vs_fp64=errs,
+                 fp64_gradient_vs_central_differences=fd_err)
+
+
+def workload_atlas(args):
+    """Batched deterministic-atlas gradient over many subjects of uneven size."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.atlas import HIPDeterministicAtlas
+    from deformetrica_code_amd.device import DeviceArray
+    from deformetrica_code_amd.meshes import bump_field, decimated_icosphere, icosphere
+    S = args.subjects
+    rng = np.random.default_rng(20261207)
+    V, F = icosphere(5, 1.0)
+    targets = []
+    for s in range(S):
+        tv, tf = decimated_icosphere(int(rng.integers(2000, 40001)), rng)
+        targets.append((tv + bump_field(tv, 8, 0.1, 0.3, rng), tf))
+    cp = _grid(-1.5, 1.5, (16, 16, 8))
+    mom = rng.normal(scale=0.01, size=(S, cp.shape[0], 3))
+    L = _lib.load()
+    A = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], targets, precision="fp32c",
+                              kind="varifold", deformation_width=0.25, attachment_width=0.1,
+                              data_sigma_squared=1.0, number_of_time_points=11, checkpoint_every=5)
+    dcp, dmom, dV = (DeviceArray.from_numpy(x, A.stream) for x in (cp, mom, V))
+    run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
+    run()
+    wall, kern, launches = _profiled(L, run, 3)
+    g = A.capture(dcp, dmom, dV)
+    g.launch()
+    A.stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        g.launch()
+    A.stream.synchronize()
+    graph_ms = (time.perf_counter() - t0) / 3 * 1e3
+    faces = [t[1].shape[0] for t in targets]
+    return _line("batched atlas gradient time (%d subjects)" % S, graph_ms, "ms", False, graph_ms,
+                 {"workload": "%d subjects, decimated icospheres of %d-%d triangles (mean %d) "
+                              "displaced by 8 Gaussian bumps each, template icosphere 20,480 "
+                              "triangles, 2,048 control points (16x16x8 grid), momenta N(0, 0.01^2), "
+                              "deformation sigma 0.25, varifold sigma 0.1, 11 time points, "
+                              "checkpoints every 5" % (S, min(faces), max(faces), int(np.mean(faces)))},
+                 dtype="f32", eager_ms=wall * 1e3, pairsum_kernel_ms=kern * 1e3,
+                 pairsum_launches=launches, device_bytes=A.device_bytes(),
+                 loglik=[float(v) for v in A.loglik.numpy(A.stream)])
+
+
+def workload_varifold(args):
+    """Varifold match + gradient between two sets of 1M oriented sphere samples."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.device import DeviceArray, Stream, Workspace
+    from deformetrica_code_amd.meshes import sphere_samples
+    from deformetrica_code_amd.surfaces import samples_match_gradient
+    n = args.n
+    rng = np.random.default_rng(20261208)
+    cs, Ns = sphere_samples(n, rng)
+    ct, Nt = sphere_samples(n, rng)
+    L = _lib.load()
+    st = Stream()
+    ws = Workspace()
+    d = [DeviceArray.from_numpy(v, st) for v in (cs, Ns, ct, Nt)]
+    res = {}
+    for prec, reps in (("fp32c", 2), ("fp64", 1)):
+        run = lambda: (samples_match_gradient("varifold", 0.02, *d, precision=prec, stream=st,
+                                              workspace=ws), st.synchronize())
+        out = run()[0]
+        wall, kern, launches = _profiled(L, run, reps)
+        res[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "launches": launches,
+                     "match": float(out[0].numpy(st)[0]), "grad_c": out[1].numpy(st),
+                     "grad_n": out[2].numpy(st)}
+    pairs = 3.0 * n * n
+    flops = (2 * 41 + 22) * float(n) * n        # two fused Dv=6 sums + one Dv=6 Convolve
+    r = res["fp32c"]
+    return _line("varifold data term pair-evals/s (1M x 1M samples, match + gradient)",
+                 pairs / (r["ms"] / 1e3), "pair-evals/s", True, r["ms"],
+                 {"workload": "two sets of %d oriented samples of the unit sphere (normals radial "
+                              "x log-normal(-9, 0.3) areas), varifold kernel, sigma 0.02: "
+                              "|S|^2 + |T|^2 - 2<S,T> and d/d centres, d/d normals" % n},
+                 dtype="f32",
+                 roofline={"bound": "valu", "achieved": flops / (r["pairsum_kernel_ms"] / 1e3) / 1e12,
+                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                           "frac": flops / (r["pairsum_kernel_ms"] / 1e3) / FP32_PEAK,
+                           "flops_per_pair": "41 (fused Dv=6) x2 + 22 (Dv=6 Convolve)"},
+                 fp64_ms=res["fp64"]["ms"], speedup_fp32c_over_fp64=res["fp64"]["ms"] / r["ms"],
+                 match_rel_err_vs_fp64=abs(r["match"] - res["fp64"]["match"]) / abs(res["fp64"]["match"]),
+                 grad_centres_rel_err_vs_fp64=_nrel(res["fp64"]["grad_c"], r["grad_c"]),
+                 grad_normals_rel_err_vs_fp64=_nrel(res["fp64"]["grad_n"], r["grad_n"]))
+
+
+WORKLOADS = {"shooting": workload_shooting, "registration": workload_registration,
+             "atlas": workload_atlas, "varifold": workload_varifold}
 
 
 if __name__ == "__main__":

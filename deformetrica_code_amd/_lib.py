@@ -108,7 +108,11 @@ def load():
     L.dfm_surface_match.argtypes = [vp, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
     L.dfm_surface_match_gradient.argtypes = [vp, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l, vp,
                                              vp, sz]
-    for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient"):
+    L.dfm_samples_workspace.argtypes = [i, l, l, i, i]
+    L.dfm_samples_workspace.restype = sz
+    L.dfm_samples_match_gradient.argtypes = [vp, i, d, i, vp, vp, l, vp, vp, l, vp, vp, vp, vp, sz]
+    for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient",
+                 "dfm_samples_match_gradient"):
         getattr(L, name).restype = i
     L.dfm_profile_enable.argtypes = [i]
     L.dfm_profile_enable.restype = i

@@ -214,7 +214,89 @@ __global__ __launch_bounds__(256) void k_seg_dot(const double *X, const double *
   if (threadIdx.x == 0) out[sub] = sh[0];
 }
 
+// Oriented samples (centre c, area-weighted normal N): kernel weights W and coefficients
+// A as for faces; Nt = N for currents, N / sqrt(|N| + 1e-20) for varifolds.
+__global__ __launch_bounds__(256) void k_sample_geometry(int kind, const double *N, long n,
+                                                         double *W, double *A, double *Nt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v[3] = {N[3 * i], N[3 * i + 1], N[3 * i + 2]};
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < 3; k++) W[3 * i + k] = A[3 * i + k] = Nt[3 * i + k] = v[k];
+    return;
+  }
+  const double s = sqrt(sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 1e-20);
+  for (int k = 0; k < 3; k++) Nt[3 * i + k] = v[k] /= s;
+  int idx = 0;
+  for (int p = 0; p < 3; p++)
+    for (int q = p; q < 3; q++, idx++) {
+      W[6 * i + idx] = v[p] * v[q];
+      A[6 * i + idx] = (p == q ? 1.0 : 2.0) * v[p] * v[q];
+    }
+}
+
+// d match / d c_i = 2 (g1_i - g2_i);  d match / d N_i = 2 (k1_i - k2_i) for currents and,
+// for varifolds, the chain rule of Nt = N / s, s = sqrt(|N| + 1e-20), applied to
+// d match / d Nt_i = 4 (S(k1_i) - S(k2_i)) Nt_i, S the symmetric matrix of a weight row:
+//   d/dN = g / s - N (N.g) / (2 s^3 |N|)
+__global__ __launch_bounds__(256) void k_sample_grads(int kind, const double *N,
+                                                      const double *Nt, const double *k1,
+                                                      const double *k2, const double *g1,
+                                                      const double *g2, long n, double *grad_c,
+                                                      double *grad_n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (grad_c)
+    for (int k = 0; k < 3; k++) grad_c[3 * i + k] = 2.0 * (g1[3 * i + k] - g2[3 * i + k]);
+  if (!grad_n) return;
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < 3; k++) grad_n[3 * i + k] = 2.0 * (k1[3 * i + k] - k2[3 * i + k]);
+    return;
+  }
+  double dm[6], g[3];
+  for (int k = 0; k < 6; k++) dm[k] = k1[6 * i + k] - k2[6 * i + k];
+  sym_times(dm, Nt + 3 * i, g);
+  const double *v = N + 3 * i;
+  const double a = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (!(a > 0.0)) {
+    for (int k = 0; k < 3; k++) grad_n[3 * i + k] = 0.0;
+    return;
+  }
+  const double s = sqrt(a + 1e-20);
+  for (int k = 0; k < 3; k++) g[k] *= 4.0;
+  const double ng = v[0] * g[0] + v[1] * g[1] + v[2] * g[2];
+  for (int k = 0; k < 3; k++) grad_n[3 * i + k] = g[k] / s - v[k] * ng / (2.0 * s * s * s * a);
+}
+
 size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct SampleLayout {
+  double *wS, *aS, *tS, *wT, *aT, *tT, *k1, *k2, *kT, *g1, *g2, *p1, *p2, *p3;
+  void *pw;
+  size_t pw_bytes, total;
+  SampleLayout(char *base, int kind, int prec, long ns, long nt, bool gradient) {
+    const int dv = kind == DFM_CURRENT ? 3 : 6;
+    size_t off = 0;
+    auto take = [&](size_t count) {
+      double *p = base ? (double *)(base + off) : nullptr;
+      off += a256(count * sizeof(double));
+      return p;
+    };
+    wS = take(dv * ns); aS = take(dv * ns); tS = take(3 * ns);
+    wT = take(dv * nt); aT = take(dv * nt); tT = take(3 * nt);
+    k1 = take(dv * ns); k2 = take(dv * ns); kT = take(dv * nt);
+    g1 = gradient ? take(3 * ns) : nullptr;
+    g2 = gradient ? take(3 * ns) : nullptr;
+    p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
+    const int op = gradient ? OP_CONVGRAD1 : OP_CONV;
+    pw_bytes = std::max({pairsum_workspace(op, prec, 3, dv, ns, ns),
+                         pairsum_workspace(op, prec, 3, dv, ns, nt),
+                         pairsum_workspace(OP_CONV, prec, 3, dv, nt, nt)});
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
 
 struct Layout {
   double *cS, *wS, *aS, *nS, *cT, *wT, *aT, *nT;
@@ -432,6 +514,59 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, c
                        incidences, nvs, 3 * nfs, 1, L.contrib, 1.0, grad);
   if ((rc = launched("gradient assembly"))) return rc;
   return report_bad(st, L);
+}
+
+size_t dfm_samples_workspace(int kind, long ns, long nt, int precision, int gradient) {
+  if (check_args(kind, precision, 1.0, 0, ns, 0, nt)) return 0;
+  return SampleLayout(nullptr, kind, precision, ns, nt, gradient != 0).total;
+}
+
+int dfm_samples_match_gradient(void *stream, int kind, double sigma, int prec, const double *cs,
+                               const double *Ns, long ns, const double *ct, const double *Nt,
+                               long nt, double *match, double *grad_centers,
+                               double *grad_normals, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_args(kind, prec, sigma, 0, ns, 0, nt);
+  if (rc) return rc;
+  if (!match || (ns > 0 && (!cs || !Ns)) || (nt > 0 && (!ct || !Nt)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  const bool gradient = grad_centers || grad_normals;
+  SampleLayout need(nullptr, kind, prec, ns, nt, gradient);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  SampleLayout L((char *)work, kind, prec, ns, nt, gradient);
+  hipStream_t st = (hipStream_t)stream;
+  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  if (ns > 0)
+    hipLaunchKernelGGL(k_sample_geometry, dim3(blocks_for(ns)), dim3(256), 0, st, kind, Ns, ns,
+                       L.wS, L.aS, L.tS);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_sample_geometry, dim3(blocks_for(nt)), dim3(256), 0, st, kind, Nt, nt,
+                       L.wT, L.aT, L.tT);
+  if ((rc = launched("k_sample_geometry"))) return rc;
+  KParams p = blank();
+  const int op = gradient ? OP_CONVGRAD1 : OP_CONV;
+  // (K W_S)(c_S) [+ Grad1(c_S, A_S)] and (K W_T)(c_S) [+ Grad1(c_S, A_S)]
+  p.x = cs; p.ta = L.aS; p.nx = ns; p.y = cs; p.w = L.wS; p.ny = ns; p.out0 = L.k1; p.out1 = L.g1;
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, ns * dv, gradient ? ns * 3 : 0, L.pw,
+                    L.pw_bytes)))
+    return rc;
+  p.y = ct; p.w = L.wT; p.ny = nt; p.out0 = L.k2; p.out1 = L.g2;
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, ns * dv, gradient ? ns * 3 : 0, L.pw,
+                    L.pw_bytes)))
+    return rc;
+  // (K W_T)(c_T): the target's squared norm
+  p = blank();
+  p.x = ct; p.nx = nt; p.y = ct; p.w = L.wT; p.ny = nt; p.out0 = L.kT;
+  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nt * dv, 0, L.pw, L.pw_bytes))) return rc;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, ns, dv, L.p1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.kT, L.aT, nt, dv, L.p2);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aS, ns, dv, L.p3);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, st, L.p1, 1.0, L.p2, 1.0, L.p3, -2.0, match);
+  if (gradient && ns > 0)
+    hipLaunchKernelGGL(k_sample_grads, dim3(blocks_for(ns)), dim3(256), 0, st, kind, Ns, L.tS,
+                       L.k1, L.k2, L.g1, L.g2, ns, grad_centers, grad_normals);
+  return launched("samples match");
 }
 
 }  // extern "C"

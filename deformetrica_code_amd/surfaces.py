@@ -62,6 +62,24 @@ def surface_match_gradient(kind, sigma, vs, fs, offsets, incidences, vt, ft, pre
     return out
 
 
+def samples_match_gradient(kind, sigma, cs, Ns, ct, Nt, precision="fp32c", gradient=True,
+                           stream=None, workspace=None):
+    """Data term between oriented samples (DeviceArrays of centres and area-weighted
+    normals): (match (1,), d/d cs, d/d Ns) as DeviceArrays (gradients None if not asked)."""
+    L = _lib.load()
+    k, prec = _KIND[kind], _lib.precision_code(precision)
+    ws = workspace or Workspace()
+    ns, nt = cs.shape[0], ct.shape[0]
+    wp, wb = ws.get(L.dfm_samples_workspace(k, ns, nt, prec, int(gradient)))
+    out = DeviceArray((1,))
+    gc = DeviceArray((ns, 3)) if gradient else None
+    gn = DeviceArray((ns, 3)) if gradient else None
+    _lib.check(L.dfm_samples_match_gradient(_sp(stream), k, float(sigma), prec, cs.ptr, Ns.ptr, ns,
+                                            ct.ptr, Nt.ptr, nt, out.ptr, gc.ptr if gc else None,
+                                            gn.ptr if gn else None, wp, wb))
+    return out, gc, gn
+
+
 class _SurfaceMesh:
     KIND = None
     TYPE_ERROR = "AbstractGeometries types mismatched"
@@ -164,4 +182,4 @@ class HIPNonOrientedSurfaceMesh(_SurfaceMesh):
 
 
 __all__ = ["HIPOrientedSurfaceMesh", "HIPNonOrientedSurfaceMesh", "mesh_incidence",
-           "surface_match", "surface_match_gradient", "Stream"]
+           "surface_match", "surface_match_gradient", "samples_match_gradient", "Stream"]

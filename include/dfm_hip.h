@@ -312,6 +312,19 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int precisi
                                long nvt, const int *ft, long nft, double *grad, void *work,
                                size_t work_bytes);
 
+/* Data term between two sets of oriented samples (centre c, area-weighted normal N; the
+ * reference's m_Centers / m_Normals of a mesh, OrientedSurfaceMesh.cxx:298-326 and
+ * NonOrientedSurfaceMesh.cxx:269-308, given directly): *match (device double) =
+ * |S|^2 + |T|^2 - 2 <S, T> with the current kernel K(c_i, c_j) N_i.N_j or the varifold
+ * kernel K(c_i, c_j) (N_i.N_j)^2 / (|N_i| |N_j|); grad_centers and grad_normals (ns x 3,
+ * either may be NULL) are d match / d c_S and d match / d N_S.  The mesh entry points
+ * above chain these to vertex positions. */
+size_t dfm_samples_workspace(int kind, long ns, long nt, int precision, int gradient);
+int dfm_samples_match_gradient(void *stream, int kind, double sigma, int precision,
+                               const double *cs, const double *Ns, long ns, const double *ct,
+                               const double *Nt, long nt, double *match, double *grad_centers,
+                               double *grad_normals, void *work, size_t work_bytes);
+
 /* ---- deterministic atlas of surfaces ---------------------------------------------------
  * Log-likelihood and gradient of DeterministicAtlas for surface subjects
  * (DeterministicAtlas::UpdateFixedEffectsAndComputeCompleteLogLikelihood and

@@ -167,3 +167,19 @@ def brute_force_varifold(Vs, Fs, Vt, Ft, width):
         return float(np.sum(np.exp(-d2 / width ** 2) * (n1 @ n2.T) ** 2 / np.outer(a1, a2)))
 
     return ip(cs, ns, cs, ns) + ip(ct, nt, ct, nt) - 2 * ip(cs, ns, ct, nt)
+
+
+def brute_force_samples(cs, Ns, ct, Nt, width, kind):
+    """Data term between oriented samples (centres, area-weighted normals) by the double sum
+    of its definition: current kernel K N_i.N_j, varifold kernel K (N_i.N_j)^2/(|N_i||N_j|)
+    with |N| + 1e-20 under the square root as in NonOrientedSurfaceMesh.cxx:269-308."""
+    def ip(c1, n1, c2, n2):
+        d2 = np.sum((c1[:, None, :] - c2[None, :, :]) ** 2, axis=2)
+        K = np.exp(-d2 / width ** 2)
+        if kind == "current":
+            return float(np.sum(K * (n1 @ n2.T)))
+        u1 = n1 / np.sqrt(np.linalg.norm(n1, axis=1, keepdims=True) + 1e-20)
+        u2 = n2 / np.sqrt(np.linalg.norm(n2, axis=1, keepdims=True) + 1e-20)
+        return float(np.sum(K * (u1 @ u2.T) ** 2))
+
+    return ip(cs, Ns, cs, Ns) + ip(ct, Nt, ct, Nt) - 2 * ip(cs, Ns, ct, Nt)

@@ -59,3 +59,39 @@ def test_gradient_matches_central_differences(kind):
             m[i, k] -= eps
             fd[i, k] = (match(p, Fs, Vt, Ft, w) - match(m, Fs, Vt, Ft, w)) / (2 * eps)
     assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["current", "varifold"])
+def test_sample_brute_force_equals_mesh_match(kind):
+    """The oriented-sample double sum at a mesh's centres and area-weighted normals is the
+    mesh data term."""
+    from oracle import surfaces as so
+    Vs, Fs, Vt, Ft = _meshes(2)
+    cs, ns = so.current_geometry(Vs, Fs)
+    ct, nt = so.current_geometry(Vt, Ft)
+    m = getattr(so, kind + "_match")(Vs, Fs, Vt, Ft, 0.3)
+    assert abs(so.brute_force_samples(cs, ns, ct, nt, 0.3, kind) - m) <= 1e-12 * abs(m)
+
+
+def test_icosphere_counts_and_orientation():
+    from deformetrica_code_amd.meshes import icosphere, bump_field, decimated_icosphere
+    from oracle import surfaces as so
+    for L in range(4):
+        V, F = icosphere(L, 0.5)
+        assert V.shape == (10 * 4 ** L + 2, 3) and F.shape == (20 * 4 ** L, 3)
+        np.testing.assert_allclose(np.linalg.norm(V, axis=1), 0.5, rtol=1e-14)
+    # same orientation as the reference icosphere (normals point inwards) and closed
+    c, n = so.current_geometry(*icosphere(3, 0.5))
+    cr, nr = so.current_geometry(*so.read_vtk_polydata(os.path.join(MESH, "sphere1280.vtk")))
+    assert np.all(np.sum(n * c, 1) < 0) and np.all(np.sum(nr * cr, 1) < 0)
+    assert np.linalg.norm(n.sum(0)) < 1e-12
+    rng = np.random.default_rng(0)
+    V, F = decimated_icosphere(3000, rng)
+    assert F.shape == (3000, 3) and F.max() < V.shape[0]
+    # one bump: the field at its centre is its amplitude vector
+    rng = np.random.default_rng(1)
+    c = rng.normal(size=(1, 3))
+    c /= np.linalg.norm(c)
+    field = bump_field(np.vstack([c, 10 * c]), 1, 0.1, 0.3, np.random.default_rng(1))
+    np.testing.assert_allclose(np.linalg.norm(field[0]), 0.1, rtol=1e-12)
+    assert np.linalg.norm(field[1]) < 1e-100

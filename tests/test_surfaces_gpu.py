@@ -140,3 +140,45 @@ def test_out_of_range_face_is_an_error_on_the_device_path():
             DeviceArray.from_numpy(Vt), DeviceArray.from_numpy(Ft, dtype=np.int32)]
     with pytest.raises(_lib.DfmError, match="outside"):
         surface_match("current", 0.3, *args)
+
+
+def _samples(seed, ns, nt):
+    from deformetrica_code_amd.meshes import sphere_samples
+    rng = np.random.default_rng(seed)
+    cs, Ns = sphere_samples(ns, rng, -3.0, 0.3)
+    ct, Nt = sphere_samples(nt, rng, -3.0, 0.3)
+    return cs, Ns, ct + 0.02, Nt
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("kind", ["current", "varifold"])
+def test_samples_match_and_gradient(kind, prec):
+    """Oriented-sample data term against the oracle's double sum; d/d centres and
+    d/d normals against central differences of that double sum."""
+    from deformetrica_code_amd.device import DeviceArray
+    from deformetrica_code_amd.surfaces import samples_match_gradient
+    from oracle import surfaces as so
+    cs, Ns, ct, Nt = _samples(11, 700, 500)
+    w = 0.15
+    m, gc, gn = samples_match_gradient(kind, w, *(DeviceArray.from_numpy(v) for v in (cs, Ns, ct, Nt)),
+                                       precision=prec)
+    ref = so.brute_force_samples(cs, Ns, ct, Nt, w, kind)
+    scale = so.brute_force_samples(cs, Ns, ct[:0], Nt[:0], w, kind) + \
+        so.brute_force_samples(ct, Nt, cs[:0], Ns[:0], w, kind)
+    tol = 1e-10 if prec == "fp64" else 1e-6
+    assert abs(m.numpy()[0] - ref) <= tol * scale
+    gc, gn = gc.numpy(), gn.numpy()
+    rng = np.random.default_rng(12)
+    eps = 1e-6
+    for i in rng.choice(cs.shape[0], 5, replace=False):
+        for k in range(3):
+            for arr, g in ((cs, gc), (Ns, gn)):
+                p, q = arr.copy(), arr.copy()
+                p[i, k] += eps * (1 if arr is cs else abs(Ns[i]).max())
+                q[i, k] -= eps * (1 if arr is cs else abs(Ns[i]).max())
+                h = p[i, k] - q[i, k]
+                args_p = (p, Ns) if arr is cs else (cs, p)
+                args_q = (q, Ns) if arr is cs else (cs, q)
+                fd = (so.brute_force_samples(*args_p, ct, Nt, w, kind) -
+                      so.brute_force_samples(*args_q, ct, Nt, w, kind)) / h
+                assert abs(g[i, k] - fd) <= (1e-6 if prec == "fp64" else 1e-4) * np.abs(g).max()
