@@ -269,6 +269,21 @@ def _profiled(L, fn, reps):
     return (t1 - t0) / reps, tot.value / 1e3 / reps, cnt.value // reps
 
 
+def _tuned(L, fn, reps, explore=30):
+    """Explore plans with the autotuner over `explore` calls, then time `reps` calls with
+    the tuned plans (autotuner back to heuristics afterwards)."""
+    L.dfm_autotune_reset()
+    L.dfm_autotune(1)
+    for _ in range(explore):
+        fn()
+    L.dfm_autotune(2)
+    try:
+        return _profiled(L, fn, reps)
+    finally:
+        L.dfm_autotune(0)
+        L.dfm_autotune_reset()
+
+
 def workload_shooting(args):
     """Landmark geodesic shooting: 1,000 control points U[0,1]^3, momenta N(0, 0.01^2),
     sigma 0.2, 10 steps (11 time points); Euler (Diffeos::Shoot) and RK2 (Heun)."""
@@ -295,8 +310,10 @@ def workload_shooting(args):
             df.SetStartMomentas(a0)
             df.Update()
             wall, kern, launches = _profiled(L, df.Update, 5)
+            twall, tkern, _ = _tuned(L, df.Update, 5)
             row[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3,
-                         "pairsum_launches": launches,
+                         "pairsum_launches": launches, "autotuned_ms": twall * 1e3,
+                         "autotuned_pairsum_kernel_ms": tkern * 1e3,
                          "rel_err_final_positions_vs_oracle":
                              _nrel(P[-1] - p0, df.GetTrajectoryPositions()[-1] - p0),
                          "rel_err_final_momenta_vs_oracle": _nrel(A[-1], df.GetTrajectoryMomentas()[-1])}
@@ -337,7 +354,9 @@ def workload_registration(args):
         run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
         run()
         wall, kern, launches = _profiled(L, run, 3 if prec == "fp32c" else 1)
+        tuned = _tuned(L, run, 3)[:2] if prec == "fp32c" else (None, None)
         out[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "pairsum_launches": launches,
+                     "autotuned_ms": tuned[0] and tuned[0] * 1e3,
                      "grads": [A.grad_momenta.numpy(A.stream), A.grad_cp.numpy(A.stream),
                                A.grad_template.numpy(A.stream)],
                      "atlas": A}
@@ -370,7 +389,8 @@ def workload_registration(args):
                               "[-1.5,1.5]^3, deformation sigma 0.3, varifold sigma 0.1, 21 time "
                               "points, checkpoints every 5"},
                  dtype="f32", pairsum_kernel_ms=r["pairsum_kernel_ms"],
-                 pairsum_launches=r["pairsum_launches"], fp64_ms=out["fp64"]["ms"],
+                 pairsum_launches=r["pairsum_launches"], autotuned_ms=r["autotuned_ms"],
+                 fp64_ms=out["fp64"]["ms"],
                  fp32c_gradient_rel_err_vs_fp64=errs,
                  fp64_gradient_vs_central_differences=fd_err)
 
@@ -398,6 +418,13 @@ def workload_atlas(args):
     run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
     run()
     wall, kern, launches = _profiled(L, run, 3)
+    # explore plans, then capture the gradient with the tuned plans
+    L.dfm_autotune_reset()
+    L.dfm_autotune(1)
+    for _ in range(30):
+        run()
+    L.dfm_autotune(2)
+    twall, tkern, _ = _profiled(L, run, 3)
     g = A.capture(dcp, dmom, dV)
     g.launch()
     A.stream.synchronize()
@@ -406,6 +433,9 @@ def workload_atlas(args):
         g.launch()
     A.stream.synchronize()
     graph_ms = (time.perf_counter() - t0) / 3 * 1e3
+    L.dfm_autotune(0)
+    report = _lib.autotune_report()
+    L.dfm_autotune_reset()
     faces = [t[1].shape[0] for t in targets]
     return _line("batched atlas gradient time (%d subjects)" % S, graph_ms, "ms", False, graph_ms,
                  {"workload": "%d subjects, decimated icospheres of %d-%d triangles (mean %d) "
@@ -414,7 +444,10 @@ def workload_atlas(args):
                               "deformation sigma 0.25, varifold sigma 0.1, 11 time points, "
                               "checkpoints every 5" % (S, min(faces), max(faces), int(np.mean(faces)))},
                  dtype="f32", eager_ms=wall * 1e3, pairsum_kernel_ms=kern * 1e3,
-                 pairsum_launches=launches, device_bytes=A.device_bytes(),
+                 pairsum_launches=launches, autotuned_eager_ms=twall * 1e3,
+                 autotuned_pairsum_kernel_ms=tkern * 1e3,
+                 graph="replay of the autotuned gradient", autotune_table=report.splitlines(),
+                 device_bytes=A.device_bytes(),
                  loglik=[float(v) for v in A.loglik.numpy(A.stream)])
 
 

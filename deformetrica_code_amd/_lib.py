@@ -114,6 +114,12 @@ def load():
     for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient",
                  "dfm_samples_match_gradient"):
         getattr(L, name).restype = i
+    L.dfm_autotune.argtypes = [i]
+    L.dfm_autotune.restype = i
+    L.dfm_autotune_reset.argtypes = []
+    L.dfm_autotune_reset.restype = i
+    L.dfm_autotune_report.argtypes = [ctypes.c_char_p, sz]
+    L.dfm_autotune_report.restype = i
     L.dfm_profile_enable.argtypes = [i]
     L.dfm_profile_enable.restype = i
     L.dfm_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
@@ -153,3 +159,16 @@ def precision_code(precision) -> int:
         return PRECISIONS[precision.lower()]
     except KeyError:
         raise ValueError("precision must be one of %s" % sorted(PRECISIONS)) from None
+
+
+def autotune(mode: int) -> None:
+    """0 heuristics, 1 explore then keep the fastest plan per shape, 2 tuned plans only."""
+    check(load().dfm_autotune(int(mode)))
+
+
+def autotune_report() -> str:
+    L = load()
+    n = L.dfm_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.dfm_autotune_report(buf, n + 1)
+    return buf.value.decode()

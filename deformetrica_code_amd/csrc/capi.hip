@@ -176,6 +176,21 @@ int dfm_graph_destroy(void *graph_exec) {
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipGraphExecDestroy: %s", hipGetErrorString(e));
 }
 
+int dfm_autotune(int mode) {
+  clear_error();
+  if (mode < 0 || mode > 2) return set_error(DFM_ERR_ARG, "autotune mode must be 0, 1 or 2");
+  tune_set_mode(mode);
+  return DFM_OK;
+}
+
+int dfm_autotune_reset(void) {
+  clear_error();
+  tune_reset();
+  return DFM_OK;
+}
+
+int dfm_autotune_report(char *buf, size_t len) { return tune_report(buf, len); }
+
 int dfm_profile_enable(int on) {
   profile_enable(on != 0);
   return DFM_OK;

@@ -147,6 +147,11 @@ size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *
 int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
 int launch_op_f32c(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
 int launch_op_f32x(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
+// autotuner of the plan choice (pairsum.hip): mode 0 heuristics, 1 explore + exploit,
+// 2 use tuned choices only
+void tune_set_mode(int mode);
+void tune_reset();
+int tune_report(char *buf, size_t len);
 void profile_enable(bool on);
 int profile_read(double *total_ms, long *launches);
 

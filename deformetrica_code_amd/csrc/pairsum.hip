@@ -6,10 +6,14 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -203,19 +207,55 @@ static int nacc_of(int op, int D, int DV) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// The tunable part of a plan: targets per lane, the wavefront count the source splits
+// aim for, and the fewest source tiles a split may hold.
+struct Choice {
+  int R, waves, min_tiles;
+};
+
+// Heuristic: the op's R, >= 4096 wavefronts (4 per SIMD on 256 CUs) and >= 4 tiles per
+// split; a grid of fewer than 1024 target workgroups (small or batched-small problems)
+// instead takes one target per lane, aims for 16384 wavefronts and lets a split hold a
+// single tile, which the autotuner measured faster on every such shape.
+static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
+  const int R = targets_per_lane(op, prec, D, DV);
+  const long blocks = (max_nx + 64L * R - 1) / (64L * R) * nseg;
+  if (blocks < 1024) return Choice{1, 16384, 1};
+  return Choice{R, 4096, 4};
+}
+
+// Candidates of the autotuner, the default first.  R never exceeds the op's default (the
+// register budget of the accumulators).
+static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg, long max_nx) {
+  const Choice d = default_choice(op, prec, D, DV, nseg, max_nx);
+  const int Rmax = targets_per_lane(op, prec, D, DV);
+  std::vector<int> Rs{Rmax};
+  if (Rmax >= 4) Rs.push_back(2);
+  if (Rmax >= 2) Rs.push_back(1);
+  std::vector<Choice> out{d};
+  for (int R : Rs)
+    for (int waves : {4096, 16384})
+      for (int mt : {4, 1}) {
+        Choice c{R, waves, mt};
+        if (!(c.R == d.R && c.waves == d.waves && c.min_tiles == d.min_tiles)) out.push_back(c);
+      }
+  return out;
+}
+
 // Work split shared by single calls and ragged batches: chunks of the largest segment's
 // targets, and source splits of the largest segment's sources chosen so that the grid
-// holds >= 4096 wavefronts (4 per SIMD on 256 CUs) with >= 4 source tiles per split.
+// holds about c.waves wavefronts with >= c.min_tiles source tiles per split.
 static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long max_nx,
-                       long max_ny) {
-  pl.R = targets_per_lane(op, prec, D, DV);
+                       long max_ny, const Choice &c) {
+  (void)op; (void)D; (void)DV;
+  pl.R = c.R;
   pl.nseg = nseg;
   pl.chunks = (max_nx + 64L * pl.R - 1) / (64L * pl.R);
   const int TS = tile_of(prec);
   const long tiles = (max_ny + TS - 1) / TS;
   const long blocks = pl.chunks * nseg;
-  long want = blocks > 0 ? (4096 + blocks - 1) / blocks : 1;
-  long maxsplit = std::max(1L, tiles / 4);
+  long want = blocks > 0 ? (c.waves + blocks - 1) / blocks : 1;
+  long maxsplit = std::max(1L, tiles / c.min_tiles);
   long ns = std::max(1L, std::min(want, maxsplit));
   ns = std::min(ns, 65535L);
   const long per = (tiles + ns - 1) / ns;
@@ -223,10 +263,10 @@ static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long
   pl.nsplit = (int)std::max(1L, (max_ny + pl.split_len - 1) / pl.split_len);
 }
 
-Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
-                     long max_ny) {
+static Plan batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
+                       long max_ny, const Choice &c) {
   Plan pl{};
-  split_plan(pl, op, prec, D, DV, nseg, max_nx, max_ny);
+  split_plan(pl, op, prec, D, DV, nseg, max_nx, max_ny, c);
   pl.sort = false;
   size_t off = 0;
   if (pl.nsplit > 1) {
@@ -237,9 +277,9 @@ Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, l
   return pl;
 }
 
-Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
+static Plan single_plan(int op, int prec, int D, int DV, long nx, long ny, const Choice &c) {
   Plan pl{};
-  split_plan(pl, op, prec, D, DV, 1, nx, ny);
+  split_plan(pl, op, prec, D, DV, 1, nx, ny, c);
   pl.sort = prec != DFM_FP64;
   size_t off = 0;
   if (pl.sort) {
@@ -267,6 +307,150 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
   }
   pl.work_bytes = off;
   return pl;
+}
+
+// Workspace bound over every candidate, so that a caller's buffer fits whatever plan the
+// autotuner picks.
+Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
+  Plan best = single_plan(op, prec, D, DV, nx, ny, default_choice(op, prec, D, DV, 1, nx));
+  for (const Choice &c : candidates(op, prec, D, DV, 1, nx)) {
+    Plan p = single_plan(op, prec, D, DV, nx, ny, c);
+    if (p.work_bytes > best.work_bytes) best = p;
+  }
+  return best;
+}
+
+Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
+                     long max_ny) {
+  Plan best = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny,
+                         default_choice(op, prec, D, DV, nseg, max_nx));
+  for (const Choice &c : candidates(op, prec, D, DV, nseg, max_nx)) {
+    Plan p = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny, c);
+    if (p.work_bytes > best.work_bytes) best = p;
+  }
+  return best;
+}
+
+// ------------------------------------------------------------------------------------
+// autotuner: per (op, precision, d, dv, batched, log2 of the largest target and source
+// counts), time every candidate twice on real calls (HIP events around the pair-sum
+// launch, read back at the next call of the same key) and keep the fastest.
+
+struct TuneKey {
+  int op, prec, D, DV, batch, lx, ly;
+  bool operator<(const TuneKey &o) const {
+    const int a[7] = {op, prec, D, DV, batch, lx, ly};
+    const int b[7] = {o.op, o.prec, o.D, o.DV, o.batch, o.lx, o.ly};
+    return std::lexicographical_compare(a, a + 7, b, b + 7);
+  }
+};
+
+struct TuneEntry {
+  std::vector<Choice> cand;
+  std::vector<float> best_ms;
+  int next = 0;
+  int pending = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool done = false;
+  int winner = 0;
+};
+
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneEntry> g_tune;
+static std::atomic<int> g_tune_mode{0};
+constexpr int kTuneReps = 2;
+
+static int ilog2(long v) {
+  int l = 0;
+  while (v > 1) {
+    v >>= 1;
+    l++;
+  }
+  return l;
+}
+
+void tune_set_mode(int mode) { g_tune_mode = mode; }
+
+void tune_reset() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto &kv : g_tune) {
+    if (kv.second.e0) (void)hipEventDestroy(kv.second.e0);
+    if (kv.second.e1) (void)hipEventDestroy(kv.second.e1);
+  }
+  g_tune.clear();
+}
+
+static void resolve_pending(TuneEntry &t) {
+  if (t.pending < 0) return;
+  if (!t.e1) {  // the timed launch recorded nothing (failed or still being issued)
+    t.pending = -1;
+    return;
+  }
+  float ms = 0.f;
+  if (hipEventSynchronize(t.e1) == hipSuccess && hipEventElapsedTime(&ms, t.e0, t.e1) == hipSuccess)
+    t.best_ms[t.pending] = std::min(t.best_ms[t.pending], ms);
+  (void)hipEventDestroy(t.e0);
+  (void)hipEventDestroy(t.e1);
+  t.e0 = t.e1 = nullptr;
+  t.pending = -1;
+  if (t.next >= (int)t.cand.size() * kTuneReps && !t.done) {
+    t.done = true;
+    t.winner = (int)(std::min_element(t.best_ms.begin(), t.best_ms.end()) - t.best_ms.begin());
+  }
+}
+
+// The plan choice for a call, and whether to time it (explore mode).
+static Choice tune_choose(const TuneKey &k, int op, int prec, int D, int DV, int nseg,
+                          long max_nx, TuneEntry **timed) {
+  *timed = nullptr;
+  const int mode = g_tune_mode;
+  const Choice d = default_choice(op, prec, D, DV, nseg, max_nx);
+  if (mode == 0) return d;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) {
+    if (mode == 2) return d;
+    TuneEntry t;
+    t.cand = candidates(op, prec, D, DV, nseg, max_nx);
+    t.best_ms.assign(t.cand.size(), 3.4e38f);
+    it = g_tune.emplace(k, std::move(t)).first;
+  }
+  TuneEntry &t = it->second;
+  if (mode == 1) resolve_pending(t);
+  if (t.done) return t.cand[t.winner];
+  if (mode == 2) return d;
+  const int c = t.next / kTuneReps;
+  t.pending = c;
+  t.next++;
+  *timed = &t;
+  return t.cand[c];
+}
+
+int tune_report(char *buf, size_t len) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  static const char *ops[] = {"conv", "grad1", "conv_grad1", "varlin", "graddim", "graddiff",
+                              "shoot_vjp"};
+  std::string out;
+  for (auto &kv : g_tune) {
+    resolve_pending(kv.second);
+    const TuneKey &k = kv.first;
+    const TuneEntry &t = kv.second;
+    char line[256];
+    const Choice &w = t.cand[t.done ? t.winner : 0];
+    snprintf(line, sizeof(line),
+             "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d -> R=%d waves=%d min_tiles=%d "
+             "%s %.4f ms (default %.4f ms)\n",
+             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, w.R, w.waves, w.min_tiles,
+             t.done ? "tuned" : "exploring", t.done ? t.best_ms[t.winner] : 0.0,
+             t.best_ms.empty() ? 0.0 : t.best_ms[0]);
+    out += line;
+  }
+  if (buf && len > 0) {
+    const size_t n = std::min(len - 1, out.size());
+    std::memcpy(buf, out.data(), n);
+    buf[n] = 0;
+  }
+  return (int)out.size();
 }
 
 // Sort targets along a Morton curve; writes perm into the workspace.
@@ -341,8 +525,16 @@ int seg_from_host(Seg &s, int n, const long *h, const long *d) {
   return DFM_OK;
 }
 
+static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
+                        const Plan &pl, char *work, TuneEntry *timed);
+
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
                    const Plan &pl, char *work) {
+  return launch_timed(st, op, prec, D, DV, p, pl, work, nullptr);
+}
+
+static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
+                        const Plan &pl, char *work, TuneEntry *timed) {
   const double sigma = p.sigma;
   // FP64: K = exp(-|x' - y'|^2) with x' = x / sigma; FP32: K = exp2(-|x' - y'|^2) with
   // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
@@ -362,7 +554,7 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
     p.perm = (const uint32_t *)(work + pl.off_perm);
   }
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (g_profile.on) {
+  if (g_profile.on || timed) {
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
     (void)hipEventRecord(ev0, st);
@@ -370,9 +562,16 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
   int rc = prec == DFM_FP64    ? launch_op_f64(st, op, D, DV, p, pl)
            : prec == DFM_FP32X ? launch_op_f32x(st, op, D, DV, p, pl)
                                : launch_op_f32c(st, op, D, DV, p, pl);
-  if (g_profile.on) {
+  if (ev0) {
     (void)hipEventRecord(ev1, st);
-    g_profile.events.push_back({ev0, ev1});
+    if (timed) {
+      // the tuner owns the events; profiling gets its own pair
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      timed->e0 = ev0;
+      timed->e1 = ev1;
+    } else {
+      g_profile.events.push_back({ev0, ev1});
+    }
   }
   if (rc != DFM_OK) return rc;
   hipError_t e = hipGetLastError();
@@ -409,12 +608,15 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
   }
-  Plan pl = make_batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max);
+  TuneEntry *timed = nullptr;
+  const Choice c = tune_choose(TuneKey{op, prec, D, DV, 1, ilog2(xs.max), ilog2(ys.max)}, op,
+                               prec, D, DV, xs.n, xs.max, &timed);
+  Plan pl = batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max, c);
   if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
                      work ? work_bytes : (size_t)0);
   p.sigma = clamp_width(sigma);
-  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed);
 }
 
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
@@ -426,12 +628,15 @@ int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KPara
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
   }
-  Plan pl = make_plan(op, prec, D, DV, p.nx, p.ny);
+  TuneEntry *timed = nullptr;
+  const Choice c = tune_choose(TuneKey{op, prec, D, DV, 0, ilog2(p.nx), ilog2(p.ny)}, op, prec,
+                               D, DV, 1, p.nx, &timed);
+  Plan pl = single_plan(op, prec, D, DV, p.nx, p.ny, c);
   if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
                      work ? work_bytes : (size_t)0);
   p.sigma = clamp_width(sigma);
-  return launch_pairsum(st, op, prec, D, DV, p, pl, (char *)work);
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed);
 }
 
 }  // namespace dfm

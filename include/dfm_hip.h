@@ -79,6 +79,24 @@ int dfm_stream_synchronize(void *stream);
 int dfm_profile_enable(int on);
 int dfm_profile_read(double *total_ms, long *launches);
 
+/* Autotuner of the pair-sum work split (no reference counterpart: the reference launches
+ * one thread per target in blocks of 192, GpuConv1D.cu:98-132).  Per (op, precision,
+ * dimpoint, dimvect, batched or not, log2 of the target and source counts) it chooses
+ * the targets per lane (1, 2 or 4), the wavefront count the source splits aim for (4096 or
+ * 16384) and the fewest source tiles per split (4 or 1).
+ *   mode 0  heuristics only (default; results deterministic across calls)
+ *   mode 1  explore: the first calls of each key run every candidate twice, timed with HIP
+ *           events on the call's stream (read back at the next call of that key), then the
+ *           fastest is kept.  Plans differ in summation order, so results differ across
+ *           candidates by rounding.  Not for use while a stream is being captured.
+ *   mode 2  use the tuned choices (default where a key was not tuned); capture-safe.
+ * Workspace queries always return the maximum over every candidate, so a buffer sized
+ * in any mode fits every plan.  dfm_autotune_report writes one line per key (truncated to
+ * len - 1 characters) and returns the full length. */
+int dfm_autotune(int mode);
+int dfm_autotune_reset(void);
+int dfm_autotune_report(char *buf, size_t len);
+
 /* ------------------------------------------------------------------------------------ */
 /* Host-pointer wrappers (synchronous; allocate, copy, launch, copy back, free).        */
 

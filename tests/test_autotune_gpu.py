@@ -1,0 +1,78 @@
+"""The plan autotuner: every candidate plan it explores gives correct results (each call in
+explore mode against the default plan's FP64 result), exploration ends in a tuned choice
+per shape, and the tuned mode is deterministic."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture
+def tuner():
+    from deformetrica_code_amd import _lib
+    _lib.load().dfm_autotune_reset()
+    try:
+        yield _lib
+    finally:
+        _lib.autotune(0)
+        _lib.load().dfm_autotune_reset()
+
+
+def _ops(n, m, seed):
+    from deformetrica_code_amd import device
+    from deformetrica_code_amd.diffeos import shoot_vjp
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(size=(n, 3))
+    Y = rng.uniform(size=(m, 3))
+    W = rng.normal(size=(m, 3))
+    A = rng.normal(size=(n, 3))
+    E = rng.normal(size=(n, 3))
+    D = device.DeviceArray.from_numpy
+    dX, dY, dW, dA, dE = D(X), D(Y), D(W), D(A), D(E)
+    dAn = D(rng.normal(size=(n, 3)))
+    s = 0.1
+    return {
+        "conv_grad1": lambda p: [a.numpy() for a in device.conv_grad1(dX, dA, dY, dW, s, p)],
+        "conv": lambda p: device.convolve(dX, dY, dW, s, p).numpy(),
+        "varlin": lambda p: device.varlin(dX, dY, dW, s, p).numpy(),
+        "graddim": lambda p: device.graddim(dX, dY, dW, s, 1, p).numpy(),
+        "graddiff": lambda p: device.graddiff(dX, dA, dE, s, p).numpy(),
+        "shoot_vjp": lambda p: list(shoot_vjp(X, A, E, dAn.numpy(), s, p)),
+        "batch": lambda p: [a.numpy() for a in device.batch_conv_grad1(
+            dX, dA, dY, dW, [0, n // 3, n], [0, m // 2, m], s, p)],
+    }
+
+
+@pytest.mark.parametrize("n,m", [(1000, 1000), (5000, 20000)])
+def test_every_explored_plan_is_correct(tuner, n, m):
+    ops = _ops(n, m, n)
+    refs = {k: f("fp64") for k, f in ops.items()}
+    tuner.autotune(1)
+    for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6)):
+        for k, f in ops.items():
+            for _ in range(28):           # > twice the largest candidate count
+                got = f(prec)
+                if isinstance(got, list):
+                    for u, v in zip(got, refs[k]):
+                        assert rel(u, v) < tol, (k, prec)
+                else:
+                    assert rel(got, refs[k]) < tol, (k, prec)
+    rep = tuner.autotune_report()
+    assert "exploring" not in rep and "tuned" in rep
+    assert "op=conv_grad1" in rep and "batch=1" in rep
+
+
+def test_tuned_mode_is_deterministic(tuner):
+    ops = _ops(3000, 3000, 3)
+    tuner.autotune(1)
+    for _ in range(28):
+        ops["conv_grad1"]("fp32c")
+    tuner.autotune(2)
+    a = ops["conv_grad1"]("fp32c")
+    b = ops["conv_grad1"]("fp32c")
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
