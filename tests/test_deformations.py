@@ -83,3 +83,42 @@ def test_reference_adjoint_matches_differences_to_first_order():
         rel.append(np.linalg.norm(ga - fd_a) / np.linalg.norm(fd_a))
     assert rel[1] < rel[0] / 2.5     # first-order agreement
     assert rel[1] < 0.05
+
+
+def test_flow_landmark_on_a_lone_control_point_follows_it_exactly():
+    """One control point p with momentum a moves at K(p, p) a = a with constant momentum;
+    a landmark placed on it sees velocity a at every stage of either scheme, so it stays on
+    the control point exactly."""
+    from oracle import deformations as dfo
+    p0 = np.array([[0.3, -0.2, 0.5]])
+    a0 = np.array([[0.4, 0.1, -0.7]])
+    for scheme in ("euler", "heun"):
+        P, A = dfo.shoot(p0, a0, 0.7, 9, scheme=scheme)
+        for improved in (True, False):
+            X = dfo.flow(P, A, p0, 0.7, 9, improved_euler=improved)
+            for t in range(9):
+                np.testing.assert_allclose(X[t], P[t], rtol=0, atol=1e-15)
+                np.testing.assert_allclose(P[t], p0 + a0 * t / 8.0, rtol=0, atol=1e-15)
+
+
+def test_flow_convergence_orders():
+    """Against a fine Heun/improved-Euler solution: the improved-Euler flow on a Heun shot
+    converges at second order in dt, the plain Euler flow at first order (a misplaced
+    stage of the improved scheme would drop it to first order)."""
+    from oracle import deformations as dfo
+    rng = np.random.default_rng(21)
+    p0 = rng.uniform(size=(12, 3))
+    a0 = rng.normal(scale=0.3, size=(12, 3))
+    x0 = rng.uniform(size=(20, 3))
+    w = 0.4
+
+    def final(T, improved):
+        P, A = dfo.shoot(p0, a0, w, T, scheme="heun")
+        return dfo.flow(P, A, x0, w, T, improved_euler=improved)[-1]
+
+    ref = final(641, True)
+    e = {(T, imp): np.abs(final(T, imp) - ref).max() for T in (11, 21, 41) for imp in (True, False)}
+    r_imp = e[(11, True)] / e[(21, True)], e[(21, True)] / e[(41, True)]
+    r_eul = e[(11, False)] / e[(21, False)], e[(21, False)] / e[(41, False)]
+    assert all(3.3 < r < 4.7 for r in r_imp), r_imp
+    assert all(1.6 < r < 2.4 for r in r_eul), r_eul
