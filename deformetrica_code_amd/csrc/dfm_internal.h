@@ -20,10 +20,13 @@ enum : int {
 
 constexpr int kBBoxParts = 256;
 
-// Source tile (LDS) per wavefront and, for FP32, sources summed in FP32 before each fold
-// into the FP64 accumulators.
+// Source tile (LDS) per wavefront (a whole number of 64-record stagings) and, for FP32,
+// sources summed in FP32 before each fold into the FP64 accumulators.  A small FP32 tile
+// leaves the wavefront count per SIMD to the registers: at N = M = 1M the fused FP32 sum
+// runs in 385 ms with 64-record tiles (3 KB) against 423 ms with 256 (12 KB, 13 wavefronts
+// per CU); FP64 runs 1087 ms with 128-record tiles and 1098 ms with 64.
 constexpr int kTileF64 = 128;
-constexpr int kTileF32 = 256;
+constexpr int kTileF32 = 64;
 constexpr int kFoldF32 = 32;
 
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
@@ -60,11 +63,13 @@ struct KParams {
   double ch1;  // GradDiff:  2 / sigma^2
   double ch2;  // GradDiff: -4 / (sigma^4 s^2)
   double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
+  int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores)
 };
 
 // Work decomposition and workspace layout of one call.
 struct Plan {
   int R;            // targets per lane
+  int mx;           // FP32C: accumulate on the matrix cores (k_pairsum_mx), R = 1
   int nseg;         // segments of a ragged batch (grid.z)
   bool sort;        // Morton-order the targets first (single-segment FP32 calls)
   long chunks;      // workgroups along the targets (of the largest segment)
@@ -142,6 +147,9 @@ int batch_adjoint_phase(hipStream_t st, int D, double sigma, int prec, int schem
                         size_t work_bytes, int phase, const double **x_final);
 size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *lm_off, int T,
                                int k, int prec);
+
+// the matrix-core form exists for this op and shape (pairsum_f32c.hip)
+bool mx_supported(int op, int D, int DV);
 
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)
 int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);

@@ -211,17 +211,22 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // aim for, and the fewest source tiles a split may hold.
 struct Choice {
   int R, waves, min_tiles;
+  int mx = 0;  // FP32C fused Convolve + Grad1 on the matrix cores (R = 1)
 };
 
-// Heuristic: the op's R, >= 4096 wavefronts (4 per SIMD on 256 CUs) and >= 4 tiles per
-// split; a grid of fewer than 1024 target workgroups (small or batched-small problems)
-// instead takes one target per lane, aims for 16384 wavefronts and lets a split hold a
-// single tile, which the autotuner measured faster on every such shape.
+// Heuristic, from the autotuner's measurements on the shapes of the workloads: one target
+// per lane (64-target wavefronts: the fewest registers, the most wavefronts per SIMD and
+// the tightest wavefront boxes); source splits aiming for 65536 wavefronts on grids of
+// >= 4096 target workgroups and 16384 below (the short tail of many small wavefronts
+// balances the CUs); splits of at least 256 FP32 sources (4 tiles), so that the split
+// reduction stays short on small grids, or 128 FP64 sources (one tile) on grids of fewer
+// than 1024 target workgroups.  At N = M = 1M the fused FP32C sum runs in 361-374 ms with
+// this plan against 483 ms with 4 targets per lane, 4096 wavefronts and 256-source tiles.
 static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
-  const int R = targets_per_lane(op, prec, D, DV);
-  const long blocks = (max_nx + 64L * R - 1) / (64L * R) * nseg;
-  if (blocks < 1024) return Choice{1, 16384, 1};
-  return Choice{R, 4096, 4};
+  (void)op; (void)D; (void)DV;
+  const long blocks = (max_nx + 63) / 64 * nseg;
+  const int min_tiles = (prec == DFM_FP64 && blocks < 1024) ? 1 : 4;
+  return Choice{1, blocks >= 4096 ? 65536 : 16384, min_tiles};
 }
 
 // Candidates of the autotuner, the default first.  R never exceeds the op's default (the
@@ -233,11 +238,20 @@ static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg,
   if (Rmax >= 4) Rs.push_back(2);
   if (Rmax >= 2) Rs.push_back(1);
   std::vector<Choice> out{d};
+  auto same = [](const Choice &a, const Choice &b) {
+    return a.R == b.R && a.waves == b.waves && a.min_tiles == b.min_tiles && a.mx == b.mx;
+  };
   for (int R : Rs)
+    for (int waves : {4096, 16384, 65536})
+      for (int mt : {4, 1}) {
+        Choice c{R, waves, mt, 0};
+        if (!same(c, d)) out.push_back(c);
+      }
+  if (prec == DFM_FP32C && mx_supported(op, D, DV))
     for (int waves : {4096, 16384})
       for (int mt : {4, 1}) {
-        Choice c{R, waves, mt};
-        if (!(c.R == d.R && c.waves == d.waves && c.min_tiles == d.min_tiles)) out.push_back(c);
+        Choice c{1, waves, mt, 1};
+        if (!same(c, d)) out.push_back(c);
       }
   return out;
 }
@@ -248,7 +262,8 @@ static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg,
 static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long max_nx,
                        long max_ny, const Choice &c) {
   (void)op; (void)D; (void)DV;
-  pl.R = c.R;
+  pl.R = c.mx ? 1 : c.R;
+  pl.mx = c.mx;
   pl.nseg = nseg;
   pl.chunks = (max_nx + 64L * pl.R - 1) / (64L * pl.R);
   const int TS = tile_of(prec);
@@ -439,11 +454,17 @@ int tune_report(char *buf, size_t len) {
     const Choice &w = t.cand[t.done ? t.winner : 0];
     snprintf(line, sizeof(line),
              "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d -> R=%d waves=%d min_tiles=%d "
-             "%s %.4f ms (default %.4f ms)\n",
-             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, w.R, w.waves, w.min_tiles,
+             "mx=%d %s %.4f ms (default %.4f ms) |",
+             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, w.R, w.waves, w.min_tiles, w.mx,
              t.done ? "tuned" : "exploring", t.done ? t.best_ms[t.winner] : 0.0,
              t.best_ms.empty() ? 0.0 : t.best_ms[0]);
     out += line;
+    for (size_t i = 0; i < t.cand.size(); i++) {
+      snprintf(line, sizeof(line), " %d/%d/%d/%d:%.3f", t.cand[i].R, t.cand[i].waves,
+               t.cand[i].min_tiles, t.cand[i].mx, t.best_ms[i] < 3e38f ? t.best_ms[i] : -1.0);
+      out += line;
+    }
+    out += "\n";
   }
   if (buf && len > 0) {
     const size_t n = std::min(len - 1, out.size());

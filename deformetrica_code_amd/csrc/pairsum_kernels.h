@@ -271,6 +271,7 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
   constexpr int TS = Prec<T>::kTile;
   constexpr int FOLD = Prec<T>::kFold;
   constexpr int REC_PER_LANE = TS / 64;
+  static_assert(TS >= 64 && TS % 64 == 0, "each lane stages whole records: tiles of 64k sources");
   // a padded source sits far outside the support of K: exp(-|x - far|^2) is exactly 0
   const T far = T(1.0e4);
 
@@ -407,6 +408,10 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   }
 
   const double s = p.s;
+  if constexpr (Prec<T>::kCenter) {
+    // the compact wavefronts of this call run on the matrix cores (k_pairsum_mx)
+    if (p.wave_filter == 1 && !(extent * s > p.exact_extent)) return;
+  }
   T xt[R][D];
   T xl[R][D];
   T te[R][NTE];

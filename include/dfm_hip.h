@@ -82,8 +82,9 @@ int dfm_profile_read(double *total_ms, long *launches);
 /* Autotuner of the pair-sum work split (no reference counterpart: the reference launches
  * one thread per target in blocks of 192, GpuConv1D.cu:98-132).  Per (op, precision,
  * dimpoint, dimvect, batched or not, log2 of the target and source counts) it chooses
- * the targets per lane (1, 2 or 4), the wavefront count the source splits aim for (4096 or
- * 16384) and the fewest source tiles per split (4 or 1).
+ * the targets per lane (1, 2 or 4), the wavefront count the source splits aim for (4096,
+ * 16384 or 65536), the fewest source tiles per split (4 or 1) and, for the FP32C fused
+ * Convolve + Grad1 in 3D, whether the per-pair accumulation runs on the matrix cores.
  *   mode 0  heuristics only (default; results deterministic across calls)
  *   mode 1  explore: the first calls of each key run every candidate twice, timed with HIP
  *           events on the call's stream (read back at the next call of that key), then the

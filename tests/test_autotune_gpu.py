@@ -34,9 +34,12 @@ def _ops(n, m, seed):
     D = device.DeviceArray.from_numpy
     dX, dY, dW, dA, dE = D(X), D(Y), D(W), D(A), D(E)
     dAn = D(rng.normal(size=(n, 3)))
+    dW6, dA6 = D(rng.normal(size=(m, 6))), D(rng.normal(size=(n, 6)))
     s = 0.1
     return {
         "conv_grad1": lambda p: [a.numpy() for a in device.conv_grad1(dX, dA, dY, dW, s, p)],
+        "conv_grad1_dv6": lambda p: [a.numpy() for a in device.conv_grad1(dX, dA6, dY, dW6, s, p)],
+        "grad1": lambda p: device.grad1(dX, dA, dY, dW, s, p).numpy(),
         "conv": lambda p: device.convolve(dX, dY, dW, s, p).numpy(),
         "varlin": lambda p: device.varlin(dX, dY, dW, s, p).numpy(),
         "graddim": lambda p: device.graddim(dX, dY, dW, s, 1, p).numpy(),
@@ -54,7 +57,7 @@ def test_every_explored_plan_is_correct(tuner, n, m):
     tuner.autotune(1)
     for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6)):
         for k, f in ops.items():
-            for _ in range(28):           # > twice the largest candidate count
+            for _ in range(50):           # > twice the largest candidate count
                 got = f(prec)
                 if isinstance(got, list):
                     for u, v in zip(got, refs[k]):
@@ -69,7 +72,7 @@ def test_every_explored_plan_is_correct(tuner, n, m):
 def test_tuned_mode_is_deterministic(tuner):
     ops = _ops(3000, 3000, 3)
     tuner.autotune(1)
-    for _ in range(28):
+    for _ in range(50):
         ops["conv_grad1"]("fp32c")
     tuner.autotune(2)
     a = ops["conv_grad1"]("fp32c")
