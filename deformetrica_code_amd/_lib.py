@@ -93,6 +93,17 @@ def load():
                  "dfm_batch_shoot_flow_adjoint", "dfm_graph_begin_capture",
                  "dfm_graph_end_capture", "dfm_graph_launch", "dfm_graph_destroy"):
         getattr(L, name).restype = i
+    # kernel matrix, ConvolveInverse, parallel transport
+    L.dfm_kernel_matrix.argtypes = [vp, i, d, vp, l, vp]
+    L.dfm_convolve_inverse_workspace.argtypes = [i, i, l]
+    L.dfm_convolve_inverse_workspace.restype = sz
+    L.dfm_convolve_inverse.argtypes = [vp, i, i, d, vp, l, vp, vp, vp, sz]
+    L.dfm_parallel_transport_workspace.argtypes = [i, l, l, i]
+    L.dfm_parallel_transport_workspace.restype = sz
+    L.dfm_parallel_transport.argtypes = [vp, i, d, i, vp, vp, l, i, d, d, vp, vp, l, d, vp, i, vp,
+                                         vp, sz]
+    for name in ("dfm_kernel_matrix", "dfm_convolve_inverse", "dfm_parallel_transport"):
+        getattr(L, name).restype = i
     # deterministic atlas
     L.dfm_atlas_create.argtypes = [vp, ctypes.POINTER(vp)]
     L.dfm_atlas_destroy.argtypes = [vp]

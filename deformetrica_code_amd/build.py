@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "libdfm_hip.so")
 
 SOURCES = [
     "pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip",
-    "integrators.hip", "surfaces.hip", "atlas.hip",
+    "integrators.hip", "surfaces.hip", "atlas.hip", "transport.hip",
 ]
 
 HIPCC_FLAGS = [
@@ -58,7 +58,7 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lrocsolver", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])

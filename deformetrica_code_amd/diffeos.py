@@ -106,6 +106,24 @@ class HIPDiffeos:
     def GetDeformedLandmarkPoints(self):
         return self._X.numpy(self._stream)[-1]
 
+    # --- parallel transport
+    def ParallelTransport(self, initial_momenta, initial_cp, initial_time, target_times):
+        """Diffeos::ParallelTransport (Diffeos.cxx:293-582) along the shot geodesic: the
+        transported momenta (list, one n x d array per target time)."""
+        L = _lib.load()
+        n, d, T = self.p0.shape[0], self.dim, self.T
+        st = self._stream
+        mom = DeviceArray.from_numpy(initial_momenta, st)
+        cp = DeviceArray.from_numpy(initial_cp, st)
+        tt = np.ascontiguousarray(target_times, dtype=np.float64)
+        out = DeviceArray((len(tt), n, d))
+        wp, wb = self._ws.get(L.dfm_parallel_transport_workspace(d, n, cp.shape[0], self.precision))
+        _lib.check(L.dfm_parallel_transport(
+            st.handle, d, self.kernel_width, self.precision, self._P.ptr, self._A.ptr, n, T,
+            self.t0, self.tN, mom.ptr, cp.ptr, cp.shape[0], float(initial_time),
+            tt.ctypes.data_as(ctypes.c_void_p), len(tt), out.ptr, wp, wb))
+        return list(out.numpy(st))
+
     # --- backward
     def GradientOfTerminalCost(self, theta_T, checkpoint_every: int = 5):
         """(dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_T."""
@@ -125,6 +143,22 @@ class HIPDiffeos:
                                             x0.ptr, m, T, self.t0, self.tN, int(checkpoint_every),
                                             th.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
         return gp.numpy(st), ga.numpy(st), gx.numpy(st)
+
+
+def convolve_inverse(y, x, sigma, stream=None):
+    """K(y, y)^-1 x in FP64 (ExactKernel::ConvolveInverse), on host arrays."""
+    L = _lib.load()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, d = y.shape
+    dv = x.shape[1]
+    st = stream or Stream()
+    dy, dx, dv_out = DeviceArray.from_numpy(y, st), DeviceArray.from_numpy(x, st), DeviceArray(x.shape)
+    ws = Workspace()
+    wp, wb = ws.get(L.dfm_convolve_inverse_workspace(d, dv, n))
+    _lib.check(L.dfm_convolve_inverse(st.handle, d, dv, float(sigma), dy.ptr, n, dx.ptr, dv_out.ptr,
+                                      wp, wb))
+    return dv_out.numpy(st)
 
 
 def shoot_vjp(p, a, mu_p, mu_a, sigma, precision="fp64"):

@@ -296,6 +296,36 @@ int dfm_batch_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int p
                                  const double *theta_T, double *grad_p0, double *grad_a0,
                                  double *grad_x0, void *work, size_t work_bytes);
 
+/* ---- kernel matrix, ConvolveInverse and parallel transport (device pointers) ----------- */
+
+/* K (n x n) = [exp(-|y_i - y_j|^2 / sigma^2)]; replaces ExactKernel::ComputeKernelMatrix
+ * (ExactKernel.cxx:131-143). */
+int dfm_kernel_matrix(void *stream, int dimpoint, double sigma, const double *y, long n,
+                      double *K);
+
+/* v (n x dv) = K(y, y)^-1 x, FP64: ExactKernel::ConvolveInverse (ExactKernel.cxx:154-164),
+ * whose Armadillo solve() is LAPACK dgesv; here the same LU with partial pivoting by
+ * rocSOLVER (dgetrf / dgetrs) on the stream.  n <= 46340.  DFM_ERR_ARG if K is singular
+ * (the call synchronises the stream to report it). */
+size_t dfm_convolve_inverse_workspace(int dimpoint, int dimvect, long n);
+int dfm_convolve_inverse(void *stream, int dimpoint, int dimvect, double sigma, const double *y,
+                         long n, const double *x, double *v, void *work, size_t work_bytes);
+
+/* Parallel transport of momenta mom0 (on the control points cp0, m == n) along the geodesic
+ * P, A (T x n x d, as written by dfm_shoot with the same sigma and time grid), reported at
+ * ntargets increasing target times into out (ntargets x n x d): Diffeos::ParallelTransport
+ * (Diffeos.cxx:293-582), with its early exits, its norm-conserving rescaling and its
+ * finite-difference Runge-Kutta 2 step; kernel sums at `precision`, the K^-1 solves in
+ * FP64.  The output of target time t is the transport of step t - (index of
+ * initial_time); the reference indexes its per-step buffer by t, which is the same when
+ * the initial time is t0. */
+size_t dfm_parallel_transport_workspace(int dimpoint, long n, long m, int precision);
+int dfm_parallel_transport(void *stream, int dimpoint, double sigma, int precision,
+                           const double *P, const double *A, long n, int T, double t0,
+                           double tN, const double *mom0, const double *cp0, long m,
+                           double initial_time, const double *target_times, int ntargets,
+                           double *out, void *work, size_t work_bytes);
+
 /* ---- surface data terms (device pointers) ----------------------------------------------
  * Meshes are vertices (nv x 3, FP64, row-major) and triangles (nf x 3, int32 indices).
  * DFM_CURRENT is OrientedSurfaceMesh (area-weighted normals as currents), DFM_VARIFOLD is

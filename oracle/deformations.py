@@ -162,11 +162,14 @@ def parallel_transport(P, A, width, T, initial_momenta, initial_cp, initial_time
     k.SetSources(P[final_index - 1])
     k.SetWeights(A[final_index - 1])
     transport[index] = k.ConvolveInverse(velocities[index])
+    # the reference indexes its per-step buffer by the absolute time index t
+    # (parallelTransport[t], Diffeos.cxx:571-579), which is the step t - initial_index
+    # when the initial time is t0; the step index is used here
     out = []
     oi = 0
     for t in range(initial_index, final_index + 1):
         if oi < nb and t == target_indices[oi]:
-            out.append(transport[t])
+            out.append(transport[t - initial_index])
             oi += 1
     return out
 

@@ -305,41 +305,53 @@ void orc_kernel_matrix(int d, double h, const double *Y, long n, double *out) {
     for (long j = 0; j < n; j++) out[i * n + j] = eval_kernel(Y + i * d, Y + j * d, d, s2);
 }
 
-/* ExactKernel::ConvolveInverse — ExactKernel.cxx:154-164: solve K V = X with K the
- * (symmetric positive definite) kernel matrix of the sources.  The reference calls
- * Armadillo solve(); this restatement uses a Cholesky factorisation, which returns the
- * same solution up to rounding for an SPD matrix.  X, V: (n x dv).  Returns -3 if K is
- * not numerically positive definite. */
+/* ExactKernel::ConvolveInverse — ExactKernel.cxx:154-164: solve K V = X with K the kernel
+ * matrix of the sources.  X, V: (n x dv).  Returns -3 if K is exactly singular. */
 int orc_convolve_inverse(int d, int dv, double h, const double *Y, long n, const double *X,
                          double *V) {
-  double *L = malloc(sizeof(double) * (size_t)n * n);
-  if (!L) return -2;
-  orc_kernel_matrix(d, h, Y, n, L);
-  for (long j = 0; j < n; j++) {
-    double s = L[j * n + j];
-    for (long k = 0; k < j; k++) s -= L[j * n + k] * L[j * n + k];
-    if (!(s > 0.0)) { free(L); return -3; }
-    double ljj = sqrt(s);
-    L[j * n + j] = ljj;
-    for (long i = j + 1; i < n; i++) {
-      double t = L[i * n + j];
-      for (long k = 0; k < j; k++) t -= L[i * n + k] * L[j * n + k];
-      L[i * n + j] = t / ljj;
+  /* ExactKernel::ConvolveInverse (ExactKernel.cxx:156-164): V = solve(K(Y, Y), X), which
+   * Armadillo hands to LAPACK's dgesv for a square system: LU factorisation with partial
+   * (row) pivoting, the pivot being the first entry of largest magnitude in the column,
+   * then forward and back substitution. */
+  double *A = malloc(sizeof(double) * (size_t)n * n);
+  long *piv = malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));
+  if (!A || !piv) { free(A); free(piv); return -2; }
+  orc_kernel_matrix(d, h, Y, n, A);
+  for (long k = 0; k < n; k++) {
+    long p = k;
+    double best = fabs(A[k * n + k]);
+    for (long i = k + 1; i < n; i++)
+      if (fabs(A[i * n + k]) > best) { best = fabs(A[i * n + k]); p = i; }
+    piv[k] = p;
+    if (best == 0.0) { free(A); free(piv); return -3; }
+    if (p != k)
+      for (long c = 0; c < n; c++) { double t = A[k * n + c]; A[k * n + c] = A[p * n + c]; A[p * n + c] = t; }
+    const double inv = 1.0 / A[k * n + k];
+    for (long i = k + 1; i < n; i++) {
+      const double l = A[i * n + k] * inv;
+      A[i * n + k] = l;
+      for (long c = k + 1; c < n; c++) A[i * n + c] -= l * A[k * n + c];
     }
   }
+  for (long i = 0; i < n; i++)
+    for (int c = 0; c < dv; c++) V[i * dv + c] = X[i * dv + c];
+  for (long k = 0; k < n; k++)          /* the row interchanges, in order */
+    if (piv[k] != k)
+      for (int c = 0; c < dv; c++) { double t = V[k * dv + c]; V[k * dv + c] = V[piv[k] * dv + c]; V[piv[k] * dv + c] = t; }
   for (int c = 0; c < dv; c++) {
-    for (long i = 0; i < n; i++) { /* forward: L z = x */
-      double t = X[i * dv + c];
-      for (long k = 0; k < i; k++) t -= L[i * n + k] * V[k * dv + c];
-      V[i * dv + c] = t / L[i * n + i];
-    }
-    for (long i = n - 1; i >= 0; i--) { /* backward: L^T v = z */
+    for (long i = 0; i < n; i++) {      /* L z = P x (unit lower) */
       double t = V[i * dv + c];
-      for (long k = i + 1; k < n; k++) t -= L[k * n + i] * V[k * dv + c];
-      V[i * dv + c] = t / L[i * n + i];
+      for (long k = 0; k < i; k++) t -= A[i * n + k] * V[k * dv + c];
+      V[i * dv + c] = t;
+    }
+    for (long i = n - 1; i >= 0; i--) { /* U v = z */
+      double t = V[i * dv + c];
+      for (long k = i + 1; k < n; k++) t -= A[i * n + k] * V[k * dv + c];
+      V[i * dv + c] = t / A[i * n + i];
     }
   }
-  free(L);
+  free(A);
+  free(piv);
   return 0;
 }
 

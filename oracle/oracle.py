@@ -181,7 +181,7 @@ class ExactKernel:
         V = np.empty((n, dv))
         rc = lib().orc_convolve_inverse(self.dim, dv, self.h, _p(self._Y), n, _p(X), _p(V))
         if rc != 0:
-            raise RuntimeError("kernel matrix is not positive definite (rc=%d)" % rc)
+            raise RuntimeError("kernel matrix is singular (rc=%d)" % rc)
         return V
 
 

@@ -178,6 +178,9 @@ def test_convolve_inverse_round_trip(oracle_mod):
     V = k.ConvolveInverse(X)
     k.SetWeights(V)
     np.testing.assert_allclose(k.Convolve(Y), X, rtol=1e-8, atol=1e-9)
+    # the reference's solve() is LAPACK gesv (LU, partial pivoting), as numpy.linalg.solve
+    K = k.ComputeKernelMatrix(Y)
+    np.testing.assert_allclose(V, np.linalg.solve(K, X), rtol=1e-12, atol=1e-12)
 
 
 def test_threaded_matches_serial(oracle_mod):
