@@ -16,6 +16,8 @@ FP64, FP32C, FP32X = 0, 1, 2
 OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP = range(7)
 EULER, HEUN = 0, 1
 CURRENT, VARIFOLD = 0, 1
+ADJOINT_DISCRETE, ADJOINT_CONTINUOUS = 0, 1
+ADJOINTS = {"discrete": ADJOINT_DISCRETE, "continuous": ADJOINT_CONTINUOUS}
 
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X}
 
@@ -69,7 +71,9 @@ def load():
     L.dfm_adjoint_workspace.restype = sz
     L.dfm_shoot_flow_adjoint.argtypes = [vp, i, d, i, i, i, vp, vp, l, vp, l, i, d, d, i, vp, vp,
                                          vp, vp, vp, sz]
-    for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint"):
+    L.dfm_shoot_flow_adjoint_equations.argtypes = L.dfm_shoot_flow_adjoint.argtypes
+    for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint",
+                 "dfm_shoot_flow_adjoint_equations"):
         getattr(L, name).restype = i
     # ragged batches and graphs
     L.dfm_batch_workspace.argtypes = [i, i, i, i, vp, vp, i]
@@ -161,6 +165,15 @@ def check(rc: int) -> None:
     if rc != DFM_OK:
         msg = load().dfm_last_error() or b""
         raise DfmError(rc, msg.decode(errors="replace"))
+
+
+def adjoint_code(adjoint) -> int:
+    if isinstance(adjoint, int):
+        return adjoint
+    try:
+        return ADJOINTS[adjoint]
+    except KeyError:
+        raise ValueError(f"unknown adjoint form {adjoint!r} (discrete or continuous)") from None
 
 
 def precision_code(precision) -> int:

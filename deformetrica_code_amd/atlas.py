@@ -29,7 +29,7 @@ class AtlasDesc(ctypes.Structure):
         ("n_template_faces", ctypes.c_long), ("template_faces", ctypes.c_void_p),
         ("nsubjects", ctypes.c_int), ("target_vertex_offsets", ctypes.c_void_p),
         ("target_face_offsets", ctypes.c_void_p), ("target_vertices", ctypes.c_void_p),
-        ("target_faces", ctypes.c_void_p),
+        ("target_faces", ctypes.c_void_p), ("adjoint", ctypes.c_int),
     ]
 
 
@@ -46,7 +46,7 @@ class HIPDeterministicAtlas:
                  kind="varifold", precision="fp64", deformation_width=1.0,
                  attachment_width=1.0, data_sigma_squared=1.0, smoothing_width=0.0,
                  number_of_time_points=11, t0=0.0, tN=1.0, scheme="euler", improved_euler=True,
-                 checkpoint_every=5):
+                 checkpoint_every=5, adjoint="discrete"):
         L = _lib.load()
         tf = np.ascontiguousarray(template_faces, dtype=np.int32)
         tv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1, 3)
@@ -76,6 +76,7 @@ class HIPDeterministicAtlas:
         d.nsubjects = len(targets)
         d.target_vertex_offsets, d.target_face_offsets = vp(vo), vp(fo)
         d.target_vertices, d.target_faces = vp(tv), vp(tfa)
+        d.adjoint = _lib.adjoint_code(adjoint)
         self.desc = d
         self.S, self.n, self.m = len(targets), int(n_control_points), int(n_template_vertices)
         self._h = ctypes.c_void_p()

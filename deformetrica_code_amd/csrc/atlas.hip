@@ -12,9 +12,10 @@
 //     d/d cp   = -sum_s (adjoint_p_s + Grad1(cp, a_s))
 //     d/d T0   = -sum_s adjoint_x_s, smoothed by Convolve at the smoothing width when
 //               that width is > 1e-20 (AbstractAtlas::ConvolveGradTemplate)
-//   where adjoint_* is the gradient of r_s / (2 sigma_data^2) through the shot and the flow
-//   (the exact discrete adjoint of integrators.hip; the reference integrates the continuous
-//   adjoint, AdjointEquationsIntegrator, equal to O(dt)).
+//   where adjoint_* is the gradient of r_s / (2 sigma_data^2) through the shot and the flow:
+//   the reference's continuous adjoint equations (AdjointEquationsIntegrator) with
+//   desc.adjoint = DFM_ADJOINT_CONTINUOUS, or the exact discrete adjoint of the forward
+//   scheme (equal to O(dt)) with DFM_ADJOINT_DISCRETE.
 //
 // One gradient call: batched shoot + flow forward with checkpoints; data-term geometry of
 // all deformed templates; two fused Convolve + ConvolveGradient pair sums over all subjects
@@ -193,6 +194,8 @@ int check_desc(const dfm_atlas_desc *d) {
     return set_error(DFM_ERR_ARG, "unknown scheme %d", d->scheme);
   if (d->number_of_time_points < 1 || d->checkpoint_every < 1)
     return set_error(DFM_ERR_ARG, "time points and checkpoint interval must be >= 1");
+  if (d->adjoint != DFM_ADJOINT_DISCRETE && d->adjoint != DFM_ADJOINT_CONTINUOUS)
+    return set_error(DFM_ERR_ARG, "unknown adjoint form %d", d->adjoint);
   if (!(d->deformation_width > 0) || !(d->attachment_width > 0) || !(d->data_sigma_squared > 0))
     return set_error(DFM_ERR_ARG, "kernel widths and the data variance must be positive");
   if (d->nsubjects < 1 || d->nsubjects > 65535)
@@ -351,7 +354,7 @@ int dfm_atlas_gradient(dfm_atlas *A, void *stream, const double *cp, const doubl
   if ((rc = batch_adjoint_phase(st, 3, c.deformation_width, prec, c.scheme, c.improved_euler, S,
                                 A->cp_off.data(), A->lm_off.data(), L.p0r, momenta, L.x0r,
                                 c.number_of_time_points, c.t0, c.tN, c.checkpoint_every, nullptr,
-                                nullptr, nullptr, nullptr, L.adj, L.adj_bytes, 1, &xT)))
+                                nullptr, nullptr, nullptr, L.adj, L.adj_bytes, 1, &xT, c.adjoint)))
     return rc;
   // data term of every deformed template against its target
   if ((rc = fill(st, L.bad, 1, 0.0))) return rc;
@@ -390,7 +393,7 @@ int dfm_atlas_gradient(dfm_atlas *A, void *stream, const double *cp, const doubl
   if ((rc = batch_adjoint_phase(st, 3, c.deformation_width, prec, c.scheme, c.improved_euler, S,
                                 A->cp_off.data(), A->lm_off.data(), L.p0r, momenta, L.x0r,
                                 c.number_of_time_points, c.t0, c.tN, c.checkpoint_every, L.theta,
-                                L.gp, L.ga, L.gx, L.adj, L.adj_bytes, 2, nullptr)))
+                                L.gp, L.ga, L.gx, L.adj, L.adj_bytes, 2, nullptr, c.adjoint)))
     return rc;
   if (grad_momenta)
     hipLaunchKernelGGL(k_neg_sum, dim3(blocks((long)S * n * 3)), dim3(256), 0, st, grad_momenta,

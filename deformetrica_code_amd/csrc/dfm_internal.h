@@ -139,12 +139,13 @@ int seg_dot(hipStream_t st, const double *X, const double *Y, int dv, int nsub, 
             const long *off, double *out);
 
 // batched shoot + flow adjoint in phases (integrators.hip): phase 1 forward (checkpoints
-// and *x_final in the workspace), phase 2 backward from them, phase 3 both
+// and *x_final in the workspace), phase 2 backward from them, phase 3 both; form
+// DFM_ADJOINT_DISCRETE or DFM_ADJOINT_CONTINUOUS
 int batch_adjoint_phase(hipStream_t st, int D, double sigma, int prec, int scheme, int improved,
                         int nseg, const long *cp_off, const long *lm_off, const double *p0,
                         const double *a0, const double *x0, int T, double t0, double tN, int k,
                         const double *theta_T, double *gp, double *ga, double *gx, void *work,
-                        size_t work_bytes, int phase, const double **x_final);
+                        size_t work_bytes, int phase, const double **x_final, int form);
 size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *lm_off, int T,
                                int k, int prec);
 

@@ -10,13 +10,19 @@
 //          targets = sources = p.
 //   flow   v(x; z) = sum_j K(x, p_j) a_j;  Euler or Heun (improved Euler, the default)
 //          (Diffeos::FlowLandmarkPointsTrajectory)
-// Backward: the exact transpose of those discrete steps (not the reference's continuous
-// adjoint, AdjointEquationsIntegrator, which agrees with it to O(dt)):
+// Backward, form DFM_ADJOINT_DISCRETE: the exact transpose of those discrete steps:
 //   J_F(z)^T (mu_p, mu_a) = (GradSym(mu_p) + SpecialHessian(mu_a), Convolve(mu_p) - Diff(mu_a))
 //     one fused OP_SHOOTVJP pair sum
 //   J_x v(x; z)^T th = Grad1(targets x, alpha th; sources p, weights a)
 //   J_z v(x; z)^T th = (Grad1, Convolve)(targets p, alpha a; sources x, weights th)
 //     one fused OP_CONVGRAD1 pair sum
+// Backward, form DFM_ADJOINT_CONTINUOUS: the reference's continuous adjoint equations
+// (AdjointEquationsIntegrator.cxx:59-136, 414-567), integrated backward along the forward
+// states with the flow's Euler / improved-Euler choice:
+//   d theta / ds = J_x v(x_s; z_s)^T theta                       (landmark cotangent)
+//   d xi / ds    = J_F(z_s)^T xi + J_z v(x_s; z_s)^T theta_s      (ComputeUpdateAt)
+// where ComputeUpdateAt's six terms are exactly J_F^T xi (dxi3 + dxi4, dxi5 + dxi6) and the
+// landmark coupling (dxi1, dxi2), so the same two fused pair sums serve both forms.
 // The forward stores (p, a, x) every k-th step; the backward recomputes each segment of
 // at most k steps from its checkpoint and walks it in reverse.
 //
@@ -454,6 +460,52 @@ int flow_impl(void *stream, int D, double sigma, int prec, int improved_euler, i
   return DFM_OK;
 }
 
+// d xi at state z = (p, a, x) with landmark cotangent th: J_F(z)^T (xp, xm) (one SHOOTVJP
+// pair sum) plus J_z v(x; z)^T th (one CONVGRAD1 pair sum: Grad1 onto p, Convolve onto a),
+// written to (op, oa); lpf / laf are scratch.
+int continuous_rate(const Ctx &cx, AdjointLayout &L, const Seg &cs, const Seg &ls,
+                    const double *p, const double *a, const double *x, const double *th,
+                    const double *xp, const double *xm, double *op, double *oa) {
+  const long nd = cs.total * cx.D;
+  int rc = shoot_vjp(cx, p, a, xp, xm, cs, op, oa);
+  if (rc || ls.total == 0) return rc;
+  if ((rc = conv_grad1(cx, p, a, cs, x, th, ls, L.laf, L.lpf))) return rc;
+  if ((rc = lin(cx, op, op, 1.0, L.lpf, 1.0, nullptr, 0.0, nd))) return rc;
+  return lin(cx, oa, oa, 1.0, L.laf, 1.0, nullptr, 0.0, nd);
+}
+
+// One backward step of the continuous adjoint equations from state t+1 (z1) to state t
+// (zt): theta in L.th and xi in (L.lp, L.la) move from t+1 to t (AEI.cxx:96-133, 438-462).
+int continuous_step(const Ctx &cx, AdjointLayout &L, const Seg &cs, const Seg &ls, int improved,
+                    double dt, const double *pt, const double *at, const double *xt,
+                    const double *p1, const double *a1, const double *x1) {
+  const long nd = cs.total * cx.D, md = ls.total * cx.D;
+  int rc;
+  if (md > 0) {
+    // theta_t = theta_{t+1} + dt Grad1(x_{t+1}, theta_{t+1}; z_{t+1})  [+ Heun corrector]
+    if ((rc = conv_grad1(cx, x1, L.th, ls, p1, a1, cs, nullptr, L.ths))) return rc;
+    if ((rc = lin(cx, L.thn, L.th, 1.0, L.ths, dt, nullptr, 0.0, md))) return rc;
+    if (improved) {
+      if ((rc = conv_grad1(cx, xt, L.thn, ls, pt, at, cs, nullptr, L.mu))) return rc;
+      if ((rc = lin(cx, L.thn, L.th, 1.0, L.ths, 0.5 * dt, L.mu, 0.5 * dt, md))) return rc;
+    }
+  }
+  // xi_t = xi_{t+1} + dt rate(z_{t+1}, theta_{t+1}, xi_{t+1})  [+ Heun corrector at z_t]
+  if ((rc = continuous_rate(cx, L, cs, ls, p1, a1, x1, L.th, L.lp, L.la, L.jp, L.ja))) return rc;
+  if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jp, dt, nullptr, 0.0, nd))) return rc;
+  if ((rc = lin(cx, L.lan, L.la, 1.0, L.ja, dt, nullptr, 0.0, nd))) return rc;
+  if (improved) {
+    if ((rc = continuous_rate(cx, L, cs, ls, pt, at, xt, L.thn, L.lpn, L.lan, L.jsp, L.jsa)))
+      return rc;
+    if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jp, 0.5 * dt, L.jsp, 0.5 * dt, nd))) return rc;
+    if ((rc = lin(cx, L.lan, L.la, 1.0, L.ja, 0.5 * dt, L.jsa, 0.5 * dt, nd))) return rc;
+  }
+  std::swap(L.lp, L.lpn);
+  std::swap(L.la, L.lan);
+  if (md > 0) std::swap(L.th, L.thn);
+  return DFM_OK;
+}
+
 // phase 1: forward pass only (checkpoints and the final landmarks, *x_final, stay in the
 // workspace); phase 2: backward pass from the checkpoints of an earlier phase 1 with the
 // same arguments; phase 3: both.
@@ -462,8 +514,10 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
                  const double *a0, long n_, const double *x0, long m_, int T, double t0,
                  double tN, int checkpoint_every, const double *theta_T, double *grad_p0,
                  double *grad_a0, double *grad_x0, void *work, size_t work_bytes, int phase = 3,
-                 const double **x_final = nullptr) {
+                 const double **x_final = nullptr, int form = DFM_ADJOINT_DISCRETE) {
   clear_error();
+  if (form != DFM_ADJOINT_DISCRETE && form != DFM_ADJOINT_CONTINUOUS)
+    return set_error(DFM_ERR_ARG, "unknown adjoint form %d", form);
   if (nseg > 0 && !lm_off) return set_error(DFM_ERR_ARG, "landmark offsets are NULL");
   Seg cs, ls;
   int rc = segs_of(nseg, cp_off, lm_off, n_, m_, cs, ls);
@@ -529,6 +583,12 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
     }
     for (int t = te - 1; t >= tb; t--) {
       double *zt = S + (size_t)(t - tb) * rec, *z1 = S + (size_t)(t - tb + 1) * rec;
+      if (form == DFM_ADJOINT_CONTINUOUS) {
+        if ((rc = continuous_step(cx, L, cs, ls, improved_euler, dt, P_(zt), A_(zt), X_(zt),
+                                  P_(z1), A_(z1), X_(z1))))
+          return rc;
+        continue;
+      }
       // flow step t: cotangent th of x_{t+1} -> thn of x_t, plus its pull onto z_t, z_{t+1}
       if (m > 0) {
         if ((rc = conv_grad1(cx, X_(zt), nullptr, ls, P_(zt), A_(zt), cs, L.ft.v1, nullptr))) return rc;
@@ -607,9 +667,10 @@ int batch_adjoint_phase(hipStream_t st, int D, double sigma, int prec, int schem
                         int nseg, const long *cp_off, const long *lm_off, const double *p0,
                         const double *a0, const double *x0, int T, double t0, double tN, int k,
                         const double *theta_T, double *gp, double *ga, double *gx, void *work,
-                        size_t work_bytes, int phase, const double **x_final) {
+                        size_t work_bytes, int phase, const double **x_final, int form) {
   return adjoint_impl(st, D, sigma, prec, scheme, improved, nseg, cp_off, lm_off, p0, a0, 0, x0,
-                      0, T, t0, tN, k, theta_T, gp, ga, gx, work, work_bytes, phase, x_final);
+                      0, T, t0, tN, k, theta_T, gp, ga, gx, work, work_bytes, phase, x_final,
+                      form);
 }
 
 size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *lm_off, int T,
@@ -670,6 +731,17 @@ int dfm_shoot_flow_adjoint(void *stream, int D, double sigma, int prec, int sche
   return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, 0, nullptr, nullptr, p0, a0,
                       n, x0, m, T, t0, tN, checkpoint_every, theta_T, grad_p0, grad_a0, grad_x0,
                       work, work_bytes);
+}
+
+int dfm_shoot_flow_adjoint_equations(void *stream, int D, double sigma, int prec, int scheme,
+                                     int improved_euler, const double *p0, const double *a0,
+                                     long n, const double *x0, long m, int T, double t0,
+                                     double tN, int checkpoint_every, const double *theta_T,
+                                     double *grad_p0, double *grad_a0, double *grad_x0,
+                                     void *work, size_t work_bytes) {
+  return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, 0, nullptr, nullptr, p0, a0,
+                      n, x0, m, T, t0, tN, checkpoint_every, theta_T, grad_p0, grad_a0, grad_x0,
+                      work, work_bytes, 3, nullptr, DFM_ADJOINT_CONTINUOUS);
 }
 
 // ---- batched subjects -----------------------------------------------------------------

@@ -5,9 +5,11 @@ SetNumberOfTimePoints, SetT0/SetTN, SetStartPositions/SetStartMomentas,
 SetLandmarkPoints, UseImprovedEuler, Update, GetTrajectoryPositions/Momentas,
 GetLandmarkPointsTrajectory; src/core/model_tools/deformations/Diffeos.h) with every
 kernel sum on the GPU and the trajectories resident in device memory between the
-shooting, the flow and the adjoint.  ``GradientOfTerminalCost`` replaces
-Diffeos::IntegrateAdjointEquations + AdjointEquationsIntegrator (Diffeos.cxx:1390-1443)
-with the exact discrete adjoint of the forward scheme, recomputed from checkpoints.
+shooting, the flow and the adjoint.  ``IntegrateAdjointEquations`` is the reference's
+Diffeos::IntegrateAdjointEquations + AdjointEquationsIntegrator (Diffeos.cxx:1390-1443):
+the continuous adjoint equations integrated backward; ``GradientOfTerminalCost`` is the
+exact discrete adjoint of the forward scheme (equal to O(dt)); both recompute the forward
+states from checkpoints.
 """
 from __future__ import annotations
 
@@ -126,7 +128,17 @@ class HIPDiffeos:
 
     # --- backward
     def GradientOfTerminalCost(self, theta_T, checkpoint_every: int = 5):
-        """(dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_T."""
+        """(dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_T: the exact discrete adjoint."""
+        return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint")
+
+    def IntegrateAdjointEquations(self, theta_T, checkpoint_every: int = 5):
+        """Diffeos::IntegrateAdjointEquations (Diffeos.cxx:1390-1443) for the landmark
+        points: (xi_pos(0), xi_mom(0), theta(0)) of the reference's continuous adjoint
+        equations for the terminal landmark cotangent theta_T; the reference's
+        GetAdjointPosAt0 / GetAdjointMomAt0 are the first two."""
+        return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint_equations")
+
+    def _adjoint(self, theta_T, checkpoint_every, entry):
         L = _lib.load()
         n, d, T = self.p0.shape[0], self.dim, self.T
         m = self.x0.shape[0]
@@ -138,10 +150,10 @@ class HIPDiffeos:
         gp, ga, gx = DeviceArray((n, d)), DeviceArray((n, d)), DeviceArray((m, d))
         wp, wb = self._ws.get(L.dfm_adjoint_workspace(d, n, m, T, int(checkpoint_every),
                                                       self.precision))
-        _lib.check(L.dfm_shoot_flow_adjoint(st.handle, d, self.kernel_width, self.precision,
-                                            self.scheme, int(self.improved_euler), p0.ptr, a0.ptr, n,
-                                            x0.ptr, m, T, self.t0, self.tN, int(checkpoint_every),
-                                            th.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
+        _lib.check(getattr(L, entry)(st.handle, d, self.kernel_width, self.precision,
+                                     self.scheme, int(self.improved_euler), p0.ptr, a0.ptr, n,
+                                     x0.ptr, m, T, self.t0, self.tN, int(checkpoint_every),
+                                     th.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
         return gp.numpy(st), ga.numpy(st), gx.numpy(st)
 
 

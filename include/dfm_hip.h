@@ -231,10 +231,11 @@ int dfm_shoot_vjp(void *stream, int dimpoint, double sigma, int precision, const
  * momenta a0 and landmarks x0, given theta_T = dE/dx_{T-1} (m x d): the exact discrete
  * adjoint (backward pass) of dfm_shoot + dfm_flow with the same scheme, precision and
  * time grid.  The forward pass is run once, storing (p, a, x) every checkpoint_every
- * steps; the backward pass recomputes each segment from its checkpoint.  Replaces
+ * steps; the backward pass recomputes each segment from its checkpoint.  An alternative to
  * Diffeos::IntegrateAdjointEquations / AdjointEquationsIntegrator::Update
  * (Diffeos.cxx:1390-1443, AdjointEquationsIntegrator.cxx:33-136, 414-567), whose continuous
  * adjoint agrees with this gradient to O(dt).  grad_x0 may be NULL. */
+enum dfm_adjoint_form { DFM_ADJOINT_DISCRETE = 0, DFM_ADJOINT_CONTINUOUS = 1 };
 size_t dfm_adjoint_workspace(int dimpoint, long n, long m, int T, int checkpoint_every,
                              int precision);
 int dfm_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precision, int scheme,
@@ -242,6 +243,20 @@ int dfm_shoot_flow_adjoint(void *stream, int dimpoint, double sigma, int precisi
                            const double *x0, long m, int T, double t0, double tN,
                            int checkpoint_every, const double *theta_T, double *grad_p0,
                            double *grad_a0, double *grad_x0, void *work, size_t work_bytes);
+/* The same gradient as the reference computes it: Diffeos::IntegrateAdjointEquations +
+ * AdjointEquationsIntegrator::Update (Diffeos.cxx:1390-1443, AdjointEquationsIntegrator.cxx:
+ * 59-136, 414-567), the continuous adjoint equations of the shoot + flow integrated
+ * backward along the forward states with the flow's Euler / improved-Euler choice:
+ * theta' = J_x v^T theta, then xi' = J_F^T xi + J_z v^T theta (ComputeUpdateAt).  Returns
+ * (xi_pos(0), xi_mom(0), theta(0)), the reference's GetAdjointPosAt0 / GetAdjointMomAt0 /
+ * landmark adjoint at 0.  Same arguments and workspace (dfm_adjoint_workspace) as
+ * dfm_shoot_flow_adjoint. */
+int dfm_shoot_flow_adjoint_equations(void *stream, int dimpoint, double sigma, int precision,
+                                     int scheme, int improved_euler, const double *p0,
+                                     const double *a0, long n, const double *x0, long m, int T,
+                                     double t0, double tN, int checkpoint_every,
+                                     const double *theta_T, double *grad_p0, double *grad_a0,
+                                     double *grad_x0, void *work, size_t work_bytes);
 
 /* ---- ragged batches of subjects (device pointers, host offset arrays) -------------------
  * A batch of nseg subjects (1 <= nseg <= 65535) stores each per-point array concatenated
@@ -413,6 +428,7 @@ typedef struct dfm_atlas_desc {
   const long *target_face_offsets;    /* host, nsubjects + 1 */
   const double *target_vertices;      /* host, concatenated, x 3 */
   const int *target_faces;            /* host, concatenated, indices local to each target */
+  int adjoint;                  /* DFM_ADJOINT_DISCRETE or DFM_ADJOINT_CONTINUOUS */
 } dfm_atlas_desc;
 
 int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **atlas);

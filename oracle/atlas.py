@@ -5,8 +5,10 @@ FP64 CPU restatement of the deterministic atlas log-likelihood for surface subje
 DeterministicAtlas.cxx:100-151) and of the template-gradient smoothing
 (AbstractAtlas::ConvolveGradTemplate, AbstractAtlas.cxx:727-751), composed from the
 oracle's Diffeos (deformations.py: Euler shooting, Heun flow) and surface data terms
-(surfaces.py), each pinned on its own.  The composition itself is a sum of those pinned
-pieces; its gradient is checked against central differences in the GPU tests.
+(surfaces.py), each pinned on its own, and of the gradient as the reference assembles it
+with its continuous adjoint (``gradient_reference``).  The compositions are sums of those
+pinned pieces; the GPU tests check the log-likelihood's gradient against central
+differences as well.
 """
 from __future__ import annotations
 
@@ -34,6 +36,38 @@ def loglik(cp, momenta, template_V, template_F, targets, *, kind, deformation_wi
         k.SetWeights(a)
         reg -= float(np.sum(k.Convolve(cp) * a))
     return 0.5 * data, 0.5 * reg, np.array(res)
+
+
+def gradient_reference(cp, momenta, template_V, template_F, targets, *, kind, deformation_width,
+                       attachment_width, data_sigma_squared, T, smoothing_width=0.0,
+                       scheme="euler", improved_euler=True):
+    """DeterministicAtlas::ComputeCompleteLogLikelihoodGradient (DeterministicAtlas.cxx:
+    159-201, 236-281) with the data term through AbstractAtlas::ComputeDataTermGradientSubject
+    (AbstractAtlas.cxx:562-616): the reference's continuous adjoint per subject, data
+    gradients summed over subjects, then the RKHS regularity.  Returns (d/dcp, d/da_s
+    stacked, d/dT0 smoothed)."""
+    grad = getattr(so, kind + "_match_gradient")
+    d = cp.shape[1]
+    g_cp = np.zeros_like(cp)
+    g_mom = np.zeros_like(momenta)
+    g_tpl = np.zeros_like(template_V)
+    for s, (a, (tv, tf)) in enumerate(zip(momenta, targets)):
+        P, A = dfo.shoot(cp, a, deformation_width, T, scheme=scheme)
+        X = dfo.flow(P, A, template_V, deformation_width, T, improved_euler=improved_euler)
+        theta = grad(X[-1], template_F, tv, tf, attachment_width) / (2.0 * data_sigma_squared)
+        xp, xm, th0 = dfo.adjoint_reference(P, A, X, theta, deformation_width, T,
+                                            improved_euler=improved_euler)
+        g_cp -= xp
+        g_mom[s] = -xm
+        g_tpl -= th0
+    g_tpl = smooth_template_gradient(template_V, g_tpl, smoothing_width)
+    k = ExactKernel(d, deformation_width)
+    k.SetSources(cp)
+    for s, a in enumerate(momenta):  # AddGradientRegularityTerm, DeterministicAtlas.cxx:236-262
+        k.SetWeights(a)
+        g_mom[s] -= k.Convolve(cp)
+        g_cp -= k.ConvolveGradient(cp, a)
+    return g_cp, g_mom, g_tpl
 
 
 def smooth_template_gradient(template_V, grad_l2, width):

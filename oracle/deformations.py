@@ -12,9 +12,11 @@ engine, on top of the oracle ExactKernel (oracle/exact_kernel.c):
 
 Pinned by tests/test_deformations.py: the reference's own parallel-transport golden
 vectors (tests/unit_tests/parallel-transport/data, GroundTruthMom.txt at 1e-4), the
-Hamiltonian's conservation to O(dt), and central differences of the shooting objective
-against the reference adjoint (to O(dt), the order at which a continuous adjoint
-discretised independently of the forward scheme agrees with the discrete gradient).
+Hamiltonian's conservation to O(dt), the adjoint's rates (reference_update,
+reference_theta_rate) term group by term group against central differences of the forward
+fields they transpose, and central differences of the shooting objective against the
+reference adjoint (to O(dt), the order at which a continuous adjoint discretised
+independently of the forward scheme agrees with the discrete gradient).
 Matrices are numpy float64 (rows x cols).
 """
 from __future__ import annotations
@@ -179,41 +181,55 @@ def hamiltonian(p, a, width):
     return 0.5 * float(np.sum(a * _kernel(p.shape[1], width, p, a).Convolve(p)))
 
 
+def reference_theta_rate(P_s, A_s, X_s, theta_s, width):
+    """Landmark-cotangent rate of AdjointEquationsIntegrator (AEI.cxx:96-133): the kernel of
+    (P_s, A_s) differentiated at the landmarks, contracted with theta."""
+    d = P_s.shape[1]
+    return _kernel(d, width, P_s, A_s).ConvolveGradient(X_s, theta_s)
+
+
+def reference_update(P_s, A_s, X_s, theta_s, xp_s, xm_s, width):
+    """ComputeUpdateAt (AEI.cxx:476-567), landmark branch: the rates (d xi_pos, d xi_mom)
+    at state s, the sum of its six kernel terms."""
+    n, d = P_s.shape
+    eta = _kernel(d, width, X_s, theta_s)
+    dxi1 = eta.ConvolveGradient(P_s, A_s)
+    dxi2 = eta.Convolve(P_s)
+    axp = np.concatenate([A_s, xp_s], axis=1)
+    k = _kernel(d, width, P_s, axp)
+    kaxp = k.Convolve(P_s)
+    g = k.ConvolveGradient(P_s)  # (n, 2d, d)
+    dxi3 = np.einsum("ikl,ik->il", g[:, :d, :], xp_s) + np.einsum("ikl,ik->il", g[:, d:, :], A_s)
+    dxi5 = kaxp[:, d:]
+    dxi6 = np.zeros((n, d))
+    for dim in range(d):
+        Wd = xm_s[:, dim:dim + 1] * A_s
+        dxi6 += _kernel(d, width, P_s, Wd).ConvolveGradient(P_s, dim)
+    dxi6 -= np.einsum("ikl,il->ik", g[:, :d, :], xm_s)
+    dxi4 = _kernel(d, width, P_s, A_s).ConvolveSpecialHessian(xm_s)
+    return dxi1 + dxi3 + dxi4, dxi2 + dxi5 + dxi6
+
+
 def adjoint_reference(P, A, X, theta_T, width, T, t0=0.0, tN=1.0, improved_euler=True):
     """AdjointEquationsIntegrator with landmark points only (AEI.cxx:59-136, 414-567):
-    returns (xi_pos_0, xi_mom_0), the reference's gradient of E(x_T) w.r.t. the initial
-    control points and momenta for dE/dx_T = theta_T."""
+    returns (xi_pos_0, xi_mom_0, theta_0), the reference's gradient of E(x_T) w.r.t. the
+    initial control points, momenta and landmarks for dE/dx_T = theta_T."""
     d = P[0].shape[1]
     n = P[0].shape[0]
     dt = time_step(T, t0, tN)
     theta = [None] * T
     theta[T - 1] = np.asarray(theta_T, dtype=np.float64).copy()
     for t in range(T - 1, 0, -1):  # AEI.cxx:96-133
-        dth = _kernel(d, width, P[t], A[t]).ConvolveGradient(X[t], theta[t])
+        dth = reference_theta_rate(P[t], A[t], X[t], theta[t], width)
         theta[t - 1] = theta[t] + dth * dt
         if improved_euler:
-            dth2 = _kernel(d, width, P[t - 1], A[t - 1]).ConvolveGradient(X[t - 1], theta[t - 1])
+            dth2 = reference_theta_rate(P[t - 1], A[t - 1], X[t - 1], theta[t - 1], width)
             theta[t - 1] = theta[t] + (dth + dth2) * (dt * 0.5)
     xp = [np.zeros((n, d)) for _ in range(T)]
     xm = [np.zeros((n, d)) for _ in range(T)]
 
-    def update(s):  # ComputeUpdateAt, AEI.cxx:476-567
-        eta = _kernel(d, width, X[s], theta[s])
-        dxi1 = eta.ConvolveGradient(P[s], A[s])
-        dxi2 = eta.Convolve(P[s])
-        axp = np.concatenate([A[s], xp[s]], axis=1)
-        k = _kernel(d, width, P[s], axp)
-        kaxp = k.Convolve(P[s])
-        g = k.ConvolveGradient(P[s])  # (n, 2d, d)
-        dxi3 = np.einsum("ikl,ik->il", g[:, :d, :], xp[s]) + np.einsum("ikl,ik->il", g[:, d:, :], A[s])
-        dxi5 = kaxp[:, d:]
-        dxi6 = np.zeros((n, d))
-        for dim in range(d):
-            Wd = xm[s][:, dim:dim + 1] * A[s]
-            dxi6 += _kernel(d, width, P[s], Wd).ConvolveGradient(P[s], dim)
-        dxi6 -= np.einsum("ikl,il->ik", g[:, :d, :], xm[s])
-        dxi4 = _kernel(d, width, P[s], A[s]).ConvolveSpecialHessian(xm[s])
-        return dxi1 + dxi3 + dxi4, dxi2 + dxi5 + dxi6
+    def update(s):
+        return reference_update(P[s], A[s], X[s], theta[s], xp[s], xm[s], width)
 
     for t in range(T - 1, 0, -1):  # AEI.cxx:438-462
         dpos, dmom = update(t)

@@ -1,0 +1,120 @@
+"""GPU continuous adjoint (dfm_shoot_flow_adjoint_equations and the atlas with
+adjoint="continuous") against the oracle's restatement of the reference's
+AdjointEquationsIntegrator, FP64 at 1e-10 relative."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _case(seed, n=9, m=23, d=3):
+    rng = np.random.default_rng(seed)
+    p0 = rng.uniform(-1, 1, size=(n, d))
+    a0 = rng.normal(scale=0.4, size=(n, d))
+    x0 = rng.uniform(-1.2, 1.2, size=(m, d))
+    theta = rng.normal(size=(m, d))
+    return p0, a0, x0, theta
+
+
+@pytest.mark.parametrize("scheme,improved,T,k", [("euler", True, 10, 1), ("euler", True, 10, 4),
+                                                  ("euler", False, 7, 3), ("heun", True, 6, 5),
+                                                  ("euler", True, 1, 1)])
+def test_adjoint_equations_match_reference_restatement(scheme, improved, T, k):
+    from oracle import deformations as dfo
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    width = 0.7
+    p0, a0, x0, theta = _case(T * 10 + k)
+    print(f"adjoint equations scheme={scheme} improved={improved} T={T} k={k}", flush=True)
+    D = HIPDiffeos(3)
+    D.SetKernelWidth(width)
+    D.SetNumberOfTimePoints(T)
+    D.SetShootingScheme(scheme)
+    (D.UseImprovedEuler if improved else D.UseStandardEuler)()
+    D.SetStartPositions(p0)
+    D.SetStartMomentas(a0)
+    D.SetLandmarkPoints(x0)
+    gp, ga, gx = D.IntegrateAdjointEquations(theta, checkpoint_every=k)
+    if T == 1:  # no step: the adjoint stays at its terminal value
+        assert np.all(gp == 0) and np.all(ga == 0)
+        np.testing.assert_array_equal(gx, theta)
+        return
+    P, A = dfo.shoot(p0, a0, width, T, scheme=scheme)
+    X = dfo.flow(P, A, x0, width, T, improved_euler=improved)
+    rp, ra, rx = dfo.adjoint_reference(P, A, X, theta, width, T, improved_euler=improved)
+    assert rel(gp, rp) < 1e-10
+    assert rel(ga, ra) < 1e-10
+    assert rel(gx, rx) < 1e-10
+
+
+def test_adjoint_equations_at_zero_momenta():
+    """Zero initial momenta: the geodesic stays put, but the momenta gradient is the landmark
+    coupling K(p, x) theta integrated over the time interval (the optimizer's first step)."""
+    from oracle import deformations as dfo
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    p0, _, x0, theta = _case(5)
+    a0 = np.zeros_like(p0)
+    D = HIPDiffeos(3)
+    D.SetKernelWidth(0.9)
+    D.SetStartPositions(p0)
+    D.SetStartMomentas(a0)
+    D.SetLandmarkPoints(x0)
+    gp, ga, gx = D.IntegrateAdjointEquations(theta, checkpoint_every=3)
+    P, A = dfo.shoot(p0, a0, 0.9, 10)
+    X = dfo.flow(P, A, x0, 0.9, 10)
+    rp, ra, rx = dfo.adjoint_reference(P, A, X, theta, 0.9, 10)
+    assert np.linalg.norm(ra) > 0
+    assert rel(ga, ra) < 1e-10 and rel(gx, rx) < 1e-10
+    assert np.linalg.norm(gp - rp) <= 1e-10 * np.linalg.norm(ra)
+
+
+def test_adjoint_equations_differ_from_discrete_adjoint_at_order_dt():
+    """The continuous and the discrete adjoints are different gradients of the same
+    objective: apart by O(dt), so closer at T = 41 than at T = 11."""
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    p0, a0, x0, theta = _case(9)
+    gaps = []
+    for T in (11, 41):
+        D = HIPDiffeos(3)
+        D.SetKernelWidth(0.7)
+        D.SetNumberOfTimePoints(T)
+        D.SetStartPositions(p0)
+        D.SetStartMomentas(a0)
+        D.SetLandmarkPoints(x0)
+        c = D.IntegrateAdjointEquations(theta)
+        g = D.GradientOfTerminalCost(theta)
+        gaps.append(rel(c[1], g[1]))
+    assert 0 < gaps[1] < gaps[0] / 2.5
+
+
+@pytest.mark.parametrize("kind", ["current", "varifold"])
+def test_atlas_continuous_adjoint_matches_reference_gradient(kind):
+    """The batched atlas gradient with adjoint="continuous" equals the reference's
+    DeterministicAtlas gradient restated with the continuous adjoint, every subject in one
+    launch, including the smoothed template gradient."""
+    import os
+    from oracle import atlas as oa
+    from oracle import surfaces as so
+    from deformetrica_code_amd.atlas import HIPDeterministicAtlas
+    mesh = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes")
+    V, F = so.read_vtk_polydata(os.path.join(mesh, "sphere80.vtk"))
+    rng = np.random.default_rng(11)
+    targets = [(V * 1.08 + rng.normal(scale=0.01, size=V.shape), F),
+               (V + np.array([0.04, -0.02, 0.03]), F)]
+    g = np.linspace(-0.6, 0.6, 3)
+    cp = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    mom = rng.normal(scale=0.03, size=(2, cp.shape[0], 3))
+    par = dict(deformation_width=0.4, attachment_width=0.2, data_sigma_squared=0.01)
+    print(f"atlas continuous adjoint kind={kind}", flush=True)
+    A = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], targets, kind=kind, precision="fp64",
+                              number_of_time_points=6, checkpoint_every=2, smoothing_width=0.3,
+                              adjoint="continuous", **par)
+    _, gcp, gmom, gtpl = A.ComputeCompleteLogLikelihoodGradient(cp, mom, V)
+    rcp, rmom, rtpl = oa.gradient_reference(cp, mom, V, F, targets, kind=kind, T=6,
+                                            smoothing_width=0.3, **par)
+    assert rel(gcp, rcp) < 1e-10
+    assert rel(gmom, rmom) < 1e-10
+    assert rel(gtpl, rtpl) < 1e-10

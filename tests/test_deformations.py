@@ -122,3 +122,61 @@ def test_flow_convergence_orders():
     r_eul = e[(11, False)] / e[(21, False)], e[(21, False)] / e[(41, False)]
     assert all(3.3 < r < 4.7 for r in r_imp), r_imp
     assert all(1.6 < r < 2.4 for r in r_eul), r_eul
+
+
+def test_reference_adjoint_rates_are_jacobian_transposes():
+    """The six terms of the reference's ComputeUpdateAt (AdjointEquationsIntegrator.cxx:
+    476-567) and its landmark-cotangent rate, as restated in the oracle, are the transposed
+    Jacobians of the forward fields, checked group by group against central differences:
+    xi' = J_F(z)^T xi + J_z v(x; z)^T theta with F = (K a, -G^T a), v(x; z) = K(x, p) a, and
+    theta' = J_x v(x; z)^T theta.  A dropped term, a flipped sign or a transposed operand
+    in any group fails one of the comparisons."""
+    from oracle import deformations as dfo
+    from oracle.oracle import ExactKernel
+    rng = np.random.default_rng(7)
+    n, m, d, w = 5, 7, 3, 0.8
+    P = rng.normal(size=(n, d)) * 0.5
+    A = rng.normal(size=(n, d))
+    X = rng.normal(size=(m, d)) * 0.6
+    th = rng.normal(size=(m, d))
+    xp = rng.normal(size=(n, d))
+    xm = rng.normal(size=(n, d))
+    zero_n, zero_m = np.zeros((n, d)), np.zeros((m, d))
+
+    def K(Y, W):
+        k = ExactKernel(d, w)
+        k.SetSources(Y)
+        k.SetWeights(W)
+        return k
+
+    def hamiltonian_pairing(p, a):  # xi . F(p, a)
+        k = K(p, a)
+        fa = -np.einsum("ikl,ik->il", k.ConvolveGradient(p), a)
+        return np.sum(xp * k.Convolve(p)) + np.sum(xm * fa)
+
+    def coupling(p, a, x):  # theta . v(x; p, a)
+        return np.sum(th * K(p, a).Convolve(x))
+
+    def fd(fun, Z, h=1e-6):
+        g = np.zeros_like(Z)
+        for idx in np.ndindex(Z.shape):
+            E = np.zeros_like(Z)
+            E[idx] = h
+            g[idx] = (fun(Z + E) - fun(Z - E)) / (2 * h)
+        return g
+
+    # xi part alone (theta = 0): J_F^T xi
+    up, um = dfo.reference_update(P, A, X, zero_m, xp, xm, w)
+    np.testing.assert_allclose(up, fd(lambda p: hamiltonian_pairing(p, A), P), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(um, fd(lambda a: hamiltonian_pairing(P, a), A), rtol=1e-6, atol=1e-7)
+    # landmark coupling alone (xi = 0): J_z v^T theta
+    cp_, cm_ = dfo.reference_update(P, A, X, th, zero_n, zero_n, w)
+    np.testing.assert_allclose(cp_, fd(lambda p: coupling(p, A, X), P), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(cm_, fd(lambda a: coupling(P, a, X), A), rtol=1e-6, atol=1e-7)
+    # both together are the sum (the rate is linear in (xi, theta))
+    bp, bm = dfo.reference_update(P, A, X, th, xp, xm, w)
+    np.testing.assert_allclose(bp, up + cp_, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(bm, um + cm_, rtol=1e-12, atol=1e-12)
+    # theta rate: J_x v^T theta
+    np.testing.assert_allclose(dfo.reference_theta_rate(P, A, X, th, w),
+                               fd(lambda x: coupling(P, A, x), X), rtol=1e-6, atol=1e-7)
