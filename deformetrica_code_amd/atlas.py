@@ -154,3 +154,77 @@ class Graph:
                 _lib.load().dfm_graph_destroy(self._g)
         except Exception:
             pass
+
+
+def initialize_control_points(bounding_box, spacing):
+    """AbstractAtlas::InitializeControlPoints without the image criterion
+    (AbstractAtlas.cxx:240-362): a lattice of the given spacing centred in the template's
+    bounding box ((d x 2): min, max per axis), x outermost, coordinates accumulated by
+    repeated addition as the reference's loops do."""
+    box = np.asarray(bounding_box, dtype=np.float64)
+    lo, hi = box[:, 0], box[:, 1]
+    sp = float(spacing)
+    off = 0.5 * (hi - lo - sp * np.floor((hi - lo) / sp))
+    axes = []
+    for k in range(box.shape[0]):
+        vals, v = [], lo[k] + off[k]
+        while v <= hi[k]:
+            vals.append(v)
+            v += sp
+        axes.append(vals)
+    grid = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.ravel() for g in grid], axis=1)
+
+
+def surface_bounding_box(vertices, faces):
+    """Bounding box of a triangle mesh as OrientedSurfaceMesh / NonOrientedSurfaceMesh keep
+    it: over the triangle centers (OrientedSurfaceMesh.cxx:113-115, UpdateBoundingBox)."""
+    v = np.asarray(vertices, dtype=np.float64)
+    f = np.asarray(faces)
+    c = (v[f[:, 0]] + v[f[:, 1]] + v[f[:, 2]]) / 3.0
+    return np.stack([c.min(axis=0), c.max(axis=0)], axis=1)
+
+
+class HIPAtlasModel:
+    """DeterministicAtlas as the estimators see it (UpdateFixedEffectsAndComputeComplete-
+    LogLikelihood / ComputeCompleteLogLikelihoodGradient, DeterministicAtlas.cxx:100-201)
+    over a HIPDeterministicAtlas: fixed effects "ControlPoints" (n x 3) and "TemplateData"
+    ([vertices]), individual random effects "Momenta" (one n x 3 per subject).
+
+    A registration is the one-subject case with a frozen template (estimate_atlas.cxx:
+    229-233).  Gradient blocks follow the reference's flags: "ControlPoints" unless frozen,
+    "TemplateData" (smoothed) unless frozen, "Momenta"; the squared norms in that order,
+    the template's as <L2, smoothed> -- available here without a smoothing width only.
+    The bounding-box test of the reference's Diffeos is not made: out_of_box is False."""
+
+    def __init__(self, atlas, *, freeze_template=True, freeze_control_points=False):
+        self.atlas = atlas
+        self.freeze_template = bool(freeze_template)
+        self.freeze_control_points = bool(freeze_control_points)
+        if not self.freeze_template and atlas.desc.smoothing_width > 1e-20:
+            raise NotImplementedError("template updates with a smoothing width need the "
+                                      "unsmoothed template gradient; freeze the template")
+
+    @staticmethod
+    def _args(fixed, ind_rer):
+        return (np.asarray(fixed["ControlPoints"], dtype=np.float64),
+                np.stack([np.asarray(a, dtype=np.float64) for a in ind_rer["Momenta"]]),
+                np.asarray(fixed["TemplateData"][0], dtype=np.float64))
+
+    def UpdateFixedEffectsAndComputeCompleteLogLikelihood(self, fixed, pop_rer, ind_rer):
+        terms, _ = self.atlas.ComputeLogLikelihood(*self._args(fixed, ind_rer))
+        return np.asarray(terms, dtype=np.float64), False
+
+    def ComputeCompleteLogLikelihoodGradient(self, fixed, pop_rer, ind_rer):
+        _, gcp, gmom, gtpl = self.atlas.ComputeCompleteLogLikelihoodGradient(
+            *self._args(fixed, ind_rer))
+        pop, sq = {}, []
+        if not self.freeze_control_points:
+            pop["ControlPoints"] = gcp
+            sq.append(float(np.sum(gcp * gcp)))
+        if not self.freeze_template:
+            pop["TemplateData"] = [gtpl]
+            sq.append(float(np.sum(gtpl * gtpl)))
+        ind = {"Momenta": [g for g in gmom]}
+        sq.append(float(np.sum(gmom * gmom)))
+        return pop, ind, np.asarray(sq)
