@@ -216,17 +216,18 @@ struct Choice {
 
 // Heuristic, from the autotuner's measurements on the shapes of the workloads: one target
 // per lane (64-target wavefronts: the fewest registers, the most wavefronts per SIMD and
-// the tightest wavefront boxes); source splits aiming for 65536 wavefronts on grids of
+// the tightest wavefront boxes); source splits aiming for 262144 wavefronts on grids of
 // >= 4096 target workgroups and 16384 below (the short tail of many small wavefronts
 // balances the CUs); splits of at least 256 FP32 sources (4 tiles), so that the split
 // reduction stays short on small grids, or 128 FP64 sources (one tile) on grids of fewer
-// than 1024 target workgroups.  At N = M = 1M the fused FP32C sum runs in 361-374 ms with
-// this plan against 483 ms with 4 targets per lane, 4096 wavefronts and 256-source tiles.
+// than 1024 target workgroups.  At N = M = 1M the fused FP32C sum runs in 363 ms with this
+// plan (366 ms aiming for 65536 wavefronts, same box) against 483 ms with 4 targets per
+// lane, 4096 wavefronts and 256-source tiles.
 static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
   (void)op; (void)D; (void)DV;
   const long blocks = (max_nx + 63) / 64 * nseg;
   const int min_tiles = (prec == DFM_FP64 && blocks < 1024) ? 1 : 4;
-  return Choice{1, blocks >= 4096 ? 65536 : 16384, min_tiles};
+  return Choice{1, blocks >= 4096 ? 262144 : 16384, min_tiles};
 }
 
 // Candidates of the autotuner, the default first.  R never exceeds the op's default (the
@@ -241,9 +242,13 @@ static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg,
   auto same = [](const Choice &a, const Choice &b) {
     return a.R == b.R && a.waves == b.waves && a.min_tiles == b.min_tiles && a.mx == b.mx;
   };
+  // long grids: finer splits shorten the tail further (one target per lane only, so the
+  // split partials of the largest candidate, and the workspace, do not grow)
+  const long blocks = (max_nx + 63) / 64 * nseg;
   for (int R : Rs)
-    for (int waves : {4096, 16384, 65536})
+    for (int waves : {4096, 16384, 65536, 131072, 262144})
       for (int mt : {4, 1}) {
+        if (waves > 65536 && (R != 1 || blocks < 4096)) continue;
         Choice c{R, waves, mt, 0};
         if (!same(c, d)) out.push_back(c);
       }

@@ -19,7 +19,7 @@
  *   DFM_FP64   plain FP64 arithmetic and FP64 exp — the reference's arithmetic.
  *   DFM_FP32C  FP32 exp and multiply-add over LDS-staged source tiles, with targets
  *              spatially sorted and re-centred in FP64 per wavefront, FP32 partial sums
- *              of 32 sources folded into FP64 accumulators (compensated, cascaded
+ *              of 64 sources folded into FP64 accumulators (compensated, cascaded
  *              summation).  A wavefront whose targets span more than 4 scaled kernel
  *              widths forms its coordinate differences exactly (FP32 hi + lo parts).
  *   DFM_FP32X  as DFM_FP32C with exact coordinate differences in every wavefront.
