@@ -269,7 +269,7 @@ def _profiled(L, fn, reps):
     return (t1 - t0) / reps, tot.value / 1e3 / reps, cnt.value // reps
 
 
-def _tuned(L, fn, reps, explore=50):
+def _tuned(L, fn, reps, explore=64):
     """Explore plans with the autotuner over `explore` calls, then time `reps` calls with
     the tuned plans (autotuner back to heuristics afterwards)."""
     L.dfm_autotune_reset()
@@ -421,7 +421,7 @@ def workload_atlas(args):
     # explore plans, then capture the gradient with the tuned plans
     L.dfm_autotune_reset()
     L.dfm_autotune(1)
-    for _ in range(50):
+    for _ in range(64):  # every candidate of every key timed twice (at most 30 candidates)
         run()
     L.dfm_autotune(2)
     twall, tkern, _ = _profiled(L, run, 3)
