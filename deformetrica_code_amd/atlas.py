@@ -185,46 +185,90 @@ def surface_bounding_box(vertices, faces):
     return np.stack([c.min(axis=0), c.max(axis=0)], axis=1)
 
 
+def _copy_effects(effects):
+    return {k: ([np.array(x, dtype=np.float64) for x in v] if isinstance(v, (list, tuple))
+                else np.array(v, dtype=np.float64)) for k, v in effects.items()}
+
+
 class HIPAtlasModel:
-    """DeterministicAtlas as the estimators see it (UpdateFixedEffectsAndComputeComplete-
-    LogLikelihood / ComputeCompleteLogLikelihoodGradient, DeterministicAtlas.cxx:100-201)
-    over a HIPDeterministicAtlas: fixed effects "ControlPoints" (n x 3) and "TemplateData"
-    ([vertices]), individual random effects "Momenta" (one n x 3 per subject).
+    """DeterministicAtlas as the estimators see it (AbstractStatisticalModel's
+    Get/SetFixedEffects, UpdateFixedEffectsAndComputeCompleteLogLikelihood,
+    ComputeCompleteLogLikelihoodGradient; DeterministicAtlas.cxx:100-201) over a
+    HIPDeterministicAtlas created with smoothing_width 0.  Fixed effects "ControlPoints"
+    (n x 3) and "TemplateData" ([vertices]); individual random effects "Momenta" (one
+    n x 3 per subject).
 
+    The reference's behaviour is kept where it shapes the numbers:
+    - SetFixedEffects (AbstractAtlas.cxx:52-63) stores the new fixed effects but moves the
+      template geometry to the template data it held *before* the call, so the geometry the
+      shot and the data term use lags one call behind the "TemplateData" fixed effect
+      (frozen template: the fixed effect is restored and the geometry stays);
+    - the template gradient is smoothed with the "TemplateData" fixed effect as sources and
+      targets (AbstractAtlas::ConvolveGradTemplate, AbstractAtlas.cxx:727-751), here by one
+      GPU Convolve at ``smoothing_width`` (<= 1e-20: none);
+    - gradient blocks "ControlPoints" unless frozen, "TemplateData" (smoothed) unless
+      frozen, "Momenta"; squared norms in that order, the template's as <L2, smoothed>
+      (DeterministicAtlas.cxx:197-200).
     A registration is the one-subject case with a frozen template (estimate_atlas.cxx:
-    229-233).  Gradient blocks follow the reference's flags: "ControlPoints" unless frozen,
-    "TemplateData" (smoothed) unless frozen, "Momenta"; the squared norms in that order,
-    the template's as <L2, smoothed> -- available here without a smoothing width only.
-    The bounding-box test of the reference's Diffeos is not made: out_of_box is False."""
+    229-233).  The bounding-box test of the reference's Diffeos is not made: out_of_box is
+    always False."""
 
-    def __init__(self, atlas, *, freeze_template=True, freeze_control_points=False):
+    def __init__(self, atlas, fixed_effects, *, freeze_template=False,
+                 freeze_control_points=False, smoothing_width=0.0):
+        if atlas.desc.smoothing_width > 1e-20:
+            raise ValueError("create the atlas with smoothing_width 0: the model smooths the "
+                             "template gradient itself")
         self.atlas = atlas
         self.freeze_template = bool(freeze_template)
         self.freeze_control_points = bool(freeze_control_points)
-        if not self.freeze_template and atlas.desc.smoothing_width > 1e-20:
-            raise NotImplementedError("template updates with a smoothing width need the "
-                                      "unsmoothed template gradient; freeze the template")
+        self.smoothing_width = float(smoothing_width)
+        self._fixed = _copy_effects(fixed_effects)
+        self._geometry = np.array(self._fixed["TemplateData"][0], dtype=np.float64)
+        self._precision = int(atlas.desc.precision)
 
-    @staticmethod
-    def _args(fixed, ind_rer):
-        return (np.asarray(fixed["ControlPoints"], dtype=np.float64),
-                np.stack([np.asarray(a, dtype=np.float64) for a in ind_rer["Momenta"]]),
-                np.asarray(fixed["TemplateData"][0], dtype=np.float64))
+    def GetFixedEffects(self):
+        return _copy_effects(self._fixed)
 
-    def UpdateFixedEffectsAndComputeCompleteLogLikelihood(self, fixed, pop_rer, ind_rer):
-        terms, _ = self.atlas.ComputeLogLikelihood(*self._args(fixed, ind_rer))
+    def SetFixedEffects(self, fixed):
+        template_memory = self._fixed["TemplateData"]
+        cp_memory = self._fixed["ControlPoints"]
+        self._fixed = _copy_effects(fixed)
+        if self.freeze_template:
+            self._fixed["TemplateData"] = template_memory
+        else:
+            self._geometry = np.array(template_memory[0], dtype=np.float64)
+        if self.freeze_control_points:
+            self._fixed["ControlPoints"] = cp_memory
+
+    def _momenta(self, ind_rer):
+        return np.stack([np.asarray(a, dtype=np.float64) for a in ind_rer["Momenta"]])
+
+    def UpdateFixedEffectsAndComputeCompleteLogLikelihood(self, pop_rer, ind_rer):
+        terms, _ = self.atlas.ComputeLogLikelihood(self._fixed["ControlPoints"],
+                                                   self._momenta(ind_rer), self._geometry)
         return np.asarray(terms, dtype=np.float64), False
 
-    def ComputeCompleteLogLikelihoodGradient(self, fixed, pop_rer, ind_rer):
-        _, gcp, gmom, gtpl = self.atlas.ComputeCompleteLogLikelihoodGradient(
-            *self._args(fixed, ind_rer))
+    def _smooth(self, gl2):
+        if self.smoothing_width < 1e-20:
+            return gl2
+        from .kernel import HIPExactKernel
+        tpl = np.asarray(self._fixed["TemplateData"][0], dtype=np.float64)
+        k = HIPExactKernel(3, self.smoothing_width, precision=self._precision)
+        k.SetSources(tpl)
+        k.SetWeights(gl2)
+        return k.Convolve(tpl)
+
+    def ComputeCompleteLogLikelihoodGradient(self, pop_rer, ind_rer):
+        _, gcp, gmom, gl2 = self.atlas.ComputeCompleteLogLikelihoodGradient(
+            self._fixed["ControlPoints"], self._momenta(ind_rer), self._geometry)
         pop, sq = {}, []
         if not self.freeze_control_points:
             pop["ControlPoints"] = gcp
             sq.append(float(np.sum(gcp * gcp)))
         if not self.freeze_template:
-            pop["TemplateData"] = [gtpl]
-            sq.append(float(np.sum(gtpl * gtpl)))
+            sob = self._smooth(gl2)
+            pop["TemplateData"] = [sob]
+            sq.append(float(np.sum(gl2 * sob)))
         ind = {"Momenta": [g for g in gmom]}
         sq.append(float(np.sum(gmom * gmom)))
         return pop, ind, np.asarray(sq)

@@ -80,6 +80,72 @@ def samples_match_gradient(kind, sigma, cs, Ns, ct, Nt, precision="fp32c", gradi
     return out, gc, gn
 
 
+def reorient_normals(vertices, faces):
+    """The triangle order OrientedSurfaceMesh gets with ``reorient-normals`` on
+    (OrientedSurfaceMesh.cxx:274-278: vtkPolyDataNormals with ConsistencyOn,
+    AutoOrientNormalsOn, FlipNormalsOn; myvtkPolyDataNormals.cxx:212-305, 549-...):
+
+    per connected component, the seed is the unvisited triangle at the leftmost point whose
+    normal (p1 - p0) x (p2 - p0) has the largest |x| component; it is oriented so that this
+    normal points to +x (outward-left, then flipped), and every triangle reached through a
+    shared edge is reversed where it runs that edge in the same direction as its visited
+    neighbour.  For a closed surface the result has (p1 - p0) x (p2 - p0) pointing inward,
+    so the current's normals (p2 - p0) x (p1 - p0) / 2 point outward.  Reversing a
+    triangle reverses its vertex order, as vtkPolyData::ReverseCell does.  Returns a new
+    (F, 3) int32 array."""
+    V = np.asarray(vertices, dtype=np.float64)
+    F = np.array(faces, dtype=np.int64, copy=True)
+    nf = F.shape[0]
+    if nf == 0:
+        return F.astype(np.int32)
+    edges = {}
+    cells_at = [[] for _ in range(V.shape[0])]
+    for c, tri in enumerate(F):
+        for k in range(3):
+            a, b = int(tri[k]), int(tri[(k + 1) % 3])
+            edges.setdefault((min(a, b), max(a, b)), []).append(c)
+            cells_at[a].append(c)
+    visited = np.zeros(nf, dtype=bool)
+
+    def normal_x(c):
+        p0, p1, p2 = V[F[c, 0]], V[F[c, 1]], V[F[c, 2]]
+        return np.cross(p1 - p0, p2 - p0)[0]
+
+    def runs(c, a, b):  # does triangle c traverse a -> b?
+        t = F[c]
+        return any(int(t[k]) == a and int(t[(k + 1) % 3]) == b for k in range(3))
+
+    for pt in np.argsort(V[:, 0], kind="stable"):
+        best, best_abs = -1, 0.0
+        for c in cells_at[pt]:
+            if visited[c]:
+                continue
+            nx = normal_x(c)
+            if abs(nx) > best_abs:
+                best, best_abs = c, abs(nx)
+        if best < 0:
+            continue
+        if not normal_x(best) > 0:  # reverse flag XOR flip: make the seed's normal face +x
+            F[best] = F[best][::-1]
+        visited[best] = True
+        wave = [best]
+        while wave:  # TraverseAndOrder
+            nxt = []
+            for c in wave:
+                t = F[c]
+                for k in range(3):
+                    a, b = int(t[k]), int(t[(k + 1) % 3])
+                    for nb in edges[(min(a, b), max(a, b))]:
+                        if nb == c or visited[nb]:
+                            continue
+                        if runs(nb, a, b):
+                            F[nb] = F[nb][::-1]
+                        visited[nb] = True
+                        nxt.append(nb)
+            wave = nxt
+    return F.astype(np.int32)
+
+
 class _SurfaceMesh:
     KIND = None
     TYPE_ERROR = "AbstractGeometries types mismatched"
