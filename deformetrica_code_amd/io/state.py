@@ -448,10 +448,11 @@ def push_fast_gradient_ascent(state, fields):
     return state
 
 
-def scalar_close(x, y, tolerance):
+def scalar_close(x, y, tolerance, rtol=0.0):
     """ScalarCompare (Tolerance.hpp:20-31): |x - y|, rounded to float, at most the tolerance
-    (an absolute criterion)."""
-    return bool(abs(np.float32(float(x) - float(y))) <= np.float32(tolerance))
+    (an absolute criterion); ``rtol`` adds rtol * |y| to the bound (0: the reference's)."""
+    bound = np.float32(tolerance) + np.float32(rtol * abs(float(y)))
+    return bool(abs(np.float32(float(x) - float(y))) <= bound)
 
 
 # GradientAscent snapshot (GradientAscent.cxx:78-80, 217-219)
@@ -475,13 +476,14 @@ def push_gradient_ascent(state, fields):
     return state
 
 
-def states_close(a, b, tolerance, exact=False):
+def states_close(a, b, tolerance, exact=False, rtol=0.0):
     """DeformationState::compare (SerializeDeformationState.h:151-200): integers, bools and
     strings equal; floating values within ``tolerance`` as ScalarCompare judges them (or
-    equal, with ``exact``)."""
+    equal, with ``exact``).  ``rtol`` widens each bound by rtol * |value of b|, for values
+    whose magnitude puts one rounding step above an absolute tolerance."""
 
     def scalar(x, y):
-        return float(x) == float(y) if exact else scalar_close(x, y, tolerance)
+        return float(x) == float(y) if exact else scalar_close(x, y, tolerance, rtol)
 
     def close(x, y):
         if isinstance(x, dict):
