@@ -31,6 +31,11 @@ constexpr int kTileF64 = 128;
 constexpr int kTileF32 = 64;
 constexpr int kFoldF32 = 64;
 
+// FP32 single calls Morton-order their targets from this many targets on.  Below it the
+// sort's launches (bounding box, keys, radix passes) cost more than the compact wavefronts
+// save, and wide wavefronts take the exact-difference path instead.
+constexpr long kSortMinTargets = 8192;
+
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
 // (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
 constexpr double kExactExtent = 4.0;

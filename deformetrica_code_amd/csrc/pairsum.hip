@@ -300,7 +300,7 @@ static Plan batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total,
 static Plan single_plan(int op, int prec, int D, int DV, long nx, long ny, const Choice &c) {
   Plan pl{};
   split_plan(pl, op, prec, D, DV, 1, nx, ny, c);
-  pl.sort = prec != DFM_FP64;
+  pl.sort = prec != DFM_FP64 && nx >= kSortMinTargets;
   size_t off = 0;
   if (pl.sort) {
     pl.off_bbox = off;
