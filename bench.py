@@ -172,8 +172,9 @@ def main():
         oracle.convolve_grad1_threaded(X[:probe], A[:probe], Y, W, sigma, threads)
         tb = time.perf_counter()
         per_row = (tb - ta) / probe
-        rows = int(max(probe, min(N, args.cpu_seconds / max(per_row, 1e-9))))
-        rows = (rows // threads) * threads or probe
+        # at least 4,096 rows for the error check, more if the time budget allows
+        rows = int(max(4096, probe, min(N, args.cpu_seconds / max(per_row, 1e-9))))
+        rows = min(N, (rows // threads) * threads or probe)
         idx = np.random.default_rng(7).choice(N, size=rows, replace=False)
         idx.sort()
         ta = time.perf_counter()
@@ -183,8 +184,10 @@ def main():
             "value": rows * M / (tb - ta), "unit": "pair-evals/s", "cores": threads,
             "kind": "port",
             "sample": "%d random target rows x all %d sources, FP64 Convolve then "
-                      "ConvolveGradient(X, alpha) (restated ExactKernel loops, libm exp), "
-                      "%.1f s on %d threads" % (rows, M, tb - ta, threads),
+                      "ConvolveGradient(X, alpha) (restated ExactKernel loops, libm exp, "
+                      "without the reference's per-pair heap allocations: faster than the "
+                      "reference), %.1f s on %d threads" % (rows, M, tb - ta, threads),
+            "scan": _cpu_scan(oracle, sigma, threads),
         }
         err_oracle = {
             "rows": rows,
@@ -236,6 +239,27 @@ def main():
         "error_vs_fp64_oracle": err_oracle,
     }
     print(json.dumps(out), flush=True)
+
+
+def _cpu_scan(oracle, sigma, threads):
+    """The oracle's fused sum at N = M in {2^12, 2^14, 2^16}, on one thread and on
+    `threads` threads (2^16 on one thread is skipped: ~70 s), in pair-evals/s; 1e12-pair
+    times are these rates extrapolated (cost proportional to N * M)."""
+    rng = np.random.default_rng(20261205)
+    out = []
+    for n in (1 << 12, 1 << 14, 1 << 16):
+        X = rng.uniform(0.0, 1.0, size=(n, 3))
+        W = rng.normal(size=(n, 3))
+        A = rng.normal(size=(n, 3))
+        for t in (1, threads):
+            if n == 1 << 16 and t == 1:
+                continue
+            ta = time.perf_counter()
+            oracle.convolve_grad1_threaded(X, A, X, W, sigma, t)
+            dt = time.perf_counter() - ta
+            out.append({"n": n, "threads": t, "pair_evals_per_s": n * n / dt,
+                        "extrapolated_1e12_pairs_s": 1e12 * dt / (n * n)})
+    return out
 
 
 # ---------------------------------------------------------------------------------------
