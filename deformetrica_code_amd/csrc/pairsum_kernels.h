@@ -55,9 +55,9 @@ template <typename T>
 struct Prec;
 
 // kFold: sources summed in the working precision before the partial sum is folded into
-// the FP64 accumulator.  In FP32 a 32-source partial keeps the rounding of each add
-// proportional to 32 terms even when a wide kernel makes every pair significant and the
-// sum cancels.
+// the FP64 accumulator.  In FP32 a one-tile (64-source) partial keeps the rounding of each
+// add proportional to 64 terms even when a wide kernel makes every pair significant and
+// the sum cancels.
 template <>
 struct Prec<float> {
   static constexpr bool kCenter = true;
