@@ -126,8 +126,16 @@ def load():
     L.dfm_samples_workspace.argtypes = [i, l, l, i, i]
     L.dfm_samples_workspace.restype = sz
     L.dfm_samples_match_gradient.argtypes = [vp, i, d, i, vp, vp, l, vp, vp, l, vp, vp, vp, vp, sz]
+    # polyline data terms
+    L.dfm_cell_incidence.argtypes = [vp, l, i, l, vp, vp]
+    L.dfm_curve_workspace.argtypes = [i, i, l, l, i, i]
+    L.dfm_curve_workspace.restype = sz
+    L.dfm_curve_match.argtypes = [vp, i, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
+    L.dfm_curve_match_gradient.argtypes = [vp, i, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l,
+                                           vp, vp, sz]
     for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient",
-                 "dfm_samples_match_gradient"):
+                 "dfm_samples_match_gradient", "dfm_cell_incidence", "dfm_curve_match",
+                 "dfm_curve_match_gradient"):
         getattr(L, name).restype = i
     L.dfm_autotune.argtypes = [i]
     L.dfm_autotune.restype = i

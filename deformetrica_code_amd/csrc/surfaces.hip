@@ -1,6 +1,8 @@
-// surfaces.hip — triangle-surface data terms on the GPU: currents (OrientedSurfaceMesh)
-// and varifolds (NonOrientedSurfaceMesh), their match values and their gradients with
-// respect to the deformed template's vertices.
+// surfaces.hip — triangle-surface and polyline data terms on the GPU: currents
+// (OrientedSurfaceMesh, OrientedPolyLine) and varifolds (NonOrientedSurfaceMesh,
+// NonOrientedPolyLine), their match values and their gradients with respect to the deformed
+// template's vertices.  The polyline path (segments in 2D or 3D) follows the same plan with
+// k_seg_geometry / k_seg_contrib in place of the face kernels.
 //
 // Every kernel sum is one pair-sum call; the rest is per-face and per-vertex work:
 //   k_face_geometry   centers, kernel weights W_f and per-face coefficients A_f
@@ -174,8 +176,9 @@ __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V,
   }
 }
 
-// grad[v] = scale * sum of the corner contributions incident to v, in incidence order
-// (nsub meshes sharing one incidence list of ncorner corners per mesh)
+// grad[v] = scale * sum of the corner contributions (D values each) incident to v, in
+// incidence order (nsub meshes sharing one incidence list of ncorner corners per mesh)
+template <int D>
 __global__ __launch_bounds__(256) void k_vertex_gather(const int *offsets, const int *inc,
                                                        long nv, long ncorner, int nsub,
                                                        const double *contrib, double scale,
@@ -183,15 +186,16 @@ __global__ __launch_bounds__(256) void k_vertex_gather(const int *offsets, const
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv * nsub) return;
   const long sub = v / nv, vl = v - sub * nv;
-  contrib += sub * ncorner * 3;
-  double g[3] = {0.0, 0.0, 0.0};
+  contrib += sub * ncorner * D;
+  double g[D];
+  for (int k = 0; k < D; k++) g[k] = 0.0;
   const long b = max(offsets[vl], 0), e = min((long)offsets[vl + 1], ncorner);
   for (long i = b; i < e; i++) {
     const int c = inc[i];
     if (c < 0 || c >= ncorner) continue;
-    for (int k = 0; k < 3; k++) g[k] += contrib[3 * (long)c + k];
+    for (int k = 0; k < D; k++) g[k] += contrib[D * (long)c + k];
   }
-  for (int k = 0; k < 3; k++) grad[3 * v + k] = scale * g[k];
+  for (int k = 0; k < D; k++) grad[D * v + k] = scale * g[k];
 }
 
 // out[s] = sum over the rows of segment s of dot(X_row, Y_row) (dv columns); segments of
@@ -268,6 +272,100 @@ __global__ __launch_bounds__(256) void k_sample_grads(int kind, const double *N,
   for (int k = 0; k < 3; k++) grad_n[3 * i + k] = g[k] / s - v[k] * ng / (2.0 * s * s * s * a);
 }
 
+// ---- polylines: segments (i0, i1) of points in D = 2 or 3 ----------------------------
+
+// symmetric D x D matrix from its upper triangle (row by row) times x
+template <int D>
+__device__ __forceinline__ void sym_times_d(const double *m, const double *x, double *y) {
+  for (int p = 0; p < D; p++) y[p] = 0.0;
+  int idx = 0;
+  for (int p = 0; p < D; p++)
+    for (int q = p; q < D; q++, idx++) {
+      y[p] += m[idx] * x[q];
+      if (q != p) y[q] += m[idx] * x[p];
+    }
+}
+
+// Geometry of each segment: centre (p0 + p1) / 2; current W = A = T = p1 - p0
+// (OrientedPolyLine.cxx:249-272); varifold T = (p1 - p0) / sqrt(|p1 - p0| + 1e-20),
+// W = upper triangle of T T^T, A = its quadratic-form coefficients
+// (NonOrientedPolyLine.cxx:266-297).  Bad indices flag the segment (weights zero).
+template <int D>
+__global__ __launch_bounds__(256) void k_seg_geometry(int kind, const double *V, long nv,
+                                                      const int *E, long ne, double *C,
+                                                      double *W, double *A, double *T,
+                                                      double *bad) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ne) return;
+  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  const int i0 = E[2 * f], i1 = E[2 * f + 1];
+  if (i0 < 0 || i1 < 0 || i0 >= nv || i1 >= nv) {
+    for (int k = 0; k < D; k++) C[D * f + k] = 0.0, T[D * f + k] = 0.0;
+    for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
+    *bad = 1.0;
+    return;
+  }
+  double t[D], t2 = 0.0;
+  for (int k = 0; k < D; k++) {
+    const double a = V[D * (long)i0 + k], b = V[D * (long)i1 + k];
+    C[D * f + k] = (a + b) / 2.0;
+    t[k] = b - a;
+    t2 += t[k] * t[k];
+  }
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < D; k++) W[D * f + k] = A[D * f + k] = T[D * f + k] = t[k];
+    return;
+  }
+  const double s = sqrt(sqrt(t2) + 1e-20);
+  for (int k = 0; k < D; k++) T[D * f + k] = t[k] /= s;
+  int idx = 0;
+  for (int p = 0; p < D; p++)
+    for (int q = p; q < D; q++, idx++) {
+      W[dv * f + idx] = t[p] * t[q];
+      A[dv * f + idx] = (p == q ? 1.0 : 2.0) * t[p] * t[q];
+    }
+}
+
+// contrib[f] (2 endpoints x D) of the match gradient, from kt = (K W_S)(c_S) - (K W_T)(c_S)
+// and g = Grad1 differences at the segment centres:
+//   current   endpoint 0: -2 kt + g, endpoint 1: 2 kt + g (OrientedPolyLine.cxx:216-234)
+//   varifold  u = 4 S(kt) T, u = (u - T (T.u) / (2 |T|^2)) / |T|; endpoints -u + g, u + g,
+//             nothing where |T|^2 <= 1e-20 (NonOrientedPolyLine.cxx:215-251)
+template <int D>
+__global__ __launch_bounds__(256) void k_seg_contrib(int kind, const int *E, long ne, long nv,
+                                                     const double *T, const double *ktS,
+                                                     const double *gS, const double *ktT,
+                                                     const double *gT, double *contrib) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ne) return;
+  double *out = contrib + 2 * D * f;
+  for (int k = 0; k < 2 * D; k++) out[k] = 0.0;
+  const int i0 = E[2 * f], i1 = E[2 * f + 1];
+  if (i0 < 0 || i1 < 0 || i0 >= nv || i1 >= nv) return;
+  double u[D], g[D];
+  for (int k = 0; k < D; k++) g[k] = gS[D * f + k] - gT[D * f + k];
+  if (kind == DFM_CURRENT) {
+    for (int k = 0; k < D; k++) u[k] = 2.0 * (ktS[D * f + k] - ktT[D * f + k]);
+  } else {
+    constexpr int dv = D * (D + 1) / 2;
+    const double *n = T + D * f;
+    double n2 = 0.0;
+    for (int k = 0; k < D; k++) n2 += n[k] * n[k];
+    if (!(n2 > 1e-20)) return;
+    double dm[dv];
+    for (int k = 0; k < dv; k++) dm[k] = ktS[dv * f + k] - ktT[dv * f + k];
+    sym_times_d<D>(dm, n, u);
+    double nu = 0.0;
+    for (int k = 0; k < D; k++) nu += n[k] * (u[k] *= 4.0);
+    const double r = 1.0 / sqrt(n2);
+    for (int k = 0; k < D; k++) u[k] = (u[k] - n[k] * nu / (2 * n2)) * r;
+  }
+  for (int k = 0; k < D; k++) {
+    out[k] = -u[k] + g[k];
+    out[D + k] = u[k] + g[k];
+  }
+}
+
 size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct SampleLayout {
@@ -337,6 +435,43 @@ struct Layout {
   }
 };
 
+struct CurveLayout {
+  double *cS, *wS, *aS, *tS, *cT, *wT, *aT, *tT;
+  double *k1, *k2, *k3, *g1, *g2;
+  double *p1, *p2, *p3;
+  double *contrib;
+  double *bad;
+  void *pw;
+  size_t pw_bytes, total;
+  CurveLayout(char *base, int kind, int D, int prec, long nes, long net, bool gradient) {
+    const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+    size_t off = 0;
+    auto take = [&](size_t count) {
+      double *p = base ? (double *)(base + off) : nullptr;
+      off += a256(count * sizeof(double));
+      return p;
+    };
+    cS = take(D * nes); wS = take(dv * nes); aS = take(dv * nes); tS = take(D * nes);
+    cT = take(D * net); wT = take(dv * net); aT = take(dv * net); tT = take(D * net);
+    const long nmax = std::max(nes, net);
+    k1 = take(dv * nmax); k2 = take(dv * nmax); k3 = take(dv * nmax);
+    g1 = take(D * nes); g2 = take(D * nes);
+    p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
+    contrib = gradient ? take(2 * D * nes) : nullptr;
+    bad = take(1);
+    if (gradient)
+      pw_bytes = std::max(pairsum_workspace(OP_CONVGRAD1, prec, D, dv, nes, nes),
+                          pairsum_workspace(OP_CONVGRAD1, prec, D, dv, nes, net));
+    else
+      pw_bytes = std::max({pairsum_workspace(OP_CONV, prec, D, dv, nes, nes),
+                           pairsum_workspace(OP_CONV, prec, D, dv, net, net),
+                           pairsum_workspace(OP_CONV, prec, D, dv, net, nes)});
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
+
 int launched(const char *what) {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
@@ -376,12 +511,88 @@ int geometry(hipStream_t st, int kind, const Layout &L, const double *vs, long n
   return launched("k_face_geometry");
 }
 
-int report_bad(hipStream_t st, const Layout &L) {
+int report_bad(hipStream_t st, const double *flag, const char *cell) {
   double bad = 0.0;
-  hipError_t e = hipMemcpyAsync(&bad, L.bad, sizeof(double), hipMemcpyDeviceToHost, st);
+  hipError_t e = hipMemcpyAsync(&bad, flag, sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return set_error(DFM_ERR_HIP, "flag read: %s", hipGetErrorString(e));
-  return bad != 0.0 ? set_error(DFM_ERR_ARG, "a face references a vertex index outside [0, nv)") : DFM_OK;
+  return bad != 0.0 ? set_error(DFM_ERR_ARG, "a %s references a vertex index outside [0, nv)", cell)
+                    : DFM_OK;
+}
+
+int report_bad(hipStream_t st, const Layout &L) { return report_bad(st, L.bad, "face"); }
+
+int check_curve_args(int kind, int dim, int prec, double sigma, long nvs, long nes, long nvt,
+                     long net) {
+  if (dim != 2 && dim != 3) return set_error(DFM_ERR_DIM, "polylines need dimension 2 or 3, got %d", dim);
+  return check_args(kind, prec, sigma, nvs, nes, nvt, net);
+}
+
+template <int D>
+int curve_geometry(hipStream_t st, int kind, const CurveLayout &L, const double *vs, long nvs,
+                   const int *es, long nes, const double *vt, long nvt, const int *et, long net) {
+  int rc = fill(st, L.bad, 1, 0.0);
+  if (rc) return rc;
+  if (nes > 0)
+    hipLaunchKernelGGL(k_seg_geometry<D>, dim3(blocks_for(nes)), dim3(256), 0, st, kind, vs, nvs,
+                       es, nes, L.cS, L.wS, L.aS, L.tS, L.bad);
+  if (net > 0)
+    hipLaunchKernelGGL(k_seg_geometry<D>, dim3(blocks_for(net)), dim3(256), 0, st, kind, vt, nvt,
+                       et, net, L.cT, L.wT, L.aT, L.tT, L.bad);
+  return launched("k_seg_geometry");
+}
+
+template <int D>
+int curve_match(hipStream_t st, int kind, double sigma, int prec, const CurveLayout &L,
+                const double *vs, long nvs, const int *es, long nes, const double *vt, long nvt,
+                const int *et, long net, double *match) {
+  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  int rc = curve_geometry<D>(st, kind, L, vs, nvs, es, nes, vt, nvt, et, net);
+  if (rc) return rc;
+  KParams p = blank();
+  // |S|^2 = sum_f A_S,f . (K W_S)(c_S,f) (OrientedPolyLine.cxx:279-296,
+  // NonOrientedPolyLine.cxx:304-324), |T|^2 likewise, <S, T> = sum_f A_T,f . (K W_S)(c_T,f)
+  p.x = L.cS; p.nx = nes; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k1;
+  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, nes * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.nx = net; p.y = L.cT; p.w = L.wT; p.ny = net; p.out0 = L.k2;
+  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.nx = net; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k3;
+  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, nes, dv, L.p1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aT, net, dv, L.p2);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, L.aT, net, dv, L.p3);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, st, L.p2, 1.0, L.p1, 1.0, L.p3, -2.0, match);
+  if ((rc = launched("match reduction"))) return rc;
+  return report_bad(st, L.bad, "segment");
+}
+
+template <int D>
+int curve_match_gradient(hipStream_t st, int kind, double sigma, int prec, const CurveLayout &L,
+                         const double *vs, long nvs, const int *es, long nes, const int *offsets,
+                         const int *incidences, const double *vt, long nvt, const int *et,
+                         long net, double *grad) {
+  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  int rc = curve_geometry<D>(st, kind, L, vs, nvs, es, nes, vt, nvt, et, net);
+  if (rc) return rc;
+  KParams p = blank();
+  // (K W_S)(c_S) with ConvolveGradient(c_S, A_S), then the same from the target; for the
+  // varifold, Grad1 with alpha = A is the reference's contraction of the per-weight
+  // gradients with T T^T (NonOrientedPolyLine.cxx:239-249)
+  p.x = L.cS; p.ta = L.aS; p.nx = nes; p.y = L.cS; p.w = L.wS; p.ny = nes;
+  p.out0 = L.k1; p.out1 = L.g1;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
+    return rc;
+  p.y = L.cT; p.w = L.wT; p.ny = net; p.out0 = L.k2; p.out1 = L.g2;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
+    return rc;
+  if (nes > 0)
+    hipLaunchKernelGGL(k_seg_contrib<D>, dim3(blocks_for(nes)), dim3(256), 0, st, kind, es, nes,
+                       nvs, L.tS, L.k1, L.g1, L.k2, L.g2, L.contrib);
+  if (nvs > 0)
+    hipLaunchKernelGGL(k_vertex_gather<D>, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
+                       incidences, nvs, 2 * nes, 1, L.contrib, 1.0, grad);
+  if ((rc = launched("gradient assembly"))) return rc;
+  return report_bad(st, L.bad, "segment");
 }
 
 }  // namespace
@@ -406,7 +617,7 @@ int surf_contrib(hipStream_t st, int kind, const double *V, long nv, const int *
 int surf_gather(hipStream_t st, const int *offsets, const int *inc, long nv, long nf, int nsub,
                 const double *contrib, double scale, double *grad) {
   if (nv * nsub <= 0) return DFM_OK;
-  hipLaunchKernelGGL(k_vertex_gather, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets, inc,
+  hipLaunchKernelGGL(k_vertex_gather<3>, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets, inc,
                      nv, 3 * nf, nsub, contrib, scale, grad);
   return launched("k_vertex_gather");
 }
@@ -425,21 +636,75 @@ using namespace dfm;
 
 extern "C" {
 
-int dfm_mesh_incidence(const int *faces, long nf, long nv, int *offsets, int *incidences) {
+int dfm_cell_incidence(const int *cells, long ncells, int corners, long nv, int *offsets,
+                       int *incidences) {
   clear_error();
-  if (nf < 0 || nv < 0 || nf > 0x2aaaaaaaL) return set_error(DFM_ERR_ARG, "bad counts");
-  if ((nf > 0 && (!faces || !incidences)) || !offsets) return set_error(DFM_ERR_ARG, "NULL argument");
+  if (corners < 1 || corners > 3) return set_error(DFM_ERR_ARG, "cells of %d corners", corners);
+  if (ncells < 0 || nv < 0 || nv > 0x7fffffffL || ncells > 0x7fffffffL / corners)
+    return set_error(DFM_ERR_ARG, "bad counts");
+  if ((ncells > 0 && (!cells || !incidences)) || !offsets)
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  const long ncorner = (long)corners * ncells;
   std::vector<int> cnt((size_t)nv + 1, 0);
-  for (long i = 0; i < 3 * nf; i++) {
-    const int v = faces[i];
-    if (v < 0 || v >= nv) return set_error(DFM_ERR_ARG, "face %ld references vertex %d outside [0, %ld)", i / 3, v, nv);
+  for (long i = 0; i < ncorner; i++) {
+    const int v = cells[i];
+    if (v < 0 || v >= nv)
+      return set_error(DFM_ERR_ARG, "cell %ld references vertex %d outside [0, %ld)", i / corners,
+                       v, nv);
     cnt[v + 1]++;
   }
   for (long v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
   for (long v = 0; v <= nv; v++) offsets[v] = cnt[v];
   std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-  for (long i = 0; i < 3 * nf; i++) incidences[pos[faces[i]]++] = (int)i;  // 3 * face + corner
+  for (long i = 0; i < ncorner; i++) incidences[pos[cells[i]]++] = (int)i;  // corners * cell + corner
   return DFM_OK;
+}
+
+int dfm_mesh_incidence(const int *faces, long nf, long nv, int *offsets, int *incidences) {
+  return dfm_cell_incidence(faces, nf, 3, nv, offsets, incidences);
+}
+
+size_t dfm_curve_workspace(int kind, int dim, long nes, long net, int precision, int gradient) {
+  if (check_curve_args(kind, dim, precision, 1.0, 0, nes, 0, net)) return 0;
+  return CurveLayout(nullptr, kind, dim, precision, nes, net, gradient != 0).total;
+}
+
+int dfm_curve_match(void *stream, int kind, int dim, double sigma, int prec, const double *vs,
+                    long nvs, const int *es, long nes, const double *vt, long nvt, const int *et,
+                    long net, double *match, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_curve_args(kind, dim, prec, sigma, nvs, nes, nvt, net);
+  if (rc) return rc;
+  if (!match || (nes > 0 && (!vs || !es)) || (net > 0 && (!vt || !et)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  CurveLayout need(nullptr, kind, dim, prec, nes, net, false);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  CurveLayout L((char *)work, kind, dim, prec, nes, net, false);
+  hipStream_t st = (hipStream_t)stream;
+  return dim == 2 ? curve_match<2>(st, kind, sigma, prec, L, vs, nvs, es, nes, vt, nvt, et, net, match)
+                  : curve_match<3>(st, kind, sigma, prec, L, vs, nvs, es, nes, vt, nvt, et, net, match);
+}
+
+int dfm_curve_match_gradient(void *stream, int kind, int dim, double sigma, int prec,
+                             const double *vs, long nvs, const int *es, long nes,
+                             const int *offsets, const int *incidences, const double *vt,
+                             long nvt, const int *et, long net, double *grad, void *work,
+                             size_t work_bytes) {
+  clear_error();
+  int rc = check_curve_args(kind, dim, prec, sigma, nvs, nes, nvt, net);
+  if (rc) return rc;
+  if (!grad || !offsets || (nes > 0 && (!vs || !es || !incidences)) || (net > 0 && (!vt || !et)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  CurveLayout need(nullptr, kind, dim, prec, nes, net, true);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  CurveLayout L((char *)work, kind, dim, prec, nes, net, true);
+  hipStream_t st = (hipStream_t)stream;
+  return dim == 2 ? curve_match_gradient<2>(st, kind, sigma, prec, L, vs, nvs, es, nes, offsets,
+                                            incidences, vt, nvt, et, net, grad)
+                  : curve_match_gradient<3>(st, kind, sigma, prec, L, vs, nvs, es, nes, offsets,
+                                            incidences, vt, nvt, et, net, grad);
 }
 
 size_t dfm_surface_workspace(int kind, long nfs, long nft, int precision, int gradient) {
@@ -510,7 +775,7 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, c
     hipLaunchKernelGGL(k_face_contrib, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
                        nfs, 1, L.nS, L.k1, L.g1, L.k2, L.g2, L.contrib);
   if (nvs > 0)
-    hipLaunchKernelGGL(k_vertex_gather, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
+    hipLaunchKernelGGL(k_vertex_gather<3>, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
                        incidences, nvs, 3 * nfs, 1, L.contrib, 1.0, grad);
   if ((rc = launched("gradient assembly"))) return rc;
   return report_bad(st, L);

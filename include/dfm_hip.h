@@ -376,6 +376,44 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int precisi
                                long nvt, const int *ft, long nft, double *grad, void *work,
                                size_t work_bytes);
 
+/* ---- polyline data terms (device pointers) ---------------------------------------------
+ * Polylines are points (nv x dim, dim = 2 or 3, FP64, row-major) and segments (ne x 2,
+ * int32 indices).  DFM_CURRENT is OrientedPolyLine (centres (p0 + p1) / 2, tangents
+ * p1 - p0), DFM_VARIFOLD is NonOrientedPolyLine (tangents scaled by |p1 - p0|^-1/2,
+ * upper-triangular T T^T weights). */
+
+/* Vertex -> corner incidence list of any cell list of `corners` (1..3) points per cell
+ * (host pointers): offsets (nv + 1), incidences (corners * ncells) holding
+ * corners * cell + corner, grouped by vertex in increasing corner order.  corners = 2 gives
+ * the segment list dfm_curve_match_gradient gathers with, in place of the reference's
+ * serial scatter over segments (OrientedPolyLine.cxx:216-234, NonOrientedPolyLine.cxx:
+ * 215-251).  DFM_ERR_ARG for an index outside [0, nv). */
+int dfm_cell_incidence(const int *cells, long ncells, int corners, long nv, int *offsets,
+                       int *incidences);
+
+/* Workspace bytes for a polyline match (gradient = 0) or match gradient (gradient = 1)
+ * between a template of nes segments and a target of net segments in dimension dim. */
+size_t dfm_curve_workspace(int kind, int dim, long nes, long net, int precision, int gradient);
+
+/* *match (device, one double) = |S|^2 + |T|^2 - 2 <S, T> of the template S (vs, es) and the
+ * target T (vt, et).  Replaces OrientedPolyLine::ComputeMatch (OrientedPolyLine.cxx:
+ * 132-170) and NonOrientedPolyLine::ComputeMatch (NonOrientedPolyLine.cxx:129-170).
+ * DFM_ERR_DIM for dim outside {2, 3}; DFM_ERR_ARG for a segment index outside [0, nv). */
+int dfm_curve_match(void *stream, int kind, int dim, double sigma, int precision,
+                    const double *vs, long nvs, const int *es, long nes, const double *vt,
+                    long nvt, const int *et, long net, double *match, void *work,
+                    size_t work_bytes);
+
+/* grad (device, nvs x dim) = d match / d vs, with the template's segment incidence list
+ * from dfm_cell_incidence (corners = 2; device copies).  Replaces
+ * OrientedPolyLine::ComputeMatchGradient (OrientedPolyLine.cxx:177-238) and
+ * NonOrientedPolyLine::ComputeMatchGradient (NonOrientedPolyLine.cxx:177-255). */
+int dfm_curve_match_gradient(void *stream, int kind, int dim, double sigma, int precision,
+                             const double *vs, long nvs, const int *es, long nes,
+                             const int *offsets, const int *incidences, const double *vt,
+                             long nvt, const int *et, long net, double *grad, void *work,
+                             size_t work_bytes);
+
 /* Data term between two sets of oriented samples (centre c, area-weighted normal N; the
  * reference's m_Centers / m_Normals of a mesh, OrientedSurfaceMesh.cxx:298-326 and
  * NonOrientedSurfaceMesh.cxx:269-308, given directly): *match (device double) =

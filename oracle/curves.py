@@ -4,11 +4,11 @@ FP64 CPU restatement of Deformetrica's polyline data terms (2D or 3D segments) o
 the oracle ExactKernel:
 
   current   OrientedPolyLine: centres (p0 + p1) / 2 and tangents p1 - p0
-            (OrientedPolyLine.cxx:251-268), self norm (:271-289), ComputeMatch (:118-160),
-            ComputeMatchGradient (:163-226)
+            (OrientedPolyLine.cxx:249-272), self norm (:279-296), ComputeMatch (:132-170),
+            ComputeMatchGradient (:177-238)
   varifold  NonOrientedPolyLine: tangents scaled by |t|^-1/2 and the upper-triangular T T^T
-            weights (NonOrientedPolyLine.cxx:278-309), self norm (:312-333), special_product
-            (:358-371), ComputeMatch (:124-163), ComputeMatchGradient (:166-244)
+            weights (NonOrientedPolyLine.cxx:266-297), self norm (:304-324), special_product
+            (:358-370), ComputeMatch (:129-170), ComputeMatchGradient (:177-255)
 
 The loops follow the reference segment by segment.  Pinned by tests/test_curves.py: each
 match against a brute-force double sum over segment pairs, each gradient against central

@@ -64,8 +64,24 @@ def test_reference_skull_polyline():
     its varifold self-distance is zero and its norm positive."""
     from deformetrica_code_amd.io import read_vtk
     from oracle import curves as oc
-    sk = read_vtk(os.path.join(HERE, "golden", "functional", "skull_template.vtk"))
+    sk = read_vtk(os.path.join(HERE, "golden", "functional", "skulls", "data", "template.vtk"))
     V, E = sk.coordinates(2), sk.segments()
     assert E.shape == (144, 2)
     assert oc.varifold_norm2(V, E, 40.0) > 0
     assert abs(oc.varifold_match(V, E, V, E, 40.0)) <= 1e-9 * oc.varifold_norm2(V, E, 40.0)
+
+
+def test_segment_incidence():
+    """dfm_cell_incidence with two corners: offsets and endpoint ids grouped by point in
+    increasing endpoint order; an index outside [0, nv) is refused."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.curves import cell_incidence
+    E = np.array([[0, 1], [1, 2], [2, 0], [3, 1]], dtype=np.int32)
+    off, inc = cell_incidence(E, 5)
+    np.testing.assert_array_equal(off, [0, 2, 5, 7, 8, 8])
+    np.testing.assert_array_equal(inc, [0, 5, 1, 2, 7, 3, 4, 6])
+    for v in range(5):
+        for c in inc[off[v]:off[v + 1]]:
+            assert E.reshape(-1)[c] == v
+    with pytest.raises(_lib.DfmError, match="outside"):
+        cell_incidence(np.array([[0, 5]], dtype=np.int32), 5)
