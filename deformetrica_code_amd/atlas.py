@@ -185,6 +185,26 @@ def surface_bounding_box(vertices, faces):
     return np.stack([c.min(axis=0), c.max(axis=0)], axis=1)
 
 
+def curve_bounding_box(points, segments):
+    """Bounding box of a polyline as OrientedPolyLine / NonOrientedPolyLine keep it: over the
+    segment centres (OrientedPolyLine.cxx:299-321, NonOrientedPolyLine.cxx:327-351)."""
+    v = np.asarray(points, dtype=np.float64)
+    e = np.asarray(segments)
+    c = (v[e[:, 0]] + v[e[:, 1]]) / 2.0
+    return np.stack([c.min(axis=0), c.max(axis=0)], axis=1)
+
+
+def union_bounding_box(boxes):
+    """Bounding box of a multi-object template: the union of its objects' boxes
+    (DeformableMultiObject::UpdateBoundingBox, DeformableMultiObject.cxx:410-429)."""
+    boxes = [np.asarray(b, dtype=np.float64) for b in boxes]
+    out = boxes[0].copy()
+    for b in boxes[1:]:
+        out[:, 0] = np.minimum(out[:, 0], b[:, 0])
+        out[:, 1] = np.maximum(out[:, 1], b[:, 1])
+    return out
+
+
 def _copy_effects(effects):
     return {k: ([np.array(x, dtype=np.float64) for x in v] if isinstance(v, (list, tuple))
                 else np.array(v, dtype=np.float64)) for k, v in effects.items()}

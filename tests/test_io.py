@@ -14,6 +14,9 @@ snapshot.
   value within its 1e-8 plus 1e-13 relative.  Both need the meshes' triangles reoriented
   as the reference's reorient-normals (on by default) does; the control-point lattices and
   the initial data terms are checked on their own.
+* Functional, Test5 (the skulls registration, a varifold polyline) and Test8 (the parabola
+  atlas, two polyline currents): the oracle replays of 10 FastGradientAscent iterations
+  equal the stored snapshots within their 1e-7 / 1e-8 tolerances and within 1e-12.
 """
 import os
 import sys
@@ -305,3 +308,39 @@ def test_oracle_replay_of_the_banana_atlas_meets_the_reference_tolerance():
     np.testing.assert_array_equal(out["step"], ref["step"])
     assert states_close(fc.banana_state(out), fc.banana_state(ref), 1e-5)
     assert states_close(fc.banana_state(out), fc.banana_state(ref), 1e-10)
+
+
+@pytest.mark.parametrize("case,tolerance", [("skulls", 1e-7), ("parabola", 1e-8)])
+def test_oracle_replays_of_the_2d_polyline_tests(case, tolerance):
+    """configure.ini Test5 (skulls registration, NonOrientedPolyline) and Test8 (parabola
+    atlas, two OrientedPolyline objects): the oracle replays of 10 FastGradientAscent
+    iterations equal the stored snapshots -- the discrete path exactly, every value within
+    the tests' own tolerances and within 1e-12 plus 1e-13 of its magnitude.  The control
+    points start on the lattice over the segment-centre box (9 and 40 points)."""
+    from deformetrica_code_amd.io import DeformationState, push_fast_gradient_ascent, states_close
+    out = getattr(fc, case + "_oracle_replay")()
+    ref = fc.snapshot(getattr(fc, case.upper() + "_STATE"))
+    assert out["iter"] == ref["iter"] == 11
+    assert out["freeze_direction_counter"] == ref["freeze_direction_counter"]
+    assert out["tau"] == ref["tau"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    assert out["fixed_effects"]["ControlPoints"].shape == {"skulls": (9, 2), "parabola": (40, 2)}[case]
+    mine = push_fast_gradient_ascent(DeformationState(), out)
+    stored = push_fast_gradient_ascent(DeformationState(), ref)
+    assert states_close(mine, stored, tolerance)
+    assert states_close(mine, stored, 1e-12, rtol=1e-13)
+
+
+def test_parabola_template_smoothing_uses_the_default_ratio():
+    """The parabola XML asks for a smoothing ratio of 0.1, but the reference reads that ratio
+    under the key smoothing-kernel-with-ratio only, so the default ratio 1 applies: with
+    0.1 x 0.2 the replay's first update already misses the stored history by more than 1e-3."""
+    from deformetrica_code_amd.estimators import FastGradientAscent
+    template, objects, targets = fc.parabola()
+    fixed, ind = fc.multi_initial_effects(template, objects, 0.2, 3)
+    model = fc.OracleMultiObjectAtlasModel(objects, targets, fc.PARABOLA_PARAMS, fixed,
+                                           freeze_template=False, smoothing_width=0.02)
+    out = FastGradientAscent(model, **fc.PARABOLA_ESTIMATOR).Update({}, ind)
+    stored = fc.snapshot(fc.PARABOLA_STATE)["log_likelihood_history"]
+    assert out["log_likelihood_history"][0][0] == pytest.approx(stored[0][0], rel=1e-13)
+    assert abs(out["log_likelihood_history"][1][0] - stored[1][0]) > 1e-3 * abs(stored[1][0])
