@@ -292,3 +292,170 @@ class HIPAtlasModel:
         ind = {"Momenta": [g for g in gmom]}
         sq.append(float(np.sum(gmom * gmom)))
         return pop, ind, np.asarray(sq)
+
+
+class HIPMultiObjectAtlasModel:
+    """DeterministicAtlas of a template made of several landmark objects (a
+    DeformableMultiObject of polylines and triangle surfaces) as the estimators see it, with
+    every kernel sum on the GPU: each subject's shooting, flow and the reference's continuous
+    adjoint through HIPDiffeos (Diffeos::Update, ::IntegrateAdjointEquations), the data
+    terms through the objects' own ComputeMatch / ComputeMatchGradient (HIPOrientedPolyLine,
+    HIPNonOrientedPolyLine, HIPOrientedSurfaceMesh, HIPNonOrientedSurfaceMesh), the RKHS
+    regularity and the template smoothing through HIPExactKernel.
+
+    ``objects`` are the template's data-term objects (their kernel widths set),
+    ``data_sigma_squared`` one noise variance per object, ``targets`` one list of target
+    objects (same types, in object order) per subject.  Fixed effects "ControlPoints"
+    (n x d) and "TemplateData" (one point array per object); individual random effects
+    "Momenta" (one n x d per subject).
+
+    The terms follow the reference:
+    - log-likelihood [-1/2 sum_s sum_i r_si / data_sigma_squared_i, -1/2 sum_s a_s^T K a_s]
+      (DeterministicAtlas.cxx:100-151), r_si the match of object i of the deformed template
+      with subject s's object i;
+    - per subject, each object's match gradient divided by 2 data_sigma_squared_i, the
+      objects' points concatenated in object order for the flow and the adjoint
+      (AbstractAtlas::ComputeDataTermGradientSubject, AbstractAtlas.cxx:562-616), then the
+      regularity (DeterministicAtlas.cxx:236-262);
+    - SetFixedEffects' one-call lag of the template geometry, the freeze flags, the
+      per-object smoothing with the "TemplateData" fixed effect as sources
+      (AbstractAtlas.cxx:52-63, 727-751) and the squared norms [control points, <L2,
+      smoothed> summed over objects, momenta] (DeterministicAtlas.cxx:234-238), as in
+      HIPAtlasModel.
+    The bounding-box test of the reference's Diffeos is not made: out_of_box is always
+    False."""
+
+    def __init__(self, objects, data_sigma_squared, targets, fixed_effects, *,
+                 deformation_width, number_of_time_points, t0=0.0, tN=1.0,
+                 improved_euler=False, scheme="euler", precision="fp64",
+                 freeze_template=False, freeze_control_points=False, smoothing_width=0.0,
+                 checkpoint_every=5):
+        if len(data_sigma_squared) != len(objects):
+            raise ValueError("one data sigma squared per object")
+        if any(len(t) != len(objects) for t in targets):
+            raise ValueError("every subject needs one target per template object")
+        self.objects = list(objects)
+        self.data_sigma_squared = [float(s) for s in data_sigma_squared]
+        self.targets = [list(t) for t in targets]
+        self.deformation_width = float(deformation_width)
+        self.T = int(number_of_time_points)
+        self.t0, self.tN = float(t0), float(tN)
+        self.improved_euler = bool(improved_euler)
+        self.scheme = scheme
+        self.precision = precision
+        self.freeze_template = bool(freeze_template)
+        self.freeze_control_points = bool(freeze_control_points)
+        self.smoothing_width = float(smoothing_width)
+        self.checkpoint_every = int(checkpoint_every)
+        self._fixed = _copy_effects(fixed_effects)
+        self._geometry = [np.array(v, dtype=np.float64) for v in self._fixed["TemplateData"]]
+        self._counts = [o.GetNumberOfPoints() for o in self.objects]
+        if [len(v) for v in self._geometry] != self._counts:
+            raise ValueError("TemplateData must hold each object's points")
+        self.dim = self._fixed["ControlPoints"].shape[1]
+
+    def GetFixedEffects(self):
+        return _copy_effects(self._fixed)
+
+    def SetFixedEffects(self, fixed):
+        template_memory = self._fixed["TemplateData"]
+        cp_memory = self._fixed["ControlPoints"]
+        self._fixed = _copy_effects(fixed)
+        if self.freeze_template:
+            self._fixed["TemplateData"] = template_memory
+        else:
+            self._geometry = [np.array(v, dtype=np.float64) for v in template_memory]
+        if self.freeze_control_points:
+            self._fixed["ControlPoints"] = cp_memory
+
+    def _diffeos(self, a):
+        from .diffeos import HIPDiffeos
+        d = HIPDiffeos(self.dim, self.precision)
+        d.SetKernelWidth(self.deformation_width)
+        d.SetNumberOfTimePoints(self.T)
+        d.SetT0(self.t0)
+        d.SetTN(self.tN)
+        d.SetShootingScheme(self.scheme)
+        if self.improved_euler:
+            d.UseImprovedEuler()
+        else:
+            d.UseStandardEuler()
+        d.SetStartPositions(self._fixed["ControlPoints"])
+        d.SetStartMomentas(np.asarray(a, dtype=np.float64))
+        d.SetLandmarkPoints(np.concatenate(self._geometry))
+        d.Update()
+        return d
+
+    def _deformed(self, d):
+        parts = np.split(d.GetDeformedLandmarkPoints(), np.cumsum(self._counts)[:-1])
+        return [o.DeformedObject(x) for o, x in zip(self.objects, parts)]
+
+    def _regularity_kernel(self):
+        from .kernel import HIPExactKernel
+        k = HIPExactKernel(self.dim, self.deformation_width, precision=self.precision)
+        k.SetSources(self._fixed["ControlPoints"])
+        return k
+
+    def UpdateFixedEffectsAndComputeCompleteLogLikelihood(self, pop_rer, ind_rer):
+        momenta = ind_rer["Momenta"]
+        if len(momenta) != len(self.targets):
+            raise RuntimeError("Number of momenta matrices and residuals mismatch in"
+                               " DeterministicAtlas::ComputeLogLikelihood")
+        data = 0.0
+        for a, target in zip(momenta, self.targets):
+            deformed = self._deformed(self._diffeos(a))
+            for obj, tgt, s2 in zip(deformed, target, self.data_sigma_squared):
+                data -= obj.ComputeMatch(tgt) / s2
+        reg = 0.0
+        k = self._regularity_kernel()
+        cp = self._fixed["ControlPoints"]
+        for a in momenta:
+            k.SetWeights(a)
+            reg -= float(np.sum(k.Convolve(cp) * np.asarray(a)))
+        return np.array([0.5 * data, 0.5 * reg]), False
+
+    def _smooth(self, gl2):
+        if self.smoothing_width < 1e-20:
+            return gl2
+        from .kernel import HIPExactKernel
+        out = []
+        for tpl, g in zip(self._fixed["TemplateData"], gl2):
+            k = HIPExactKernel(self.dim, self.smoothing_width, precision=self.precision)
+            k.SetSources(tpl)
+            k.SetWeights(g)
+            out.append(k.Convolve(tpl))
+        return out
+
+    def ComputeCompleteLogLikelihoodGradient(self, pop_rer, ind_rer):
+        momenta = ind_rer["Momenta"]
+        if len(momenta) != len(self.targets):
+            raise RuntimeError("Number of Momentas and target multi-object mismatch in"
+                               " DeterministicAtlas::ComputeFunctionalGradient")
+        cp = self._fixed["ControlPoints"]
+        gcp = np.zeros_like(cp)
+        gmom = []
+        gtpl = np.zeros((sum(self._counts), self.dim))
+        for a, target in zip(momenta, self.targets):
+            d = self._diffeos(a)
+            theta = np.concatenate([obj.ComputeMatchGradient(tgt) / (2.0 * s2) for obj, tgt, s2
+                                    in zip(self._deformed(d), target, self.data_sigma_squared)])
+            xp, xm, th0 = d.IntegrateAdjointEquations(theta, self.checkpoint_every)
+            gcp -= xp
+            gmom.append(-xm)
+            gtpl -= th0
+        k = self._regularity_kernel()
+        for s, a in enumerate(momenta):
+            k.SetWeights(a)
+            gmom[s] -= k.Convolve(cp)
+            gcp -= k.ConvolveGradient(cp, np.asarray(a, dtype=np.float64))
+        pop, sq = {}, []
+        if not self.freeze_control_points:
+            pop["ControlPoints"] = gcp
+            sq.append(float(np.sum(gcp * gcp)))
+        if not self.freeze_template:
+            gl2 = np.split(gtpl, np.cumsum(self._counts)[:-1])
+            sob = self._smooth(gl2)
+            pop["TemplateData"] = sob
+            sq.append(sum(float(np.sum(a * b)) for a, b in zip(gl2, sob)))
+        sq.append(float(sum(np.sum(g * g) for g in gmom)))
+        return pop, {"Momenta": gmom}, np.asarray(sq)

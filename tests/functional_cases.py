@@ -114,6 +114,13 @@ def parabola():
     return [v for v, _ in tpl], [("current", e, 0.2, 1.0) for _, e in tpl], targets
 
 
+def amygdala_multi():
+    """The amygdala registration in the multi-object form: ([template], [object], targets)."""
+    V0, F, targets = amygdala()
+    p = AMYG_PARAMS
+    return [V0], [(p["kind"], F, p["attachment_width"], p["data_sigma_squared"])], [[t] for t in targets]
+
+
 def multi_initial_effects(template, objects, spacing, nsubjects):
     """Control points on the lattice over the union of the objects' bounding boxes, the
     template points, zero momenta."""

@@ -11,7 +11,12 @@ FP64:
   value within the reference's 1e-8 plus 1e-13 of its magnitude;
 * the GPU log-likelihood at the stored amygdala state: the regularity term to 1e-12 of the
   stored one, the data term of the undeformed template to 1e-12 of the stored one;
-* FP32C kernel sums take the same optimizer path as FP64.
+* FP32C kernel sums take the same optimizer path as FP64;
+* the skulls registration (Test5, a varifold polyline) and the parabola atlas (Test8, two
+  polyline currents) through the multi-object model (HIPMultiObjectAtlasModel): within
+  1e-9 relative of the oracle replays, the stored snapshots' discrete paths exactly and
+  every value within the tests' tolerances; the amygdala registration through the same
+  model agrees with the batched atlas's replay.
 """
 import numpy as np
 import pytest
@@ -145,3 +150,87 @@ def test_gpu_replays_in_fp32c_take_the_same_paths():
                  np.concatenate(ref["individual_rer"]["Momenta"]))
         print(f"{name} fp32c vs fp64: history {d:.3e}, momenta {dm:.3e}")
         assert d < 1e-5 and dm < 1e-5
+
+
+def _multi_gpu(case, precision="fp64"):
+    """FastGradientAscent over HIPMultiObjectAtlasModel for the skulls, parabola or (as a
+    one-surface multi-object) amygdala test."""
+    from deformetrica_code_amd.atlas import HIPMultiObjectAtlasModel
+    from deformetrica_code_amd.curves import HIPNonOrientedPolyLine, HIPOrientedPolyLine
+    from deformetrica_code_amd.estimators import FastGradientAscent
+    from deformetrica_code_amd.surfaces import HIPNonOrientedSurfaceMesh, HIPOrientedSurfaceMesh
+    if case == "amygdala":
+        template, objects, targets = fc.amygdala_multi()
+        p = fc.AMYG_PARAMS
+        spacing, estimator, smoothing = 10.0, fc.AMYG_ESTIMATOR, 0.0
+    else:
+        template, objects, targets = getattr(fc, case)()
+        p = getattr(fc, case.upper() + "_PARAMS")
+        spacing, estimator = p["deformation_width"], getattr(fc, case.upper() + "_ESTIMATOR")
+        smoothing = fc.PARABOLA_SMOOTHING if case == "parabola" else 0.0
+    fixed, ind = fc.multi_initial_effects(template, objects, spacing, len(targets))
+
+    def make(kind, points, cells, width):
+        if np.asarray(cells).shape[1] == 3:
+            cls = HIPOrientedSurfaceMesh if kind == "current" else HIPNonOrientedSurfaceMesh
+        else:
+            cls = HIPOrientedPolyLine if kind == "current" else HIPNonOrientedPolyLine
+        return cls(points, cells, width, precision)
+
+    tpl = [make(k, v, c, w) for v, (k, c, w, _) in zip(template, objects)]
+    tg = [[make(k, v, c, w) for (v, c), (k, _, w, _) in zip(t, objects)] for t in targets]
+    model = HIPMultiObjectAtlasModel(tpl, [o[3] for o in objects], tg, fixed,
+                                     deformation_width=p["deformation_width"],
+                                     number_of_time_points=p["T"],
+                                     improved_euler=p["improved_euler"], precision=precision,
+                                     freeze_template=case != "parabola",
+                                     smoothing_width=smoothing, checkpoint_every=3)
+    return FastGradientAscent(model, **estimator).Update({}, ind)
+
+
+@pytest.mark.parametrize("case,tolerance", [("skulls", 1e-7), ("parabola", 1e-8)])
+def test_gpu_replays_of_the_2d_polyline_tests(case, tolerance):
+    from deformetrica_code_amd.io import DeformationState, push_fast_gradient_ascent, states_close
+    print(f"GPU FastGradientAscent replay of {case}, fp64, multi-object model", flush=True)
+    out = _multi_gpu(case)
+    ref_oracle = getattr(fc, case + "_oracle_replay")()
+    for k in ("iter", "freeze_direction_counter", "tau"):
+        assert out[k] == ref_oracle[k]
+    np.testing.assert_array_equal(out["step"], ref_oracle["step"])
+    worst = _worst_difference(out, ref_oracle)
+    print(f"largest relative difference GPU vs oracle replay: {worst:.3e}")
+    assert worst < 1e-9
+    ref = fc.snapshot(getattr(fc, case.upper() + "_STATE"))
+    assert out["iter"] == ref["iter"] and out["tau"] == ref["tau"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    mine = push_fast_gradient_ascent(DeformationState(), out)
+    stored = push_fast_gradient_ascent(DeformationState(), ref)
+    assert states_close(mine, stored, tolerance)
+    assert states_close(mine, stored, 1e-10, rtol=1e-12)
+
+
+def test_multi_object_model_agrees_with_the_batched_atlas():
+    """The amygdala registration through HIPMultiObjectAtlasModel (one surface object,
+    HIPDiffeos per subject) and through the batched C-ABI atlas: the same path, values
+    within 1e-9 relative."""
+    print("GPU amygdala replay through the multi-object model", flush=True)
+    out, ref = _multi_gpu("amygdala"), _amygdala_gpu()
+    assert out["iter"] == ref["iter"] and out["tau"] == ref["tau"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    worst = _worst_difference(out, ref)
+    print(f"largest relative difference multi-object vs batched atlas: {worst:.3e}")
+    assert worst < 1e-9
+
+
+@pytest.mark.parametrize("case", ["skulls", "parabola"])
+def test_gpu_2d_replays_in_fp32c_take_the_same_paths(case):
+    print(f"GPU replay {case}, fp32c", flush=True)
+    out, ref = _multi_gpu(case, "fp32c"), _multi_gpu(case, "fp64")
+    assert out["iter"] == ref["iter"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    d = max(rel(a, b) for a, b in zip(out["log_likelihood_history"],
+                                      ref["log_likelihood_history"]) if b.size)
+    dm = rel(np.concatenate(out["individual_rer"]["Momenta"]),
+             np.concatenate(ref["individual_rer"]["Momenta"]))
+    print(f"{case} fp32c vs fp64: history {d:.3e}, momenta {dm:.3e}")
+    assert d < 1e-5 and dm < 1e-5
