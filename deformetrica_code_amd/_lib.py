@@ -72,8 +72,10 @@ def load():
     L.dfm_shoot_flow_adjoint.argtypes = [vp, i, d, i, i, i, vp, vp, l, vp, l, i, d, d, i, vp, vp,
                                          vp, vp, vp, sz]
     L.dfm_shoot_flow_adjoint_equations.argtypes = L.dfm_shoot_flow_adjoint.argtypes
+    L.dfm_shoot_flow_adjoint_jumps.argtypes = [vp, i, d, i, i, i, vp, vp, l, vp, l, i, d, d, i,
+                                               i, vp, vp, vp, vp, vp, vp, sz]
     for name in ("dfm_shoot", "dfm_flow", "dfm_shoot_vjp", "dfm_shoot_flow_adjoint",
-                 "dfm_shoot_flow_adjoint_equations"):
+                 "dfm_shoot_flow_adjoint_equations", "dfm_shoot_flow_adjoint_jumps"):
         getattr(L, name).restype = i
     # ragged batches and graphs
     L.dfm_batch_workspace.argtypes = [i, i, i, i, vp, vp, i]

@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "dfm_hip.h"
 #include "dfm_internal.h"
@@ -476,17 +477,22 @@ int continuous_rate(const Ctx &cx, AdjointLayout &L, const Seg &cs, const Seg &l
 
 // One backward step of the continuous adjoint equations from state t+1 (z1) to state t
 // (zt): theta in L.th and xi in (L.lp, L.la) move from t+1 to t (AEI.cxx:96-133, 438-462).
+// A jump condition src (m x d, or null) is added to both rates of theta, as the reference
+// adds a regression observation's condition (AEI.cxx:104-110, 123-129).
 int continuous_step(const Ctx &cx, AdjointLayout &L, const Seg &cs, const Seg &ls, int improved,
                     double dt, const double *pt, const double *at, const double *xt,
-                    const double *p1, const double *a1, const double *x1) {
+                    const double *p1, const double *a1, const double *x1,
+                    const double *src = nullptr) {
   const long nd = cs.total * cx.D, md = ls.total * cx.D;
   int rc;
   if (md > 0) {
     // theta_t = theta_{t+1} + dt Grad1(x_{t+1}, theta_{t+1}; z_{t+1})  [+ Heun corrector]
     if ((rc = conv_grad1(cx, x1, L.th, ls, p1, a1, cs, nullptr, L.ths))) return rc;
+    if (src && (rc = lin(cx, L.ths, L.ths, 1.0, src, 1.0, nullptr, 0.0, md))) return rc;
     if ((rc = lin(cx, L.thn, L.th, 1.0, L.ths, dt, nullptr, 0.0, md))) return rc;
     if (improved) {
       if ((rc = conv_grad1(cx, xt, L.thn, ls, pt, at, cs, nullptr, L.mu))) return rc;
+      if (src && (rc = lin(cx, L.mu, L.mu, 1.0, src, 1.0, nullptr, 0.0, md))) return rc;
       if ((rc = lin(cx, L.thn, L.th, 1.0, L.ths, 0.5 * dt, L.mu, 0.5 * dt, md))) return rc;
     }
   }
@@ -506,18 +512,52 @@ int continuous_step(const Ctx &cx, AdjointLayout &L, const Seg &cs, const Seg &l
   return DFM_OK;
 }
 
+// Jump conditions of the continuous adjoint (AEI.cxx:59-136): theta starts from the
+// condition of the last jump if that jump is at T-1 (else from zero), and the condition at
+// jump time t is added to theta's rate over the step t -> t-1.  The reference walks its
+// list index down only in the improved-Euler branch, so under standard Euler at most one
+// jump after the terminal one is applied; one condition alone is no jump (it starts theta
+// whatever its time), and a jump at time 0 is never reached.  terminal = -1 for zero;
+// step_src[t] = condition index of step t -> t-1, or -1.
+struct JumpPlan {
+  int terminal = 0;
+  std::vector<int> step_src;
+};
+
+JumpPlan plan_jumps(int njumps, const int *times, int T, int improved_euler) {
+  JumpPlan jp;
+  jp.step_src.assign((size_t)std::max(T, 1), -1);
+  if (njumps <= 1) return jp;
+  int subj = njumps - 1;
+  jp.terminal = -1;
+  if (times[subj] == T - 1) {
+    jp.terminal = subj;
+    subj = std::max(subj - 1, 0);
+  }
+  for (int t = T - 1; t > 0; t--)
+    if (t == times[subj]) {
+      jp.step_src[t] = subj;
+      if (improved_euler) subj = std::max(subj - 1, 0);
+    }
+  return jp;
+}
+
 // phase 1: forward pass only (checkpoints and the final landmarks, *x_final, stay in the
 // workspace); phase 2: backward pass from the checkpoints of an earlier phase 1 with the
-// same arguments; phase 3: both.
+// same arguments; phase 3: both.  jumps: the continuous form's jump plan over the
+// conditions in theta_T (njumps x m x d), or null for one condition at T-1.
 int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int improved_euler,
                  int nseg, const long *cp_off, const long *lm_off, const double *p0,
                  const double *a0, long n_, const double *x0, long m_, int T, double t0,
                  double tN, int checkpoint_every, const double *theta_T, double *grad_p0,
                  double *grad_a0, double *grad_x0, void *work, size_t work_bytes, int phase = 3,
-                 const double **x_final = nullptr, int form = DFM_ADJOINT_DISCRETE) {
+                 const double **x_final = nullptr, int form = DFM_ADJOINT_DISCRETE,
+                 const JumpPlan *jumps = nullptr) {
   clear_error();
   if (form != DFM_ADJOINT_DISCRETE && form != DFM_ADJOINT_CONTINUOUS)
     return set_error(DFM_ERR_ARG, "unknown adjoint form %d", form);
+  if (jumps && form != DFM_ADJOINT_CONTINUOUS)
+    return set_error(DFM_ERR_ARG, "jump conditions need the continuous adjoint");
   if (nseg > 0 && !lm_off) return set_error(DFM_ERR_ARG, "landmark offsets are NULL");
   Seg cs, ls;
   int rc = segs_of(nseg, cp_off, lm_off, n_, m_, cs, ls);
@@ -566,7 +606,15 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
   // terminal cotangents
   if ((rc = lin(cx, L.lp, p0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
   if ((rc = lin(cx, L.la, a0, 0.0, nullptr, 0.0, nullptr, 0.0, nd))) return rc;
-  if (m > 0 && (rc = copy(cx, L.th, theta_T, md))) return rc;
+  if (m > 0) {
+    if (!jumps)
+      rc = copy(cx, L.th, theta_T, md);
+    else if (jumps->terminal >= 0)
+      rc = copy(cx, L.th, theta_T + (size_t)jumps->terminal * md, md);
+    else
+      rc = fill(cx.st, L.th, md, 0.0);
+    if (rc) return rc;
+  }
 
   // ---- backward, one checkpoint segment at a time
   for (int s = L.segs - 1; s >= 0; s--) {
@@ -584,8 +632,10 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
     for (int t = te - 1; t >= tb; t--) {
       double *zt = S + (size_t)(t - tb) * rec, *z1 = S + (size_t)(t - tb + 1) * rec;
       if (form == DFM_ADJOINT_CONTINUOUS) {
+        const int j = jumps ? jumps->step_src[t + 1] : -1;
+        const double *src = j >= 0 && m > 0 ? theta_T + (size_t)j * md : nullptr;
         if ((rc = continuous_step(cx, L, cs, ls, improved_euler, dt, P_(zt), A_(zt), X_(zt),
-                                  P_(z1), A_(z1), X_(z1))))
+                                  P_(z1), A_(z1), X_(z1), src)))
           return rc;
         continue;
       }
@@ -742,6 +792,24 @@ int dfm_shoot_flow_adjoint_equations(void *stream, int D, double sigma, int prec
   return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, 0, nullptr, nullptr, p0, a0,
                       n, x0, m, T, t0, tN, checkpoint_every, theta_T, grad_p0, grad_a0, grad_x0,
                       work, work_bytes, 3, nullptr, DFM_ADJOINT_CONTINUOUS);
+}
+
+int dfm_shoot_flow_adjoint_jumps(void *stream, int D, double sigma, int prec, int scheme,
+                                 int improved_euler, const double *p0, const double *a0, long n,
+                                 const double *x0, long m, int T, double t0, double tN,
+                                 int checkpoint_every, int njumps, const int *jump_times,
+                                 const double *conditions, double *grad_p0, double *grad_a0,
+                                 double *grad_x0, void *work, size_t work_bytes) {
+  clear_error();
+  if (njumps < 1 || !jump_times)
+    return set_error(DFM_ERR_ARG, "at least one jump time is needed");
+  for (int j = 0; j < njumps; j++)
+    if (jump_times[j] < 0 || jump_times[j] >= T)
+      return set_error(DFM_ERR_ARG, "jump time %d outside [0, %d)", jump_times[j], T);
+  const JumpPlan plan = plan_jumps(njumps, jump_times, T, improved_euler);
+  return adjoint_impl(stream, D, sigma, prec, scheme, improved_euler, 0, nullptr, nullptr, p0, a0,
+                      n, x0, m, T, t0, tN, checkpoint_every, conditions, grad_p0, grad_a0,
+                      grad_x0, work, work_bytes, 3, nullptr, DFM_ADJOINT_CONTINUOUS, &plan);
 }
 
 // ---- batched subjects -----------------------------------------------------------------

@@ -131,14 +131,25 @@ class HIPDiffeos:
         """(dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_T: the exact discrete adjoint."""
         return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint")
 
-    def IntegrateAdjointEquations(self, theta_T, checkpoint_every: int = 5):
-        """Diffeos::IntegrateAdjointEquations (Diffeos.cxx:1390-1443) for the landmark
+    def IntegrateAdjointEquations(self, theta_T, checkpoint_every: int = 5, jump_times=None):
+        """Diffeos::IntegrateAdjointEquations (Diffeos.cxx:1374-1443) for the landmark
         points: (xi_pos(0), xi_mom(0), theta(0)) of the reference's continuous adjoint
         equations for the terminal landmark cotangent theta_T; the reference's
-        GetAdjointPosAt0 / GetAdjointMomAt0 are the first two."""
-        return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint_equations")
+        GetAdjointPosAt0 / GetAdjointMomAt0 are the first two.  With ``jump_times`` (the
+        geodesic regression's time indices), theta_T is the list of landmark conditions at
+        those indices, applied as the reference's AdjointEquationsIntegrator applies them
+        (dfm_shoot_flow_adjoint_jumps)."""
+        if jump_times is None:
+            return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint_equations")
+        times = np.ascontiguousarray(jump_times, dtype=np.int32)
+        conditions = np.ascontiguousarray(np.stack([np.asarray(c, dtype=np.float64)
+                                                    for c in theta_T]))
+        if conditions.shape[0] != times.shape[0]:
+            raise ValueError("one condition per jump time")
+        return self._adjoint(conditions, checkpoint_every, "dfm_shoot_flow_adjoint_jumps",
+                             times)
 
-    def _adjoint(self, theta_T, checkpoint_every, entry):
+    def _adjoint(self, theta_T, checkpoint_every, entry, jump_times=None):
         L = _lib.load()
         n, d, T = self.p0.shape[0], self.dim, self.T
         m = self.x0.shape[0]
@@ -150,10 +161,12 @@ class HIPDiffeos:
         gp, ga, gx = DeviceArray((n, d)), DeviceArray((n, d)), DeviceArray((m, d))
         wp, wb = self._ws.get(L.dfm_adjoint_workspace(d, n, m, T, int(checkpoint_every),
                                                       self.precision))
-        _lib.check(getattr(L, entry)(st.handle, d, self.kernel_width, self.precision,
-                                     self.scheme, int(self.improved_euler), p0.ptr, a0.ptr, n,
-                                     x0.ptr, m, T, self.t0, self.tN, int(checkpoint_every),
-                                     th.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
+        args = [st.handle, d, self.kernel_width, self.precision, self.scheme,
+                int(self.improved_euler), p0.ptr, a0.ptr, n, x0.ptr, m, T, self.t0, self.tN,
+                int(checkpoint_every)]
+        if jump_times is not None:
+            args += [len(jump_times), jump_times.ctypes.data_as(ctypes.c_void_p)]
+        _lib.check(getattr(L, entry)(*args, th.ptr, gp.ptr, ga.ptr, gx.ptr, wp, wb))
         return gp.numpy(st), ga.numpy(st), gx.numpy(st)
 
 

@@ -258,6 +258,26 @@ int dfm_shoot_flow_adjoint_equations(void *stream, int dimpoint, double sigma, i
                                      const double *theta_T, double *grad_p0, double *grad_a0,
                                      double *grad_x0, void *work, size_t work_bytes);
 
+/* The continuous adjoint with landmark conditions at several time indices: the reference's
+ * Diffeos::IntegrateAdjointEquations with a list and jump times (Diffeos.cxx:1393-1443,
+ * AdjointEquationsIntegrator.cxx:59-136), the backward pass of the geodesic regression
+ * (Regression.cxx:600-662).  conditions (device, njumps x m x d) are the observations'
+ * match gradients in the order of jump_times (host, njumps indices in [0, T)).  As in the
+ * reference: theta starts from the last condition if its time is T-1 (else from zero);
+ * the condition at jump time t is added to theta's rate over the step t -> t-1 (both
+ * stages under improved Euler); the reference's list index moves to the next jump only
+ * under improved Euler, so under standard Euler at most one jump after the terminal one
+ * is applied; a single condition is no jump and starts theta at T-1 whatever its time; a
+ * jump at time 0 is never reached.  Outputs and workspace (dfm_adjoint_workspace) as
+ * dfm_shoot_flow_adjoint_equations. */
+int dfm_shoot_flow_adjoint_jumps(void *stream, int dimpoint, double sigma, int precision,
+                                 int scheme, int improved_euler, const double *p0,
+                                 const double *a0, long n, const double *x0, long m, int T,
+                                 double t0, double tN, int checkpoint_every, int njumps,
+                                 const int *jump_times, const double *conditions,
+                                 double *grad_p0, double *grad_a0, double *grad_x0, void *work,
+                                 size_t work_bytes);
+
 /* ---- ragged batches of subjects (device pointers, host offset arrays) -------------------
  * A batch of nseg subjects (1 <= nseg <= 65535) stores each per-point array concatenated
  * over subjects: subject s owns rows [off[s], off[s + 1]) where off is a host array of

@@ -210,20 +210,57 @@ def reference_update(P_s, A_s, X_s, theta_s, xp_s, xm_s, width):
     return dxi1 + dxi3 + dxi4, dxi2 + dxi5 + dxi6
 
 
-def adjoint_reference(P, A, X, theta_T, width, T, t0=0.0, tN=1.0, improved_euler=True):
+def jump_schedule(jump_times, T, improved_euler):
+    """Where AdjointEquationsIntegrator::IntegrateAdjointOfLandmarkPointsEquations
+    (AEI.cxx:59-136) uses the conditions of a list with jump times: (index of the condition
+    that starts theta at T-1, or None for zero; {step t: condition index}), the condition
+    of step t being added to the rate of theta over t -> t-1.  Its index walks down the
+    list only in the improved-Euler branch, so under standard Euler at most one jump after
+    the terminal one is applied; a list of one condition has no jumps and starts theta
+    whatever its time; a jump at time 0 is never reached."""
+    if jump_times is None or len(jump_times) <= 1:
+        return 0, {}
+    subj = len(jump_times) - 1
+    terminal = None
+    if jump_times[subj] == T - 1:
+        terminal = subj
+        subj = max(subj - 1, 0)
+    steps = {}
+    for t in range(T - 1, 0, -1):
+        if t == jump_times[subj]:
+            steps[t] = subj
+            if improved_euler:
+                subj = max(subj - 1, 0)
+    return terminal, steps
+
+
+def adjoint_reference(P, A, X, theta_T, width, T, t0=0.0, tN=1.0, improved_euler=True,
+                      jump_times=None):
     """AdjointEquationsIntegrator with landmark points only (AEI.cxx:59-136, 414-567):
     returns (xi_pos_0, xi_mom_0, theta_0), the reference's gradient of E(x_T) w.r.t. the
-    initial control points, momenta and landmarks for dE/dx_T = theta_T."""
+    initial control points, momenta and landmarks for dE/dx_T = theta_T.  With jump_times
+    (Diffeos::IntegrateAdjointEquations with a list, Diffeos.cxx:1393-1443), theta_T is
+    the list of conditions at those time indices, used as ``jump_schedule`` says."""
     d = P[0].shape[1]
     n = P[0].shape[0]
     dt = time_step(T, t0, tN)
+    if jump_times is None:
+        conditions = [np.asarray(theta_T, dtype=np.float64)]
+    else:
+        conditions = [np.asarray(c, dtype=np.float64) for c in theta_T]
+    terminal, steps = jump_schedule(jump_times, T, improved_euler)
     theta = [None] * T
-    theta[T - 1] = np.asarray(theta_T, dtype=np.float64).copy()
+    theta[T - 1] = (conditions[terminal].copy() if terminal is not None
+                    else np.zeros_like(conditions[0]))
     for t in range(T - 1, 0, -1):  # AEI.cxx:96-133
         dth = reference_theta_rate(P[t], A[t], X[t], theta[t], width)
+        if t in steps:
+            dth = dth + conditions[steps[t]]
         theta[t - 1] = theta[t] + dth * dt
         if improved_euler:
             dth2 = reference_theta_rate(P[t - 1], A[t - 1], X[t - 1], theta[t - 1], width)
+            if t in steps:
+                dth2 = dth2 + conditions[steps[t]]
             theta[t - 1] = theta[t] + (dth + dth2) * (dt * 0.5)
     xp = [np.zeros((n, d)) for _ in range(T)]
     xm = [np.zeros((n, d)) for _ in range(T)]

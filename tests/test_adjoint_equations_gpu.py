@@ -118,3 +118,51 @@ def test_atlas_continuous_adjoint_matches_reference_gradient(kind):
     assert rel(gcp, rcp) < 1e-10
     assert rel(gmom, rmom) < 1e-10
     assert rel(gtpl, rtpl) < 1e-10
+
+
+@pytest.mark.parametrize("improved,T,k,times", [(False, 5, 2, [0, 2, 4]), (True, 5, 2, [0, 2, 4]),
+                                                (False, 8, 3, [1, 3, 5, 7]),
+                                                (True, 8, 3, [1, 3, 5, 7]),
+                                                (True, 8, 8, [2, 6]), (False, 6, 1, [3]),
+                                                (True, 7, 2, [1, 1, 6])])
+def test_adjoint_with_jump_conditions_matches_reference_restatement(improved, T, k, times):
+    """dfm_shoot_flow_adjoint_jumps (the geodesic regression's backward pass) against the
+    oracle's restatement of the reference's jump walk, FP64 at 1e-10: terminal jump or not,
+    standard and improved Euler (which walk the list differently), one condition alone,
+    a repeated time index, checkpoint intervals crossing the jumps."""
+    from oracle import deformations as dfo
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    width = 0.7
+    p0, a0, x0, _ = _case(T * 7 + k)
+    rng = np.random.default_rng(len(times) + T)
+    conditions = [rng.normal(size=x0.shape) for _ in times]
+    print(f"adjoint with jumps improved={improved} T={T} k={k} times={times}", flush=True)
+    D = HIPDiffeos(3)
+    D.SetKernelWidth(width)
+    D.SetNumberOfTimePoints(T)
+    D.SetTN(float(T - 1))
+    (D.UseImprovedEuler if improved else D.UseStandardEuler)()
+    D.SetStartPositions(p0)
+    D.SetStartMomentas(a0)
+    D.SetLandmarkPoints(x0)
+    gp, ga, gx = D.IntegrateAdjointEquations(conditions, checkpoint_every=k, jump_times=times)
+    P, A = dfo.shoot(p0, a0, width, T, tN=float(T - 1))
+    X = dfo.flow(P, A, x0, width, T, tN=float(T - 1), improved_euler=improved)
+    rp, ra, rx = dfo.adjoint_reference(P, A, X, conditions, width, T, tN=float(T - 1),
+                                       improved_euler=improved, jump_times=times)
+    assert rel(gp, rp) < 1e-10
+    assert rel(ga, ra) < 1e-10
+    assert rel(gx, rx) < 1e-10
+
+
+def test_jump_times_outside_the_grid_are_refused():
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.diffeos import HIPDiffeos
+    p0, a0, x0, theta = _case(3)
+    D = HIPDiffeos(3)
+    D.SetNumberOfTimePoints(5)
+    D.SetStartPositions(p0)
+    D.SetStartMomentas(a0)
+    D.SetLandmarkPoints(x0)
+    with pytest.raises(_lib.DfmError, match="jump time 5 outside"):
+        D.IntegrateAdjointEquations([theta, theta], jump_times=[2, 5])

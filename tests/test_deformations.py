@@ -180,3 +180,18 @@ def test_reference_adjoint_rates_are_jacobian_transposes():
     # theta rate: J_x v^T theta
     np.testing.assert_allclose(dfo.reference_theta_rate(P, A, X, th, w),
                                fd(lambda x: coupling(P, A, x), X), rtol=1e-6, atol=1e-7)
+
+
+def test_jump_schedule_walks_the_list_as_the_reference_loop():
+    """AdjointEquationsIntegrator's jump walk (AEI.cxx:69-131) traced by hand: the index
+    moves down the list only under improved Euler; a last jump off T-1 starts theta at
+    zero; one condition is no jump; time 0 is never reached."""
+    from oracle.deformations import jump_schedule
+    assert jump_schedule([0, 2, 4], 5, False) == (2, {2: 1})
+    assert jump_schedule([0, 2, 4], 5, True) == (2, {2: 1})
+    assert jump_schedule([1, 2, 3, 5], 6, False) == (3, {3: 2})
+    assert jump_schedule([1, 2, 3, 5], 6, True) == (3, {3: 2, 2: 1, 1: 0})
+    assert jump_schedule([1, 3], 6, False) == (None, {3: 1})
+    assert jump_schedule([1, 3], 6, True) == (None, {3: 1, 1: 0})
+    assert jump_schedule([2], 6, True) == (0, {})
+    assert jump_schedule(None, 6, True) == (0, {})

@@ -16,7 +16,10 @@ FP64:
   polyline currents) through the multi-object model (HIPMultiObjectAtlasModel): within
   1e-9 relative of the oracle replays, the stored snapshots' discrete paths exactly and
   every value within the tests' tolerances; the amygdala registration through the same
-  model agrees with the batched atlas's replay.
+  model agrees with the batched atlas's replay;
+* the skulls regression (Test7) through HIPRegressionModel, whose backward pass is the
+  continuous adjoint with the observations as jump conditions: within 1e-9 relative of the
+  oracle replay, the snapshot's path exactly, every value within the test's 1e-8.
 """
 import numpy as np
 import pytest
@@ -233,4 +236,56 @@ def test_gpu_2d_replays_in_fp32c_take_the_same_paths(case):
     dm = rel(np.concatenate(out["individual_rer"]["Momenta"]),
              np.concatenate(ref["individual_rer"]["Momenta"]))
     print(f"{case} fp32c vs fp64: history {d:.3e}, momenta {dm:.3e}")
+    assert d < 1e-5 and dm < 1e-5
+
+
+def _regression_gpu(precision="fp64"):
+    from deformetrica_code_amd.curves import HIPNonOrientedPolyLine
+    from deformetrica_code_amd.estimators import FastGradientAscent
+    from deformetrica_code_amd.regression import HIPRegressionModel
+    template, objects, obs, ti = fc.regression_skulls()
+    (kind, E, width, s2), = objects
+    assert kind == "varifold"
+    tpl = [HIPNonOrientedPolyLine(template[0], E, width, precision)]
+    targets = [[HIPNonOrientedPolyLine(v, c, width, precision)] for (v, c), in obs]
+    p = fc.REGRESSION_PARAMS
+    model = HIPRegressionModel(tpl, [s2], targets, ti, fc.regression_initial_effects(template, objects),
+                               deformation_width=p["deformation_width"], number_of_time_points=p["T"],
+                               t0=p["t0"], tN=p["tN"], improved_euler=p["improved_euler"],
+                               precision=precision, smoothing_width=fc.REGRESSION_SMOOTHING,
+                               checkpoint_every=2)
+    return FastGradientAscent(model, **fc.REGRESSION_ESTIMATOR).Update({}, {})
+
+
+def test_gpu_replay_of_the_skulls_regression():
+    """Test7 through HIPRegressionModel (the adjoint with jump conditions on the GPU): within
+    1e-9 relative of the oracle replay, the stored snapshot's path exactly, every value
+    within the test's 1e-8 and within 1e-10 plus 1e-12 of its magnitude."""
+    from deformetrica_code_amd.io import DeformationState, push_fast_gradient_ascent, states_close
+    print("GPU FastGradientAscent replay of the skulls regression, fp64", flush=True)
+    out = _regression_gpu()
+    ref_oracle = fc.regression_oracle_replay()
+    for k in ("iter", "freeze_direction_counter", "tau"):
+        assert out[k] == ref_oracle[k]
+    np.testing.assert_array_equal(out["step"], ref_oracle["step"])
+    worst = _worst_difference(out, ref_oracle)
+    print(f"largest relative difference GPU vs oracle replay: {worst:.3e}")
+    assert worst < 1e-9
+    ref = fc.snapshot(fc.REGRESSION_STATE)
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    mine = push_fast_gradient_ascent(DeformationState(), out)
+    stored = push_fast_gradient_ascent(DeformationState(), ref)
+    assert states_close(mine, stored, 1e-8)
+    assert states_close(mine, stored, 1e-10, rtol=1e-12)
+
+
+def test_gpu_regression_in_fp32c_takes_the_same_path():
+    print("GPU regression replay, fp32c", flush=True)
+    out, ref = _regression_gpu("fp32c"), _regression_gpu("fp64")
+    assert out["iter"] == ref["iter"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    d = max(rel(a, b) for a, b in zip(out["log_likelihood_history"],
+                                      ref["log_likelihood_history"]) if b.size)
+    dm = rel(out["fixed_effects"]["InitialMomenta"], ref["fixed_effects"]["InitialMomenta"])
+    print(f"regression fp32c vs fp64: history {d:.3e}, momenta {dm:.3e}")
     assert d < 1e-5 and dm < 1e-5

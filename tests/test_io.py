@@ -344,3 +344,25 @@ def test_parabola_template_smoothing_uses_the_default_ratio():
     stored = fc.snapshot(fc.PARABOLA_STATE)["log_likelihood_history"]
     assert out["log_likelihood_history"][0][0] == pytest.approx(stored[0][0], rel=1e-13)
     assert abs(out["log_likelihood_history"][1][0] - stored[1][0]) > 1e-3 * abs(stored[1][0])
+
+
+def test_oracle_replay_of_the_skulls_regression():
+    """configure.ini Test7 (geodesic regression of a varifold polyline through three skulls):
+    the oracle replay of 10 FastGradientAscent iterations equals the stored snapshot, the
+    discrete path exactly and every value within the test's 1e-8 and within 1e-12 plus 1e-13
+    of its magnitude.  It takes the reference's regression as it runs: the observation
+    conditions enter the adjoint as its jump walk applies them (the time-0 skull never
+    does), and the regularity gradient is absent (AddGradientRegularityTerm updates copies)."""
+    from deformetrica_code_amd.io import DeformationState, push_fast_gradient_ascent, states_close
+    out = fc.regression_oracle_replay()
+    ref = fc.snapshot(fc.REGRESSION_STATE)
+    assert out["iter"] == ref["iter"] == 11
+    assert out["freeze_direction_counter"] == ref["freeze_direction_counter"]
+    assert out["tau"] == ref["tau"]
+    np.testing.assert_array_equal(out["step"], ref["step"])
+    assert out["fixed_effects"]["ControlPoints"].shape == (16, 2)
+    mine = push_fast_gradient_ascent(DeformationState(), out)
+    stored = push_fast_gradient_ascent(DeformationState(), ref)
+    assert states_close(mine, stored, 1e-8)
+    assert states_close(mine, stored, 1e-12, rtol=1e-13)
+    assert mine.dumps().count("\n") == stored.dumps().count("\n")
