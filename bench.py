@@ -43,6 +43,12 @@ import numpy as np
 
 FLOPS_PER_PAIR = 29           # fused Convolve (Dv=3) + ConvolveGradient(X, alpha), d = 3
 FP32_PEAK = 157.3e12          # MI355X FP32 vector peak, FLOP/s
+# HBM bytes per launch of the headline k_pairsum dispatch at N = M = 1M: rocprofv3 --pmc
+# FETCH_SIZE (5.97 GB) and WRITE_SIZE (0.816 GB, the FP64 split partials), each scaled by
+# the same counters on a 24 MB device copy (profiles/k_pairsum_convgrad1_N1M_sigma0.05.md).
+# Counter runs cannot be made from inside this process, so the figure is the profiled one.
+PROFILED_TRAFFIC_BYTES = 5.97e9 + 0.816e9
+PROFILED_TRAFFIC_SOURCE = "profiles/k_pairsum_convgrad1_N1M_sigma0.05.md"
 METRIC = "kernel pair-evals/s (Convolve+Gradient), FP32-comp vs FP64, at ≤1e-6 rel err"
 
 
@@ -225,7 +231,9 @@ def main():
             "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK,
-            "traffic": None,
+            "traffic": PROFILED_TRAFFIC_BYTES if (N, M, sigma) == (1000000, 1000000, 0.05) else None,
+            "traffic_unit": "bytes per launch",
+            "traffic_source": PROFILED_TRAFFIC_SOURCE,
             "kernel": "k_pairsum<float, CONVGRAD1, 3, 3, 0, 1, ADAPTIVE> + k_reduce_splits",
             "kernel_ms": kern_s * 1e3,
             "flops_per_pair": FLOPS_PER_PAIR,
