@@ -1,4 +1,4 @@
-"""Deterministic atlas of surfaces on the GPU: the log-likelihood and gradient of
+"""Deterministic atlas of surfaces or polylines on the GPU: the log-likelihood and gradient of
 Deformetrica's DeterministicAtlas (DeterministicAtlas.cxx:100-281, AbstractAtlas.cxx:490-751)
 for many subjects at once, through libdfm_hip.so's dfm_atlas_* entry points.
 
@@ -30,6 +30,7 @@ class AtlasDesc(ctypes.Structure):
         ("nsubjects", ctypes.c_int), ("target_vertex_offsets", ctypes.c_void_p),
         ("target_face_offsets", ctypes.c_void_p), ("target_vertices", ctypes.c_void_p),
         ("target_faces", ctypes.c_void_p), ("adjoint", ctypes.c_int),
+        ("dimension", ctypes.c_int), ("cell_corners", ctypes.c_int),
     ]
 
 
@@ -40,7 +41,9 @@ def _offsets(counts):
 
 
 class HIPDeterministicAtlas:
-    """targets: list of (vertices (v_s x 3), faces (f_s x 3)) per subject."""
+    """targets: list of (vertices (v_s x d), cells (f_s x c)) per subject, with the template's
+    cells (triangles, c = 3, d = 3; or polyline segments, c = 2, d = 2 or 3); d is taken
+    from the targets' vertices."""
 
     def __init__(self, template_faces, n_template_vertices, n_control_points, targets, *,
                  kind="varifold", precision="fp64", deformation_width=1.0,
@@ -49,9 +52,12 @@ class HIPDeterministicAtlas:
                  checkpoint_every=5, adjoint="discrete"):
         L = _lib.load()
         tf = np.ascontiguousarray(template_faces, dtype=np.int32)
-        tv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1, 3)
+        corners = tf.shape[1]
+        dim = np.asarray(targets[0][0]).shape[1]
+        self.dim = dim
+        tv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1, dim)
                                                   for v, _ in targets]))
-        tfa = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32).reshape(-1, 3)
+        tfa = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32).reshape(-1, corners)
                                                    for _, f in targets]))
         vo = _offsets([len(v) for v, _ in targets])
         fo = _offsets([len(f) for _, f in targets])
@@ -77,6 +83,7 @@ class HIPDeterministicAtlas:
         d.target_vertex_offsets, d.target_face_offsets = vp(vo), vp(fo)
         d.target_vertices, d.target_faces = vp(tv), vp(tfa)
         d.adjoint = _lib.adjoint_code(adjoint)
+        d.dimension, d.cell_corners = int(dim), int(corners)
         self.desc = d
         self.S, self.n, self.m = len(targets), int(n_control_points), int(n_template_vertices)
         self._h = ctypes.c_void_p()
@@ -84,9 +91,9 @@ class HIPDeterministicAtlas:
         self.stream = Stream()
         self.loglik = DeviceArray((2,))
         self.residuals = DeviceArray((self.S,))
-        self.grad_cp = DeviceArray((self.n, 3))
-        self.grad_momenta = DeviceArray((self.S, self.n, 3))
-        self.grad_template = DeviceArray((self.m, 3))
+        self.grad_cp = DeviceArray((self.n, dim))
+        self.grad_momenta = DeviceArray((self.S, self.n, dim))
+        self.grad_template = DeviceArray((self.m, dim))
 
     def device_bytes(self):
         return int(_lib.load().dfm_atlas_device_bytes(self._h))
@@ -273,7 +280,7 @@ class HIPAtlasModel:
             return gl2
         from .kernel import HIPExactKernel
         tpl = np.asarray(self._fixed["TemplateData"][0], dtype=np.float64)
-        k = HIPExactKernel(3, self.smoothing_width, precision=self._precision)
+        k = HIPExactKernel(self.atlas.dim, self.smoothing_width, precision=self._precision)
         k.SetSources(tpl)
         k.SetWeights(gl2)
         return k.Convolve(tpl)

@@ -1,5 +1,6 @@
 // atlas.hip — the deterministic atlas log-likelihood and its gradient for a population of
-// surface subjects, every subject in the same launches.
+// subjects, every subject in the same launches: triangle surfaces in 3D, or polylines
+// (segments) in 2D or 3D.
 //
 // Model (DeterministicAtlas.cxx:100-151, 159-281; AbstractAtlas.cxx:490-620, 727-751):
 //   subject s: control points cp (shared), momenta a_s, template T0 flowed by the geodesic
@@ -19,7 +20,7 @@
 //
 // One gradient call: batched shoot + flow forward with checkpoints; data-term geometry of
 // all deformed templates; two fused Convolve + ConvolveGradient pair sums over all subjects
-// (template-template and template-target, targets ragged); per-face corner contributions
+// (template-template and template-target, targets ragged); per-cell corner contributions
 // and a deterministic per-vertex gather; the batched adjoint backward; the regularity pair
 // sum; deterministic reductions over subjects.  Everything is allocated at creation, so a
 // gradient call launches kernels only and can be captured in a HIP graph.
@@ -123,6 +124,7 @@ using namespace dfm;
 struct dfm_atlas {
   dfm_atlas_desc c{};
   int dv = 3;
+  int D = 3, corners = 3;  // dimension, points per cell (3 triangles, 2 segments)
   long F = 0, m = 0, n = 0;
   int S = 0;
   std::vector<long> cp_off, lm_off, face_off, tgt_off;
@@ -152,22 +154,22 @@ struct dfm_atlas {
       return p;
     };
     const long SF = (long)S * F, Sn = (long)S * n, Sm = (long)S * m;
-    L.p0r = take(Sn * 3); L.x0r = take(Sm * 3);
-    L.cS = take(SF * 3); L.wS = take(SF * dv); L.aS = take(SF * dv); L.nS = take(SF * 3);
-    L.k1 = take(SF * dv); L.k2 = take(SF * dv); L.g1 = take(SF * 3); L.g2 = take(SF * 3);
-    L.contrib = take(SF * 9); L.theta = take(Sm * 3);
-    L.gp = take(Sn * 3); L.ga = take(Sn * 3); L.gx = take(Sm * 3);
-    L.ka = take(Sn * 3); L.gr = take(Sn * 3);
-    L.S2 = take(S); L.ST = take(S); L.reg = take(S); L.bad = take(1); L.gl2 = take(m * 3);
-    L.adj_bytes = batch_adjoint_workspace(3, S, cp_off.data(), lm_off.data(), c.number_of_time_points,
+    L.p0r = take(Sn * D); L.x0r = take(Sm * D);
+    L.cS = take(SF * D); L.wS = take(SF * dv); L.aS = take(SF * dv); L.nS = take(SF * D);
+    L.k1 = take(SF * dv); L.k2 = take(SF * dv); L.g1 = take(SF * D); L.g2 = take(SF * D);
+    L.contrib = take(SF * corners * D); L.theta = take(Sm * D);
+    L.gp = take(Sn * D); L.ga = take(Sn * D); L.gx = take(Sm * D);
+    L.ka = take(Sn * D); L.gr = take(Sn * D);
+    L.S2 = take(S); L.ST = take(S); L.reg = take(S); L.bad = take(1); L.gl2 = take(m * D);
+    L.adj_bytes = batch_adjoint_workspace(D, S, cp_off.data(), lm_off.data(), c.number_of_time_points,
                                           c.checkpoint_every, c.precision);
     L.adj = base ? (void *)(base + off) : nullptr;
     off += a256(L.adj_bytes);
     size_t pw = 0;
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, 3, dv, face_seg, face_seg));
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, 3, dv, face_seg, tgt_seg));
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, 3, 3, cp_seg, cp_seg));
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONV, c.precision, 3, 3, Seg::single(m),
+    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, dv, face_seg, face_seg));
+    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, dv, face_seg, tgt_seg));
+    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, D, cp_seg, cp_seg));
+    pw = std::max(pw, pairsum_batch_workspace(OP_CONV, c.precision, D, D, Seg::single(m),
                                               Seg::single(m)));
     L.pw_bytes = pw;
     L.pw = base ? (void *)(base + off) : nullptr;
@@ -205,6 +207,10 @@ int check_desc(const dfm_atlas_desc *d) {
   if (!d->template_faces || !d->target_vertex_offsets || !d->target_face_offsets ||
       !d->target_vertices || !d->target_faces)
     return set_error(DFM_ERR_ARG, "NULL mesh argument");
+  const int D = d->dimension ? d->dimension : 3, corners = d->cell_corners ? d->cell_corners : 3;
+  if (!((corners == 3 && D == 3) || (corners == 2 && (D == 2 || D == 3))))
+    return set_error(DFM_ERR_DIM, "cells of %d points in dimension %d: triangles need 3, "
+                                  "segments 2 or 3", corners, D);
   return DFM_OK;
 }
 
@@ -220,7 +226,12 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   if (rc) return rc;
   std::unique_ptr<dfm_atlas> A(new dfm_atlas());
   A->c = *desc;
-  A->dv = desc->kind == DFM_CURRENT ? 3 : 6;
+  A->D = desc->dimension ? desc->dimension : 3;
+  A->corners = desc->cell_corners ? desc->cell_corners : 3;
+  const int D = A->D, K = A->corners;
+  // weights per cell: the normal or tangent (current), the upper triangle of its square
+  // (varifold)
+  A->dv = desc->kind == DFM_CURRENT ? D : D * (D + 1) / 2;
   A->S = desc->nsubjects;
   A->n = desc->n_control_points;
   A->m = desc->n_template_vertices;
@@ -232,22 +243,22 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
     A->face_off.push_back((long)s * A->F);
   }
   // template topology
-  std::vector<int> off(A->m + 1), inc(3 * A->F);
-  if ((rc = dfm_mesh_incidence(desc->template_faces, A->F, A->m, off.data(), inc.data())))
+  std::vector<int> off(A->m + 1), inc((size_t)K * A->F);
+  if ((rc = dfm_cell_incidence(desc->template_faces, A->F, K, A->m, off.data(), inc.data())))
     return rc;
   // targets: local face indices -> indices into the concatenated vertices
   const long *vo = desc->target_vertex_offsets, *fo = desc->target_face_offsets;
   if (vo[0] != 0 || fo[0] != 0) return set_error(DFM_ERR_ARG, "target offsets must start at 0");
-  std::vector<int> tf(3 * (size_t)fo[S]);
+  std::vector<int> tf((size_t)K * fo[S]);
   for (int s = 0; s < S; s++) {
     if (vo[s + 1] < vo[s] || fo[s + 1] < fo[s])
       return set_error(DFM_ERR_ARG, "target offsets must be non-decreasing");
     const long nvs = vo[s + 1] - vo[s];
-    for (long i = 3 * fo[s]; i < 3 * fo[s + 1]; i++) {
+    for (long i = K * fo[s]; i < K * fo[s + 1]; i++) {
       const int v = desc->target_faces[i];
       if (v < 0 || v >= nvs)
-        return set_error(DFM_ERR_ARG, "target %d: face %ld references vertex %d outside [0, %ld)", s,
-                         i / 3 - fo[s], v, nvs);
+        return set_error(DFM_ERR_ARG, "target %d: cell %ld references vertex %d outside [0, %ld)", s,
+                         i / K - fo[s], v, nvs);
       tf[i] = (int)(v + vo[s]);
     }
   }
@@ -256,7 +267,7 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   A->tgt_off.assign(fo, fo + S + 1);
   // device copies
   auto own = [&](void *p) { A->owned.push_back(p); };
-  if ((rc = upload(&A->faces, desc->template_faces, 3 * (size_t)A->F))) return rc;
+  if ((rc = upload(&A->faces, desc->template_faces, (size_t)K * A->F))) return rc;
   own(A->faces);
   if ((rc = upload(&A->inc_off, off.data(), off.size()))) return rc;
   own(A->inc_off);
@@ -274,7 +285,7 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   // target geometry and squared norms, once
   const long ntf = fo[S];
   const int dv = A->dv;
-  if ((rc = dmalloc(&A->cT, ntf * 3)) || (own(A->cT), rc = dmalloc(&A->wT, ntf * dv)) ||
+  if ((rc = dmalloc(&A->cT, ntf * D)) || (own(A->cT), rc = dmalloc(&A->wT, ntf * dv)) ||
       (own(A->wT), rc = dmalloc(&A->aT, ntf * dv)) || (own(A->aT), rc = dmalloc(&A->T2, S)))
     return rc;
   own(A->T2);
@@ -282,7 +293,7 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   int *tfd = nullptr;
   void *pw = nullptr;
   const size_t pwb =
-      pairsum_batch_workspace(OP_CONV, desc->precision, 3, dv, A->tgt_seg, A->tgt_seg);
+      pairsum_batch_workspace(OP_CONV, desc->precision, D, dv, A->tgt_seg, A->tgt_seg);
   struct Tmp {
     std::vector<void *> v;
     ~Tmp() {
@@ -290,11 +301,11 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
         if (p) (void)hipFree(p);
     }
   } tmp;
-  if ((rc = upload(&tv, desc->target_vertices, 3 * (size_t)vo[S]))) return rc;
+  if ((rc = upload(&tv, desc->target_vertices, (size_t)D * vo[S]))) return rc;
   tmp.v.push_back(tv);
   if ((rc = upload(&tfd, tf.data(), tf.size()))) return rc;
   tmp.v.push_back(tfd);
-  if ((rc = dmalloc(&nT, ntf * 3)) || (tmp.v.push_back(nT), rc = dmalloc(&kT, ntf * dv)) ||
+  if ((rc = dmalloc(&nT, ntf * D)) || (tmp.v.push_back(nT), rc = dmalloc(&kT, ntf * dv)) ||
       (tmp.v.push_back(kT), rc = dmalloc(&bad, 1)))
     return rc;
   tmp.v.push_back(bad);
@@ -304,11 +315,12 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   }
   hipStream_t st = nullptr;
   if ((rc = fill(st, bad, 1, 0.0))) return rc;
-  if ((rc = surf_geometry(st, desc->kind, tv, vo[S], tfd, ntf, 1, A->cT, A->wT, A->aT, nT, bad)))
+  if ((rc = cells_geometry(st, desc->kind, D, K, tv, vo[S], tfd, ntf, 1, A->cT, A->wT, A->aT, nT,
+                           bad)))
     return rc;
   KParams p = blank();
   p.x = A->cT; p.y = A->cT; p.w = A->wT; p.out0 = kT;
-  if ((rc = pairsum_batch(st, OP_CONV, desc->precision, 3, dv, desc->attachment_width, p,
+  if ((rc = pairsum_batch(st, OP_CONV, desc->precision, D, dv, desc->attachment_width, p,
                           A->tgt_seg, A->tgt_seg, ntf * dv, 0, pw, pwb)))
     return rc;
   if ((rc = seg_dot(st, kT, A->aT, dv, S, 0, A->d_tgt_off, A->T2))) return rc;
@@ -340,79 +352,80 @@ int dfm_atlas_gradient(dfm_atlas *A, void *stream, const double *cp, const doubl
     return set_error(DFM_ERR_ARG, "NULL argument");
   hipStream_t st = (hipStream_t)stream;
   const dfm_atlas_desc &c = A->c;
-  const int S = A->S, dv = A->dv, prec = c.precision;
+  const int S = A->S, dv = A->dv, prec = c.precision, D = A->D, K = A->corners;
   const long n = A->n, m = A->m, F = A->F, SF = (long)S * F;
   dfm_atlas::Layout L = A->layout(A->work);
   int rc;
   // per-subject copies of the shared control points and template
-  hipLaunchKernelGGL(k_tile, dim3(blocks(n * 3 * S)), dim3(256), 0, st, L.p0r, cp, n * 3, S);
-  hipLaunchKernelGGL(k_tile, dim3(blocks(m * 3 * S)), dim3(256), 0, st, L.x0r, template_vertices,
-                     m * 3, S);
+  hipLaunchKernelGGL(k_tile, dim3(blocks(n * D * S)), dim3(256), 0, st, L.p0r, cp, n * D, S);
+  hipLaunchKernelGGL(k_tile, dim3(blocks(m * D * S)), dim3(256), 0, st, L.x0r, template_vertices,
+                     m * D, S);
   if ((rc = launched("k_tile"))) return rc;
   // forward: shoot + flow of every subject, checkpointed
   const double *xT = nullptr;
-  if ((rc = batch_adjoint_phase(st, 3, c.deformation_width, prec, c.scheme, c.improved_euler, S,
+  if ((rc = batch_adjoint_phase(st, D, c.deformation_width, prec, c.scheme, c.improved_euler, S,
                                 A->cp_off.data(), A->lm_off.data(), L.p0r, momenta, L.x0r,
                                 c.number_of_time_points, c.t0, c.tN, c.checkpoint_every, nullptr,
                                 nullptr, nullptr, nullptr, L.adj, L.adj_bytes, 1, &xT, c.adjoint)))
     return rc;
   // data term of every deformed template against its target
   if ((rc = fill(st, L.bad, 1, 0.0))) return rc;
-  if ((rc = surf_geometry(st, c.kind, xT, m, A->faces, F, S, L.cS, L.wS, L.aS, L.nS, L.bad)))
+  if ((rc = cells_geometry(st, c.kind, D, K, xT, m, A->faces, F, S, L.cS, L.wS, L.aS, L.nS,
+                           L.bad)))
     return rc;
   KParams p = blank();
   p.x = L.cS; p.ta = L.aS; p.y = L.cS; p.w = L.wS; p.out0 = L.k1; p.out1 = L.g1;
-  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, 3, dv, c.attachment_width, p, A->face_seg,
-                          A->face_seg, SF * dv, SF * 3, L.pw, L.pw_bytes)))
+  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, D, dv, c.attachment_width, p, A->face_seg,
+                          A->face_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
     return rc;
   p.y = A->cT; p.w = A->wT; p.out0 = L.k2; p.out1 = L.g2;
-  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, 3, dv, c.attachment_width, p, A->face_seg,
-                          A->tgt_seg, SF * dv, SF * 3, L.pw, L.pw_bytes)))
+  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, D, dv, c.attachment_width, p, A->face_seg,
+                          A->tgt_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
     return rc;
   if ((rc = seg_dot(st, L.k1, L.aS, dv, S, F, nullptr, L.S2))) return rc;
   if ((rc = seg_dot(st, L.k2, L.aS, dv, S, F, nullptr, L.ST))) return rc;
   // regularity: K a_s and Grad1(cp, a_s) in one pair sum over all subjects
   p = blank();
   p.x = L.p0r; p.ta = momenta; p.y = L.p0r; p.w = momenta; p.out0 = L.ka; p.out1 = L.gr;
-  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, 3, 3, c.deformation_width, p, A->cp_seg,
-                          A->cp_seg, (long)S * n * 3, (long)S * n * 3, L.pw, L.pw_bytes)))
+  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, D, D, c.deformation_width, p, A->cp_seg,
+                          A->cp_seg, (long)S * n * D, (long)S * n * D, L.pw, L.pw_bytes)))
     return rc;
-  if ((rc = seg_dot(st, L.ka, momenta, 3, S, n, nullptr, L.reg))) return rc;
+  if ((rc = seg_dot(st, L.ka, momenta, D, S, n, nullptr, L.reg))) return rc;
   hipLaunchKernelGGL(k_loglik, dim3(1), dim3(64), 0, st, A->T2, L.S2, L.ST, L.reg, S,
                      1.0 / c.data_sigma_squared, residuals, loglik);
   if ((rc = launched("k_loglik"))) return rc;
   if (!grad_cp && !grad_momenta && !grad_template) return DFM_OK;
-  if ((rc = surf_contrib(st, c.kind, xT, m, A->faces, F, S, L.nS, L.k1, L.g1, L.k2, L.g2,
-                         L.contrib)))
+  if ((rc = cells_contrib(st, c.kind, D, K, xT, m, A->faces, F, S, L.nS, L.k1, L.g1, L.k2, L.g2,
+                          L.contrib)))
     return rc;
   // theta = d (r_s / (2 sigma_data^2)) / d x_T
-  if ((rc = surf_gather(st, A->inc_off, A->inc, m, F, S, L.contrib, 0.5 / c.data_sigma_squared,
-                        L.theta)))
+  if ((rc = cells_gather(st, D, K, A->inc_off, A->inc, m, F, S, L.contrib,
+                         0.5 / c.data_sigma_squared, L.theta)))
     return rc;
   // backward through the flow and the shot
-  if ((rc = batch_adjoint_phase(st, 3, c.deformation_width, prec, c.scheme, c.improved_euler, S,
+  if ((rc = batch_adjoint_phase(st, D, c.deformation_width, prec, c.scheme, c.improved_euler, S,
                                 A->cp_off.data(), A->lm_off.data(), L.p0r, momenta, L.x0r,
                                 c.number_of_time_points, c.t0, c.tN, c.checkpoint_every, L.theta,
                                 L.gp, L.ga, L.gx, L.adj, L.adj_bytes, 2, nullptr, c.adjoint)))
     return rc;
   if (grad_momenta)
-    hipLaunchKernelGGL(k_neg_sum, dim3(blocks((long)S * n * 3)), dim3(256), 0, st, grad_momenta,
-                       L.ga, L.ka, (long)S * n * 3);
+    hipLaunchKernelGGL(k_neg_sum, dim3(blocks((long)S * n * D)), dim3(256), 0, st, grad_momenta,
+                       L.ga, L.ka, (long)S * n * D);
   if (grad_cp)
-    hipLaunchKernelGGL(k_sum_subjects, dim3(blocks(n * 3)), dim3(256), 0, st, grad_cp, L.gp, L.gr,
-                       n * 3, S, -1.0);
+    hipLaunchKernelGGL(k_sum_subjects, dim3(blocks(n * D)), dim3(256), 0, st, grad_cp, L.gp, L.gr,
+                       n * D, S, -1.0);
   if ((rc = launched("atlas gradient assembly"))) return rc;
   if (grad_template) {
     const bool smooth = c.smoothing_width > 1e-20;
-    hipLaunchKernelGGL(k_sum_subjects, dim3(blocks(m * 3)), dim3(256), 0, st,
-                       smooth ? L.gl2 : grad_template, L.gx, (const double *)nullptr, m * 3, S,
+    hipLaunchKernelGGL(k_sum_subjects, dim3(blocks(m * D)), dim3(256), 0, st,
+                       smooth ? L.gl2 : grad_template, L.gx, (const double *)nullptr, m * D, S,
                        -1.0);
     if ((rc = launched("k_sum_subjects"))) return rc;
     if (smooth) {
       p = blank();
       p.x = template_vertices; p.y = template_vertices; p.w = L.gl2; p.out0 = grad_template;
-      if ((rc = pairsum_batch(st, OP_CONV, prec, 3, 3, c.smoothing_width, p, Seg::single(m),
-                              Seg::single(m), m * 3, 0, L.pw, L.pw_bytes)))
+      if ((rc = pairsum_batch(st, OP_CONV, prec, D, D, c.smoothing_width, p, Seg::single(m),
+                              Seg::single(m), m * D, 0, L.pw, L.pw_bytes)))
         return rc;
     }
   }

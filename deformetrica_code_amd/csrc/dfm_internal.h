@@ -140,6 +140,17 @@ int surf_geometry(hipStream_t st, int kind, const double *V, long nv, const int 
 int surf_contrib(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
                  int nsub, const double *N, const double *ktS, const double *gS,
                  const double *ktT, const double *gT, double *contrib);
+// the same for cells of `corners` points in dimension D: triangles (3, D = 3) through the
+// surface launchers, segments (2, D = 2 or 3) through the polyline kernels; contrib holds
+// corners x D values per cell
+int cells_geometry(hipStream_t st, int kind, int D, int corners, const double *V, long nv,
+                   const int *cells, long nc, int nsub, double *C, double *W, double *A,
+                   double *N, double *bad);
+int cells_contrib(hipStream_t st, int kind, int D, int corners, const double *V, long nv,
+                  const int *cells, long nc, int nsub, const double *N, const double *ktS,
+                  const double *gS, const double *ktT, const double *gT, double *contrib);
+int cells_gather(hipStream_t st, int D, int corners, const int *offsets, const int *inc, long nv,
+                 long nc, int nsub, const double *contrib, double scale, double *grad);
 int surf_gather(hipStream_t st, const int *offsets, const int *inc, long nv, long nf, int nsub,
                 const double *contrib, double scale, double *grad);
 int seg_dot(hipStream_t st, const double *X, const double *Y, int dv, int nsub, long seg_len,

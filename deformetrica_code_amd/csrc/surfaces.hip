@@ -290,15 +290,19 @@ __device__ __forceinline__ void sym_times_d(const double *m, const double *x, do
 // (OrientedPolyLine.cxx:249-272); varifold T = (p1 - p0) / sqrt(|p1 - p0| + 1e-20),
 // W = upper triangle of T T^T, A = its quadratic-form coefficients
 // (NonOrientedPolyLine.cxx:266-297).  Bad indices flag the segment (weights zero).
+// nsub polylines share one segment list E (ne segments over nv points): segment f of the
+// output belongs to polyline f / ne, whose points start at row (f / ne) * nv of V.
 template <int D>
 __global__ __launch_bounds__(256) void k_seg_geometry(int kind, const double *V, long nv,
-                                                      const int *E, long ne, double *C,
+                                                      const int *E, long ne, int nsub, double *C,
                                                       double *W, double *A, double *T,
                                                       double *bad) {
   const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= ne) return;
+  if (f >= ne * nsub) return;
+  const long sub = f / ne, fl = f - sub * ne;
+  V += sub * nv * D;
   const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
-  const int i0 = E[2 * f], i1 = E[2 * f + 1];
+  const int i0 = E[2 * fl], i1 = E[2 * fl + 1];
   if (i0 < 0 || i1 < 0 || i0 >= nv || i1 >= nv) {
     for (int k = 0; k < D; k++) C[D * f + k] = 0.0, T[D * f + k] = 0.0;
     for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
@@ -332,15 +336,16 @@ __global__ __launch_bounds__(256) void k_seg_geometry(int kind, const double *V,
 //   varifold  u = 4 S(kt) T, u = (u - T (T.u) / (2 |T|^2)) / |T|; endpoints -u + g, u + g,
 //             nothing where |T|^2 <= 1e-20 (NonOrientedPolyLine.cxx:215-251)
 template <int D>
-__global__ __launch_bounds__(256) void k_seg_contrib(int kind, const int *E, long ne, long nv,
-                                                     const double *T, const double *ktS,
+__global__ __launch_bounds__(256) void k_seg_contrib(int kind, const int *E, long ne, int nsub,
+                                                     long nv, const double *T, const double *ktS,
                                                      const double *gS, const double *ktT,
                                                      const double *gT, double *contrib) {
   const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= ne) return;
+  if (f >= ne * nsub) return;
+  const long fl = f - (f / ne) * ne;
   double *out = contrib + 2 * D * f;
   for (int k = 0; k < 2 * D; k++) out[k] = 0.0;
-  const int i0 = E[2 * f], i1 = E[2 * f + 1];
+  const int i0 = E[2 * fl], i1 = E[2 * fl + 1];
   if (i0 < 0 || i1 < 0 || i0 >= nv || i1 >= nv) return;
   double u[D], g[D];
   for (int k = 0; k < D; k++) g[k] = gS[D * f + k] - gT[D * f + k];
@@ -535,10 +540,10 @@ int curve_geometry(hipStream_t st, int kind, const CurveLayout &L, const double 
   if (rc) return rc;
   if (nes > 0)
     hipLaunchKernelGGL(k_seg_geometry<D>, dim3(blocks_for(nes)), dim3(256), 0, st, kind, vs, nvs,
-                       es, nes, L.cS, L.wS, L.aS, L.tS, L.bad);
+                       es, nes, 1, L.cS, L.wS, L.aS, L.tS, L.bad);
   if (net > 0)
     hipLaunchKernelGGL(k_seg_geometry<D>, dim3(blocks_for(net)), dim3(256), 0, st, kind, vt, nvt,
-                       et, net, L.cT, L.wT, L.aT, L.tT, L.bad);
+                       et, net, 1, L.cT, L.wT, L.aT, L.tT, L.bad);
   return launched("k_seg_geometry");
 }
 
@@ -587,7 +592,7 @@ int curve_match_gradient(hipStream_t st, int kind, double sigma, int prec, const
     return rc;
   if (nes > 0)
     hipLaunchKernelGGL(k_seg_contrib<D>, dim3(blocks_for(nes)), dim3(256), 0, st, kind, es, nes,
-                       nvs, L.tS, L.k1, L.g1, L.k2, L.g2, L.contrib);
+                       1, nvs, L.tS, L.k1, L.g1, L.k2, L.g2, L.contrib);
   if (nvs > 0)
     hipLaunchKernelGGL(k_vertex_gather<D>, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
                        incidences, nvs, 2 * nes, 1, L.contrib, 1.0, grad);
@@ -619,6 +624,52 @@ int surf_gather(hipStream_t st, const int *offsets, const int *inc, long nv, lon
   if (nv * nsub <= 0) return DFM_OK;
   hipLaunchKernelGGL(k_vertex_gather<3>, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets, inc,
                      nv, 3 * nf, nsub, contrib, scale, grad);
+  return launched("k_vertex_gather");
+}
+
+int cells_geometry(hipStream_t st, int kind, int D, int corners, const double *V, long nv,
+                   const int *cells, long nc, int nsub, double *C, double *W, double *A,
+                   double *N, double *bad) {
+  if (corners == 3) {
+    if (D != 3) return set_error(DFM_ERR_DIM, "triangle meshes need dimension 3");
+    return surf_geometry(st, kind, V, nv, cells, nc, nsub, C, W, A, N, bad);
+  }
+  if (corners != 2 || (D != 2 && D != 3))
+    return set_error(DFM_ERR_DIM, "cells of %d points in dimension %d", corners, D);
+  if (nc * nsub <= 0) return DFM_OK;
+  if (D == 2)
+    hipLaunchKernelGGL(k_seg_geometry<2>, dim3(blocks_for(nc * nsub)), dim3(256), 0, st, kind, V,
+                       nv, cells, nc, nsub, C, W, A, N, bad);
+  else
+    hipLaunchKernelGGL(k_seg_geometry<3>, dim3(blocks_for(nc * nsub)), dim3(256), 0, st, kind, V,
+                       nv, cells, nc, nsub, C, W, A, N, bad);
+  return launched("k_seg_geometry");
+}
+
+int cells_contrib(hipStream_t st, int kind, int D, int corners, const double *V, long nv,
+                  const int *cells, long nc, int nsub, const double *N, const double *ktS,
+                  const double *gS, const double *ktT, const double *gT, double *contrib) {
+  if (corners == 3) return surf_contrib(st, kind, V, nv, cells, nc, nsub, N, ktS, gS, ktT, gT, contrib);
+  if (nc * nsub <= 0) return DFM_OK;
+  if (D == 2)
+    hipLaunchKernelGGL(k_seg_contrib<2>, dim3(blocks_for(nc * nsub)), dim3(256), 0, st, kind,
+                       cells, nc, nsub, nv, N, ktS, gS, ktT, gT, contrib);
+  else
+    hipLaunchKernelGGL(k_seg_contrib<3>, dim3(blocks_for(nc * nsub)), dim3(256), 0, st, kind,
+                       cells, nc, nsub, nv, N, ktS, gS, ktT, gT, contrib);
+  return launched("k_seg_contrib");
+}
+
+int cells_gather(hipStream_t st, int D, int corners, const int *offsets, const int *inc, long nv,
+                 long nc, int nsub, const double *contrib, double scale, double *grad) {
+  if (corners == 3) return surf_gather(st, offsets, inc, nv, nc, nsub, contrib, scale, grad);
+  if (nv * nsub <= 0) return DFM_OK;
+  if (D == 2)
+    hipLaunchKernelGGL(k_vertex_gather<2>, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets,
+                       inc, nv, 2 * nc, nsub, contrib, scale, grad);
+  else
+    hipLaunchKernelGGL(k_vertex_gather<3>, dim3(blocks_for(nv * nsub)), dim3(256), 0, st, offsets,
+                       inc, nv, 2 * nc, nsub, contrib, scale, grad);
   return launched("k_vertex_gather");
 }
 

@@ -447,7 +447,7 @@ int dfm_samples_match_gradient(void *stream, int kind, double sigma, int precisi
                                const double *Nt, long nt, double *match, double *grad_centers,
                                double *grad_normals, void *work, size_t work_bytes);
 
-/* ---- deterministic atlas of surfaces ---------------------------------------------------
+/* ---- deterministic atlas of surfaces or polylines ----------------------------------------
  * Log-likelihood and gradient of DeterministicAtlas for surface subjects
  * (DeterministicAtlas::UpdateFixedEffectsAndComputeCompleteLogLikelihood and
  * ::ComputeCompleteLogLikelihoodGradient, DeterministicAtlas.cxx:100-151, 159-281, with
@@ -480,22 +480,25 @@ typedef struct dfm_atlas_desc {
   long n_control_points;
   long n_template_vertices;
   long n_template_faces;
-  const int *template_faces;    /* host, n_template_faces x 3 */
+  const int *template_faces;    /* host, n_template_faces x cell_corners */
   int nsubjects;                /* 1 .. 65535 */
   const long *target_vertex_offsets;  /* host, nsubjects + 1 */
   const long *target_face_offsets;    /* host, nsubjects + 1 */
-  const double *target_vertices;      /* host, concatenated, x 3 */
+  const double *target_vertices;      /* host, concatenated, x dimension */
   const int *target_faces;            /* host, concatenated, indices local to each target */
   int adjoint;                  /* DFM_ADJOINT_DISCRETE or DFM_ADJOINT_CONTINUOUS */
+  int dimension;                /* 2 or 3 (0: 3); every point array below is x dimension */
+  int cell_corners;             /* 3: triangle surfaces (dimension 3; 0: 3), 2: polylines
+                                   (OrientedPolyLine / NonOrientedPolyLine segments) */
 } dfm_atlas_desc;
 
 int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **atlas);
 int dfm_atlas_destroy(dfm_atlas *atlas);
 size_t dfm_atlas_device_bytes(const dfm_atlas *atlas);
-/* Device pointers: cp (n x 3), momenta (nsubjects x n x 3), template_vertices (nv x 3);
- * outputs loglik (2 doubles), residuals (nsubjects, may be NULL), grad_cp (n x 3, may be
- * NULL), grad_momenta (nsubjects x n x 3, may be NULL), grad_template (nv x 3, may be
- * NULL). */
+/* Device pointers (d = the descriptor's dimension): cp (n x d), momenta
+ * (nsubjects x n x d), template_vertices (nv x d); outputs loglik (2 doubles), residuals
+ * (nsubjects, may be NULL), grad_cp (n x d, may be NULL), grad_momenta (nsubjects x n x d,
+ * may be NULL), grad_template (nv x d, may be NULL). */
 int dfm_atlas_gradient(dfm_atlas *atlas, void *stream, const double *cp, const double *momenta,
                        const double *template_vertices, double *loglik, double *residuals,
                        double *grad_cp, double *grad_momenta, double *grad_template);
