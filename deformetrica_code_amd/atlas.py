@@ -222,8 +222,8 @@ class HIPAtlasModel:
     Get/SetFixedEffects, UpdateFixedEffectsAndComputeCompleteLogLikelihood,
     ComputeCompleteLogLikelihoodGradient; DeterministicAtlas.cxx:100-201) over a
     HIPDeterministicAtlas created with smoothing_width 0.  Fixed effects "ControlPoints"
-    (n x 3) and "TemplateData" ([vertices]); individual random effects "Momenta" (one
-    n x 3 per subject).
+    (n x d) and "TemplateData" ([vertices]); individual random effects "Momenta" (one
+    n x d per subject), d the atlas's dimension.
 
     The reference's behaviour is kept where it shapes the numbers:
     - SetFixedEffects (AbstractAtlas.cxx:52-63) stores the new fixed effects but moves the
