@@ -30,6 +30,8 @@ Other workloads (one JSON line each, their own metric; the default is the headli
                              2,048 shared control points, eager and HIP-graph replay
     --workload varifold      varifold match + gradient between two sets of 1,000,000
                              oriented sphere samples, sigma 0.02, FP32C vs FP64
+    --workload curves        polyline varifold match + gradient between two closed 2D
+                             curves of 1,000,000 segments, sigma 0.02, FP32C vs FP64
 """
 from __future__ import annotations
 
@@ -65,7 +67,8 @@ def parse():
                     help="approximate wall time for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="kernel",
-                    choices=["kernel", "shooting", "registration", "atlas", "varifold"])
+                    choices=["kernel", "shooting", "registration", "atlas", "varifold",
+                             "curves"])
     ap.add_argument("--subjects", type=int, default=64)
     return ap.parse_args()
 
@@ -525,8 +528,71 @@ def workload_varifold(args):
                  grad_normals_rel_err_vs_fp64=_nrel(res["fp64"]["grad_n"], r["grad_n"]))
 
 
+def workload_curves(args):
+    """Polyline varifold match + gradient between two closed 2D curves of 1M segments each
+    (NonOrientedPolyLine::ComputeMatch / ComputeMatchGradient at scale)."""
+    from deformetrica_code_amd import _lib
+    from deformetrica_code_amd.curves import cell_incidence, curve_match, curve_match_gradient
+    from deformetrica_code_amd.device import DeviceArray, Stream, Workspace
+    n = args.n
+    t = np.linspace(0.0, 2 * np.pi, n, endpoint=False)
+    rs = 1.0 + 0.2 * np.sin(5 * t)
+    rt = 1.05 + 0.15 * np.sin(5 * t + 0.3) + 0.02 * np.cos(17 * t)
+    VS = np.stack([rs * np.cos(t), rs * np.sin(t)], axis=1)
+    VT = np.stack([rt * np.cos(t + 0.01), rt * np.sin(t + 0.01)], axis=1)
+    E = np.stack([np.arange(n), (np.arange(n) + 1) % n], axis=1).astype(np.int32)
+    off, inc = cell_incidence(E, n)
+    L = _lib.load()
+    st = Stream()
+    ws = Workspace()
+    dV, dW = DeviceArray.from_numpy(VS, st), DeviceArray.from_numpy(VT, st)
+    dE = DeviceArray.from_numpy(E, st, dtype=np.int32)
+    dOff = DeviceArray.from_numpy(off, st, dtype=np.int32)
+    dInc = DeviceArray.from_numpy(inc, st, dtype=np.int32)
+    sigma = 0.02
+    res = {}
+    for prec, reps in (("fp32c", 2), ("fp32x", 1), ("fp64", 1)):
+        def run():
+            m = curve_match("varifold", sigma, dV, dE, dW, dE, prec, st, ws)
+            g = curve_match_gradient("varifold", sigma, dV, dE, dOff, dInc, dW, dE, prec, st, ws)
+            st.synchronize()
+            return m, g
+        m, g = run()
+        wall, kern, launches = _profiled(L, run, reps)
+        res[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "launches": launches,
+                     "match": float(m.numpy(st)[0]), "grad": g.numpy(st)}
+    pairs = 5.0 * n * n                    # three Convolve sums (match), two fused (gradient)
+    # 2D, Dv = 3: shared part 7 (2 differences, 3 for r^2, 1 scale, 1 exp); Convolve 6;
+    # gradient 11 (alpha.w 5, coefficient 2, 2 FMA 4)
+    flops = (3 * 13 + 2 * 24) * float(n) * n
+    r = res["fp32c"]
+    return _line("polyline varifold pair-evals/s (2D, 1M x 1M segments, match + gradient)",
+                 pairs / (r["ms"] / 1e3), "pair-evals/s", True, r["ms"],
+                 {"workload": "two closed 2D polylines of %d segments each (radius 1 + 0.2 "
+                              "sin 5t and a perturbed copy), NonOrientedPolyLine varifold, "
+                              "sigma 0.02: match and d match / d points" % n},
+                 dtype="f32",
+                 roofline={"bound": "valu",
+                           "achieved": flops / (r["pairsum_kernel_ms"] / 1e3) / 1e12,
+                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                           "frac": flops / (r["pairsum_kernel_ms"] / 1e3) / FP32_PEAK,
+                           "flops_per_pair": "13 (2D Dv=3 Convolve) x3 + 24 (2D fused) x2"},
+                 fp64_ms=res["fp64"]["ms"], speedup_fp32c_over_fp64=res["fp64"]["ms"] / r["ms"],
+                 match_rel_err_vs_fp64=abs(r["match"] - res["fp64"]["match"]) / abs(res["fp64"]["match"]),
+                 grad_rel_err_vs_fp64=_nrel(res["fp64"]["grad"], r["grad"]),
+                 fp32x={"ms": res["fp32x"]["ms"],
+                        "match_rel_err_vs_fp64": abs(res["fp32x"]["match"] - res["fp64"]["match"])
+                        / abs(res["fp64"]["match"]),
+                        "grad_rel_err_vs_fp64": _nrel(res["fp64"]["grad"], res["fp32x"]["grad"])},
+                 fp32c_pairsum_kernel_ms=r["pairsum_kernel_ms"], launches=r["launches"],
+                 note="the gradient at a point is the difference of its two segments' kernel "
+                      "sums, which differ by about h/sigma (segment length over kernel width, "
+                      "1/3000 here): the FP32 sums' ~1e-7 error reaches the gradient amplified "
+                      "about sigma/h-fold, in FP32X too; the match is not affected")
+
+
 WORKLOADS = {"shooting": workload_shooting, "registration": workload_registration,
-             "atlas": workload_atlas, "varifold": workload_varifold}
+             "atlas": workload_atlas, "varifold": workload_varifold, "curves": workload_curves}
 
 
 if __name__ == "__main__":
