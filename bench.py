@@ -468,8 +468,24 @@ def workload_atlas(args):
         g.launch()
     A.stream.synchronize()
     graph_ms = (time.perf_counter() - t0) / 3 * 1e3
-    L.dfm_autotune(0)
     report = _lib.autotune_report()
+    # the same gradient with the whole trajectory resident (checkpoint every step: nothing
+    # recomputed in the backward), tuned plans, graph replay
+    L.dfm_autotune(2)
+    A1 = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], targets, precision="fp32c",
+                               kind="varifold", deformation_width=0.25, attachment_width=0.1,
+                               data_sigma_squared=1.0, number_of_time_points=11,
+                               checkpoint_every=1)
+    g1 = A1.capture(dcp, dmom, dV)
+    g1.launch()
+    A1.stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        g1.launch()
+    A1.stream.synchronize()
+    resident_ms = (time.perf_counter() - t0) / 3 * 1e3
+    grad_diff = float(np.max(np.abs(A1.grad_momenta.numpy(A1.stream) - A.grad_momenta.numpy(A.stream))))
+    L.dfm_autotune(0)
     L.dfm_autotune_reset()
     faces = [t[1].shape[0] for t in targets]
     return _line("batched atlas gradient time (%d subjects)" % S, graph_ms, "ms", False, graph_ms,
@@ -483,7 +499,10 @@ def workload_atlas(args):
                  autotuned_pairsum_kernel_ms=tkern * 1e3,
                  graph="replay of the autotuned gradient", autotune_table=report.splitlines(),
                  device_bytes=A.device_bytes(),
-                 loglik=[float(v) for v in A.loglik.numpy(A.stream)])
+                 loglik=[float(v) for v in A.loglik.numpy(A.stream)],
+                 resident_trajectory={"checkpoint_every": 1, "graph_ms": resident_ms,
+                                      "device_bytes": A1.device_bytes(),
+                                      "max_abs_grad_momenta_diff_vs_k5": grad_diff})
 
 
 def workload_varifold(args):

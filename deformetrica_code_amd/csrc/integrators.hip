@@ -23,8 +23,10 @@
 //   d xi / ds    = J_F(z_s)^T xi + J_z v(x_s; z_s)^T theta_s      (ComputeUpdateAt)
 // where ComputeUpdateAt's six terms are exactly J_F^T xi (dxi3 + dxi4, dxi5 + dxi6) and the
 // landmark coupling (dxi1, dxi2), so the same two fused pair sums serve both forms.
-// The forward stores (p, a, x) every k-th step; the backward recomputes each segment of
-// at most k steps from its checkpoint and walks it in reverse.
+// The forward stores (p, a, x) every k-th step and at the end; the backward recomputes the
+// interior states of each segment of at most k steps from its checkpoint (the segment's
+// last state is the next checkpoint) and walks it in reverse.  With k = 1 nothing is
+// recomputed: the whole trajectory stays resident, (T + 1) x (2n + m) x d doubles.
 //
 // Batches (the per-subject loop of AbstractAtlas::ComputeDataTermGradient,
 // AbstractAtlas.cxx:490-560): subject s owns control points [cp[s], cp[s + 1]) and
@@ -335,7 +337,7 @@ struct FlowLayout {
 struct AdjointLayout {
   int segs, k;
   SegArrays sa;
-  double *ckpt;   // segs x (2n + m) x D
+  double *ckpt;   // (segs + 1) x (2n + m) x D: states 0, k, 2k, ... and the final one
   double *seg;    // (k + 1) x (2n + m) x D
   ShootTmp st;
   FlowTmp ft;
@@ -352,7 +354,7 @@ struct AdjointLayout {
     segs = T > 1 ? (T - 1 + k - 1) / k : 0;
     const long rec = (2 * n + m) * D;
     sa.take(ar, cs.n);
-    ckpt = ar.take((size_t)std::max(segs, 1) * rec);
+    ckpt = ar.take((size_t)(segs + 1) * rec);
     seg = ar.take((size_t)(std::min(k, std::max(T - 1, 0)) + 1) * rec);
     st.take(ar, n * D);
     ft.take(ar, m * D);
@@ -600,6 +602,9 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
         return rc;
       std::swap(cur, nxt);
     }
+    // the final state closes the last segment: the backward reads it instead of
+    // recomputing it
+    if ((rc = copy(cx, L.ckpt + (size_t)L.segs * rec, cur, rec))) return rc;
     if (x_final) *x_final = X_(cur);
   }
   if (!(phase & 2)) return DFM_OK;
@@ -616,12 +621,16 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
     if (rc) return rc;
   }
 
-  // ---- backward, one checkpoint segment at a time
+  // ---- backward, one checkpoint segment at a time: states tb and te come from the
+  // checkpoints, the interior ones are recomputed (the same steps as the forward, so the
+  // same values)
   for (int s = L.segs - 1; s >= 0; s--) {
     const int tb = s * L.k, te = std::min(T - 1, tb + L.k);
     double *S = L.seg;  // states tb..te
     if ((rc = copy(cx, S, L.ckpt + (size_t)s * rec, rec))) return rc;
-    for (int t = tb; t < te; t++) {
+    if ((rc = copy(cx, S + (size_t)(te - tb) * rec, L.ckpt + (size_t)(s + 1) * rec, rec)))
+      return rc;
+    for (int t = tb; t + 1 < te; t++) {
       double *a_ = S + (size_t)(t - tb) * rec, *b_ = S + (size_t)(t - tb + 1) * rec;
       if ((rc = shoot_step(cx, scheme, P_(a_), A_(a_), cs, dt, P_(b_), A_(b_), L.st, gate)))
         return rc;
