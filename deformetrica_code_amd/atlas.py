@@ -49,7 +49,7 @@ class HIPDeterministicAtlas:
                  kind="varifold", precision="fp64", deformation_width=1.0,
                  attachment_width=1.0, data_sigma_squared=1.0, smoothing_width=0.0,
                  number_of_time_points=11, t0=0.0, tN=1.0, scheme="euler", improved_euler=True,
-                 checkpoint_every=5, adjoint="discrete"):
+                 checkpoint_every=0, adjoint="discrete"):
         L = _lib.load()
         tf = np.ascontiguousarray(template_faces, dtype=np.int32)
         corners = tf.shape[1]
@@ -336,7 +336,7 @@ class HIPMultiObjectAtlasModel:
                  deformation_width, number_of_time_points, t0=0.0, tN=1.0,
                  improved_euler=False, scheme="euler", precision="fp64",
                  freeze_template=False, freeze_control_points=False, smoothing_width=0.0,
-                 checkpoint_every=5):
+                 checkpoint_every=0):
         if len(data_sigma_squared) != len(objects):
             raise ValueError("one data sigma squared per object")
         if any(len(t) != len(objects) for t in targets):

@@ -194,8 +194,9 @@ int check_desc(const dfm_atlas_desc *d) {
     return set_error(DFM_ERR_ARG, "unknown precision %d", d->precision);
   if (d->scheme != DFM_EULER && d->scheme != DFM_HEUN)
     return set_error(DFM_ERR_ARG, "unknown scheme %d", d->scheme);
-  if (d->number_of_time_points < 1 || d->checkpoint_every < 1)
-    return set_error(DFM_ERR_ARG, "time points and checkpoint interval must be >= 1");
+  if (d->number_of_time_points < 1 || d->checkpoint_every < 0)
+    return set_error(DFM_ERR_ARG, "time points must be >= 1 and the checkpoint interval "
+                                  ">= 1, or 0 for automatic");
   if (d->adjoint != DFM_ADJOINT_DISCRETE && d->adjoint != DFM_ADJOINT_CONTINUOUS)
     return set_error(DFM_ERR_ARG, "unknown adjoint form %d", d->adjoint);
   if (!(d->deformation_width > 0) || !(d->attachment_width > 0) || !(d->data_sigma_squared > 0))

@@ -39,6 +39,10 @@ constexpr long kSortMinTargets = 8192;
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
 // (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
 constexpr double kExactExtent = 4.0;
+// checkpoint_every = 0: keep every state (nothing recomputed in the backward) when the
+// trajectory takes at most this many bytes, else every kAutoCheckpoint-th state
+constexpr double kResidentTrajectoryBytes = 4.0 * 1024 * 1024 * 1024;
+constexpr int kAutoCheckpoint = 5;
 
 enum : int { MODE_ADAPTIVE = 0, MODE_EXACT = 1 };
 

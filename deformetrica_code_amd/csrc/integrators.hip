@@ -368,6 +368,15 @@ struct AdjointLayout {
   }
 };
 
+// The checkpoint interval of a call: k >= 1 as given; k = 0 keeps the whole trajectory,
+// (T + 1) x (2n + m) x D doubles, when it fits kResidentTrajectoryBytes (HBM is plentiful:
+// no backward recompute), else every kAutoCheckpoint-th state.  Results do not depend on it.
+int resolve_checkpoint(int k, int T, long n, long m, int D) {
+  if (k != 0) return k;
+  const double bytes = (double)(T + 1) * (double)(2 * n + m) * D * sizeof(double);
+  return bytes <= kResidentTrajectoryBytes ? 1 : kAutoCheckpoint;
+}
+
 // segment descriptors of a call (nseg == 0: one subject with plain counts n, m)
 int segs_of(int nseg, const long *cp_h, const long *lm_h, long n, long m, Seg &cs, Seg &ls) {
   if (nseg == 0) {
@@ -566,8 +575,10 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
   if (rc) return rc;
   if ((rc = check_common(D, prec, sigma, cs.total, ls.total, T))) return rc;
   if (scheme != DFM_EULER && scheme != DFM_HEUN) return set_error(DFM_ERR_ARG, "unknown scheme %d", scheme);
-  if (checkpoint_every < 1) return set_error(DFM_ERR_ARG, "checkpoint interval must be >= 1");
+  if (checkpoint_every < 0)
+    return set_error(DFM_ERR_ARG, "checkpoint interval must be >= 1, or 0 for automatic");
   const long n = cs.total, m = ls.total;
+  checkpoint_every = resolve_checkpoint(checkpoint_every, T, n, m, D);
   if (n == 0) return DFM_OK;
   if (!p0 || !a0 || (m > 0 && !x0)) return set_error(DFM_ERR_ARG, "NULL argument");
   if ((phase & 2) && (!grad_p0 || !grad_a0 || (m > 0 && !theta_T)))
@@ -716,7 +727,8 @@ size_t workspace_or_zero(int nseg, const long *cp_off, const long *lm_off, long 
   const bool batch = nseg > 0;
   if (which == 0) return ShootLayout(nullptr, prec, D, cs, batch).total;
   if (which == 1) return FlowLayout(nullptr, prec, D, cs, ls, batch).total;
-  if (k < 1) return 0;
+  if (k < 0) return 0;
+  k = resolve_checkpoint(k, T, cs.total, ls.total, D);
   return AdjointLayout(nullptr, prec, D, cs, ls, T, k, batch).total;
 }
 

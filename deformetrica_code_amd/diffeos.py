@@ -127,11 +127,11 @@ class HIPDiffeos:
         return list(out.numpy(st))
 
     # --- backward
-    def GradientOfTerminalCost(self, theta_T, checkpoint_every: int = 5):
+    def GradientOfTerminalCost(self, theta_T, checkpoint_every: int = 0):
         """(dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_T: the exact discrete adjoint."""
         return self._adjoint(theta_T, checkpoint_every, "dfm_shoot_flow_adjoint")
 
-    def IntegrateAdjointEquations(self, theta_T, checkpoint_every: int = 5, jump_times=None):
+    def IntegrateAdjointEquations(self, theta_T, checkpoint_every: int = 0, jump_times=None):
         """Diffeos::IntegrateAdjointEquations (Diffeos.cxx:1374-1443) for the landmark
         points: (xi_pos(0), xi_mom(0), theta(0)) of the reference's continuous adjoint
         equations for the terminal landmark cotangent theta_T; the reference's
@@ -272,7 +272,7 @@ class HIPBatchDiffeos(HIPDiffeos):
     def GetDeformedLandmarkPoints(self):
         return self._split(self._X.numpy(self._stream)[-1], self.lm_off)
 
-    def GradientOfTerminalCost(self, theta_list, checkpoint_every: int = 5):
+    def GradientOfTerminalCost(self, theta_list, checkpoint_every: int = 0):
         """Per-subject lists (dE/dp0, dE/da0, dE/dx0) for dE/dx_final = theta_list[s]."""
         L = _lib.load()
         d, T, S = self.dim, self.T, len(self.p0_list)

@@ -51,7 +51,7 @@ class HIPRegressionModel:
     def __init__(self, objects, data_sigma_squared, observations, time_indices, fixed_effects,
                  *, deformation_width, number_of_time_points, t0=0.0, tN=1.0,
                  improved_euler=False, scheme="euler", precision="fp64", freeze_template=False,
-                 freeze_control_points=False, smoothing_width=0.0, checkpoint_every=5):
+                 freeze_control_points=False, smoothing_width=0.0, checkpoint_every=0):
         if len(data_sigma_squared) != len(objects):
             raise ValueError("one data sigma squared per object")
         if len(observations) != len(time_indices):

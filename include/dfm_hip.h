@@ -231,7 +231,10 @@ int dfm_shoot_vjp(void *stream, int dimpoint, double sigma, int precision, const
  * momenta a0 and landmarks x0, given theta_T = dE/dx_{T-1} (m x d): the exact discrete
  * adjoint (backward pass) of dfm_shoot + dfm_flow with the same scheme, precision and
  * time grid.  The forward pass is run once, storing (p, a, x) every checkpoint_every
- * steps; the backward pass recomputes each segment from its checkpoint.  An alternative to
+ * steps and at the end; the backward pass recomputes the interior states of each segment
+ * from its checkpoint.  checkpoint_every = 0 chooses: every state (nothing recomputed)
+ * when the trajectory, (T + 1) x (2n + m) x d doubles, takes at most 4 GiB, else every 5th.
+ * Results are bitwise independent of the interval.  An alternative to
  * Diffeos::IntegrateAdjointEquations / AdjointEquationsIntegrator::Update
  * (Diffeos.cxx:1390-1443, AdjointEquationsIntegrator.cxx:33-136, 414-567), whose continuous
  * adjoint agrees with this gradient to O(dt).  grad_x0 may be NULL. */
@@ -471,7 +474,7 @@ typedef struct dfm_atlas_desc {
   int scheme;                   /* DFM_EULER or DFM_HEUN shooting */
   int improved_euler;           /* Heun (1) or Euler (0) landmark flow */
   int number_of_time_points;    /* T >= 1 */
-  int checkpoint_every;         /* adjoint checkpoint interval >= 1 */
+  int checkpoint_every;         /* adjoint checkpoint interval >= 1, or 0: automatic */
   double t0, tN;
   double deformation_width;     /* kernel width of the deformation */
   double attachment_width;      /* kernel width of the current / varifold norm */

@@ -201,7 +201,7 @@ def test_adjoint_checkpoint_interval_does_not_change_the_gradient():
     p0, a0, x0 = _setup(21, 40, 50, 2, scale=0.2)
     target = x0 + 0.02
     grads = []
-    for k in (1, 2, 5, 16, 100):
+    for k in (1, 2, 5, 16, 100, 0):   # 0: automatic (here every state resident)
         df = _diffeo(2, 0.35, 17, p0, a0, x0, scheme="heun")
         df.Update()
         grads.append(df.GradientOfTerminalCost(df.GetDeformedLandmarkPoints() - target, checkpoint_every=k))
