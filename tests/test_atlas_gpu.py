@@ -224,3 +224,31 @@ def test_cells_and_dimension_are_checked():
     flat = [(v[:, :2], f) for v, f in tgs]       # triangles in 2D
     with pytest.raises(_lib.DfmError, match="triangles need 3"):
         _atlas(Vs[:, :2], Fs, cps[:, :2], flat, "current")
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_graph_replay_of_the_polyline_atlas_gradient(d):
+    """The polyline atlas gradient captured once and replayed on new momenta, with eager
+    calls in between: every replay equals the eager evaluation bit for bit."""
+    from deformetrica_code_amd.device import DeviceArray
+    V, E, cp, mom, targets = _polyline_problem(d, 30 + d)
+    A = _atlas(V, E, cp, targets, "varifold", precision="fp32c", deformation_width=0.6,
+               attachment_width=0.25, data_sigma_squared=0.05, smoothing_width=0.3)
+    st = A.stream
+    dcp, dmom, dV = (DeviceArray.from_numpy(v, st) for v in (cp, mom, V))
+
+    def read():
+        st.synchronize()
+        return [a.numpy(st) for a in (A.loglik, A.grad_cp, A.grad_momenta, A.grad_template)]
+
+    g = A.capture(dcp, dmom, dV)
+    rng = np.random.default_rng(9)
+    for r in range(3):
+        dmom.upload(mom * (1.0 + 0.3 * r) + rng.normal(scale=1e-3, size=mom.shape), st)
+        A.gradient_device(dcp, dmom, dV)
+        ref = read()
+        for a in (A.loglik, A.grad_cp, A.grad_momenta, A.grad_template):
+            a.upload(np.zeros(a.shape), st)
+        g.launch()
+        for u, v in zip(read(), ref):
+            np.testing.assert_array_equal(u, v)
