@@ -237,6 +237,10 @@ def main():
             "traffic": PROFILED_TRAFFIC_BYTES if (N, M, sigma) == (1000000, 1000000, 0.05) else None,
             "traffic_unit": "bytes per launch",
             "traffic_source": PROFILED_TRAFFIC_SOURCE,
+            "bound_note": "VALU-issue bound, neither HBM nor MFMA: the default plan issues no "
+                          "matrix-core instructions and reads 6.8 GB per launch (0.2% of HBM "
+                          "bandwidth); the peak is the FP32 vector peak (the dense FP32 MFMA "
+                          "figure is the same 157.3 TFLOP/s)",
             "kernel": "k_pairsum<float, CONVGRAD1, 3, 3, 0, 1, ADAPTIVE> + k_reduce_splits",
             "kernel_ms": kern_s * 1e3,
             "flops_per_pair": FLOPS_PER_PAIR,
