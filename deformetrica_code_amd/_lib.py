@@ -135,9 +135,15 @@ def load():
     L.dfm_curve_match.argtypes = [vp, i, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
     L.dfm_curve_match_gradient.argtypes = [vp, i, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l,
                                            vp, vp, sz]
+    # volume-mesh data terms
+    L.dfm_volume_workspace.argtypes = [i, l, l, i, i]
+    L.dfm_volume_workspace.restype = sz
+    L.dfm_volume_match.argtypes = [vp, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
+    L.dfm_volume_match_gradient.argtypes = [vp, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l, vp,
+                                            vp, sz]
     for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient",
                  "dfm_samples_match_gradient", "dfm_cell_incidence", "dfm_curve_match",
-                 "dfm_curve_match_gradient"):
+                 "dfm_curve_match_gradient", "dfm_volume_match", "dfm_volume_match_gradient"):
         getattr(L, name).restype = i
     L.dfm_autotune.argtypes = [i]
     L.dfm_autotune.restype = i

@@ -1,7 +1,7 @@
-// surfaces.hip — triangle-surface and polyline data terms on the GPU: currents
-// (OrientedSurfaceMesh, OrientedPolyLine) and varifolds (NonOrientedSurfaceMesh,
-// NonOrientedPolyLine), their match values and their gradients with respect to the deformed
-// template's vertices.  The polyline path (segments in 2D or 3D) follows the same plan with
+// surfaces.hip — triangle-surface, polyline and volume-mesh data terms on the GPU:
+// currents (OrientedSurfaceMesh, OrientedPolyLine), varifolds (NonOrientedSurfaceMesh,
+// NonOrientedPolyLine) and volume measures (OrientedVolumeMesh), their match values and
+// their gradients with respect to the deformed template's vertices.  The polyline path (segments in 2D or 3D) follows the same plan with
 // k_seg_geometry / k_seg_contrib in place of the face kernels.
 //
 // Every kernel sum is one pair-sum call; the rest is per-face and per-vertex work:
@@ -371,6 +371,93 @@ __global__ __launch_bounds__(256) void k_seg_contrib(int kind, const int *E, lon
   }
 }
 
+// ---- volume meshes: cells of D + 1 points (triangles in 2D, tetrahedra in 3D) ------------
+
+// Centre (p0 + ... + pD) / (D + 1) and volume |det(p1 - p0, ..., pD - p0)| / D! of each
+// cell (OrientedVolumeMesh.cxx:305-334); bad indices flag the cell (weight zero).
+template <int D>
+__global__ __launch_bounds__(256) void k_vol_geometry(const double *V, long nv, const int *Cl,
+                                                      long nc, double *C, double *W,
+                                                      double *bad) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nc) return;
+  int id[D + 1];
+  bool ok = true;
+  for (int c = 0; c <= D; c++) {
+    id[c] = Cl[(D + 1) * f + c];
+    ok = ok && id[c] >= 0 && id[c] < nv;
+  }
+  if (!ok) {
+    for (int k = 0; k < D; k++) C[D * f + k] = 0.0;
+    W[f] = 0.0;
+    *bad = 1.0;
+    return;
+  }
+  double p[D + 1][D];
+  for (int c = 0; c <= D; c++)
+    for (int k = 0; k < D; k++) p[c][k] = V[D * (long)id[c] + k];
+  for (int k = 0; k < D; k++) {
+    double s = p[0][k];
+    for (int c = 1; c <= D; c++) s += p[c][k];
+    C[D * f + k] = s / (D + 1.0);
+  }
+  double e[D][D];
+  for (int r = 0; r < D; r++)
+    for (int k = 0; k < D; k++) e[r][k] = p[r + 1][k] - p[0][k];
+  double det;
+  if constexpr (D == 3)
+    det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+          e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+          e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+  else
+    det = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+  W[f] = fabs(det) / (D == 3 ? 6.0 : 2.0);
+}
+
+// contrib[f] ((D + 1) corners x D) of the match gradient as OrientedVolumeMesh writes it
+// (OrientedVolumeMesh.cxx:196-283): the determinant's cofactor vector of each corner times
+// Ktau = (K V_S)(c_f) - (K V_T)(c_f), scaled by the reference's float 1/3 in 3D, plus the
+// centre term (Grad1_S - Grad1_T)(c_f) / ((D + 1) / 2).  The cofactors carry no sign of the
+// determinant: the derivative of the match for positively oriented cells.
+template <int D>
+__global__ __launch_bounds__(256) void k_vol_contrib(const double *V, long nv, const int *Cl,
+                                                     long nc, const double *ktS, const double *gS,
+                                                     const double *ktT, const double *gT,
+                                                     double *contrib) {
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nc) return;
+  double *out = contrib + (D + 1) * D * f;
+  for (int k = 0; k < (D + 1) * D; k++) out[k] = 0.0;
+  int id[D + 1];
+  for (int c = 0; c <= D; c++) {
+    id[c] = Cl[(D + 1) * f + c];
+    if (id[c] < 0 || id[c] >= nv) return;
+  }
+  double p[D + 1][D];
+  for (int c = 0; c <= D; c++)
+    for (int k = 0; k < D; k++) p[c][k] = V[D * (long)id[c] + k];
+  double v[D + 1][D];
+  if constexpr (D == 3) {
+    double a[3], b[3];
+    auto sub = [&](int i, int j, double *o) {
+      for (int k = 0; k < 3; k++) o[k] = p[i][k] - p[j][k];
+    };
+    sub(3, 1, a); sub(2, 1, b); cross3(a, b, v[0]);
+    sub(2, 0, a); sub(3, 0, b); cross3(a, b, v[1]);
+    sub(3, 0, a); sub(1, 0, b); cross3(a, b, v[2]);
+    sub(1, 0, a); sub(2, 0, b); cross3(a, b, v[3]);
+  } else {
+    v[0][0] = p[1][1] - p[2][1]; v[0][1] = p[2][0] - p[1][0];
+    v[1][0] = p[2][1] - p[0][1]; v[1][1] = p[0][0] - p[2][0];
+    v[2][0] = p[0][1] - p[1][1]; v[2][1] = p[1][0] - p[0][0];
+  }
+  const double factor = D == 3 ? (double)(1.0f / 3.0f) : 1.0;
+  const double ktau = ktS[f] - ktT[f];
+  for (int c = 0; c <= D; c++)
+    for (int k = 0; k < D; k++)
+      out[D * c + k] = factor * v[c][k] * ktau + (gS[D * f + k] - gT[D * f + k]) / ((D + 1.0) / 2.0);
+}
+
 size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct SampleLayout {
@@ -471,6 +558,37 @@ struct CurveLayout {
       pw_bytes = std::max({pairsum_workspace(OP_CONV, prec, D, dv, nes, nes),
                            pairsum_workspace(OP_CONV, prec, D, dv, net, net),
                            pairsum_workspace(OP_CONV, prec, D, dv, net, nes)});
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
+
+struct VolumeLayout {
+  double *cS, *wS, *cT, *wT, *k1, *k2, *k3, *g1, *g2, *p1, *p2, *p3, *contrib, *bad;
+  void *pw;
+  size_t pw_bytes, total;
+  VolumeLayout(char *base, int D, int prec, long ncs, long nct, bool gradient) {
+    size_t off = 0;
+    auto take = [&](size_t count) {
+      double *p = base ? (double *)(base + off) : nullptr;
+      off += a256(count * sizeof(double));
+      return p;
+    };
+    cS = take(D * ncs); wS = take(ncs); cT = take(D * nct); wT = take(nct);
+    const long nmax = std::max(ncs, nct);
+    k1 = take(nmax); k2 = take(nmax); k3 = take(nmax);
+    g1 = take(D * ncs); g2 = take(D * ncs);
+    p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
+    contrib = gradient ? take((D + 1) * D * ncs) : nullptr;
+    bad = take(1);
+    if (gradient)
+      pw_bytes = std::max(pairsum_workspace(OP_CONVGRAD1, prec, D, 1, ncs, ncs),
+                          pairsum_workspace(OP_CONVGRAD1, prec, D, 1, ncs, nct));
+    else
+      pw_bytes = std::max({pairsum_workspace(OP_CONV, prec, D, 1, ncs, ncs),
+                           pairsum_workspace(OP_CONV, prec, D, 1, nct, nct),
+                           pairsum_workspace(OP_CONV, prec, D, 1, nct, ncs)});
     pw = base ? (void *)(base + off) : nullptr;
     off += a256(pw_bytes);
     total = off;
@@ -600,6 +718,77 @@ int curve_match_gradient(hipStream_t st, int kind, double sigma, int prec, const
   return report_bad(st, L.bad, "segment");
 }
 
+int check_volume_args(int dim, int prec, double sigma, long nvs, long ncs, long nvt, long nct) {
+  if (dim != 2 && dim != 3)
+    return set_error(DFM_ERR_DIM, "volume meshes need dimension 2 or 3, got %d", dim);
+  return check_args(DFM_CURRENT, prec, sigma, nvs, ncs, nvt, nct);
+}
+
+template <int D>
+int volume_geometry(hipStream_t st, const VolumeLayout &L, const double *vs, long nvs,
+                    const int *cs, long ncs, const double *vt, long nvt, const int *ct,
+                    long nct) {
+  int rc = fill(st, L.bad, 1, 0.0);
+  if (rc) return rc;
+  if (ncs > 0)
+    hipLaunchKernelGGL(k_vol_geometry<D>, dim3(blocks_for(ncs)), dim3(256), 0, st, vs, nvs, cs,
+                       ncs, L.cS, L.wS, L.bad);
+  if (nct > 0)
+    hipLaunchKernelGGL(k_vol_geometry<D>, dim3(blocks_for(nct)), dim3(256), 0, st, vt, nvt, ct,
+                       nct, L.cT, L.wT, L.bad);
+  return launched("k_vol_geometry");
+}
+
+template <int D>
+int volume_match(hipStream_t st, double sigma, int prec, const VolumeLayout &L, const double *vs,
+                 long nvs, const int *cs, long ncs, const double *vt, long nvt, const int *ct,
+                 long nct, double *match) {
+  int rc = volume_geometry<D>(st, L, vs, nvs, cs, ncs, vt, nvt, ct, nct);
+  if (rc) return rc;
+  // |S|^2 = sum_f V_f (K V_S)(c_f) (OrientedVolumeMesh.cxx:341-358), |T|^2 likewise,
+  // <S, T> = sum_f V_T,f (K V_S)(c_T,f) (:142-149)
+  KParams p = blank();
+  p.x = L.cS; p.nx = ncs; p.y = L.cS; p.w = L.wS; p.ny = ncs; p.out0 = L.k1;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, ncs, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.nx = nct; p.y = L.cT; p.w = L.wT; p.ny = nct; p.out0 = L.k2;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, nct, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.nx = nct; p.y = L.cS; p.w = L.wS; p.ny = ncs; p.out0 = L.k3;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, nct, 0, L.pw, L.pw_bytes))) return rc;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.wS, ncs, 1, L.p1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.wT, nct, 1, L.p2);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, L.wT, nct, 1, L.p3);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, st, L.p2, 1.0, L.p1, 1.0, L.p3, -2.0, match);
+  if ((rc = launched("match reduction"))) return rc;
+  return report_bad(st, L.bad, "cell");
+}
+
+template <int D>
+int volume_match_gradient(hipStream_t st, double sigma, int prec, const VolumeLayout &L,
+                          const double *vs, long nvs, const int *cs, long ncs,
+                          const int *offsets, const int *incidences, const double *vt, long nvt,
+                          const int *ct, long nct, double *grad) {
+  int rc = volume_geometry<D>(st, L, vs, nvs, cs, ncs, vt, nvt, ct, nct);
+  if (rc) return rc;
+  // (K V)(c_S) and ConvolveGradient(c_S, V_S) from the template and from the target
+  // (OrientedVolumeMesh.cxx:186-194)
+  KParams p = blank();
+  p.x = L.cS; p.ta = L.wS; p.nx = ncs; p.y = L.cS; p.w = L.wS; p.ny = ncs;
+  p.out0 = L.k1; p.out1 = L.g1;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, 1, sigma, p, ncs, ncs * D, L.pw, L.pw_bytes)))
+    return rc;
+  p.y = L.cT; p.w = L.wT; p.ny = nct; p.out0 = L.k2; p.out1 = L.g2;
+  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, 1, sigma, p, ncs, ncs * D, L.pw, L.pw_bytes)))
+    return rc;
+  if (ncs > 0)
+    hipLaunchKernelGGL(k_vol_contrib<D>, dim3(blocks_for(ncs)), dim3(256), 0, st, vs, nvs, cs,
+                       ncs, L.k1, L.g1, L.k2, L.g2, L.contrib);
+  if (nvs > 0)
+    hipLaunchKernelGGL(k_vertex_gather<D>, dim3(blocks_for(nvs)), dim3(256), 0, st, offsets,
+                       incidences, nvs, (D + 1) * ncs, 1, L.contrib, 1.0, grad);
+  if ((rc = launched("gradient assembly"))) return rc;
+  return report_bad(st, L.bad, "cell");
+}
+
 }  // namespace
 
 int surf_geometry(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
@@ -690,7 +879,7 @@ extern "C" {
 int dfm_cell_incidence(const int *cells, long ncells, int corners, long nv, int *offsets,
                        int *incidences) {
   clear_error();
-  if (corners < 1 || corners > 3) return set_error(DFM_ERR_ARG, "cells of %d corners", corners);
+  if (corners < 1 || corners > 4) return set_error(DFM_ERR_ARG, "cells of %d corners", corners);
   if (ncells < 0 || nv < 0 || nv > 0x7fffffffL || ncells > 0x7fffffffL / corners)
     return set_error(DFM_ERR_ARG, "bad counts");
   if ((ncells > 0 && (!cells || !incidences)) || !offsets)
@@ -830,6 +1019,48 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, c
                        incidences, nvs, 3 * nfs, 1, L.contrib, 1.0, grad);
   if ((rc = launched("gradient assembly"))) return rc;
   return report_bad(st, L);
+}
+
+size_t dfm_volume_workspace(int dim, long ncs, long nct, int precision, int gradient) {
+  if (check_volume_args(dim, precision, 1.0, 0, ncs, 0, nct)) return 0;
+  return VolumeLayout(nullptr, dim, precision, ncs, nct, gradient != 0).total;
+}
+
+int dfm_volume_match(void *stream, int dim, double sigma, int prec, const double *vs, long nvs,
+                     const int *cs, long ncs, const double *vt, long nvt, const int *ct, long nct,
+                     double *match, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_volume_args(dim, prec, sigma, nvs, ncs, nvt, nct);
+  if (rc) return rc;
+  if (!match || (ncs > 0 && (!vs || !cs)) || (nct > 0 && (!vt || !ct)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  VolumeLayout need(nullptr, dim, prec, ncs, nct, false);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  VolumeLayout L((char *)work, dim, prec, ncs, nct, false);
+  hipStream_t st = (hipStream_t)stream;
+  return dim == 2 ? volume_match<2>(st, sigma, prec, L, vs, nvs, cs, ncs, vt, nvt, ct, nct, match)
+                  : volume_match<3>(st, sigma, prec, L, vs, nvs, cs, ncs, vt, nvt, ct, nct, match);
+}
+
+int dfm_volume_match_gradient(void *stream, int dim, double sigma, int prec, const double *vs,
+                              long nvs, const int *cs, long ncs, const int *offsets,
+                              const int *incidences, const double *vt, long nvt, const int *ct,
+                              long nct, double *grad, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_volume_args(dim, prec, sigma, nvs, ncs, nvt, nct);
+  if (rc) return rc;
+  if (!grad || !offsets || (ncs > 0 && (!vs || !cs || !incidences)) || (nct > 0 && (!vt || !ct)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  VolumeLayout need(nullptr, dim, prec, ncs, nct, true);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  VolumeLayout L((char *)work, dim, prec, ncs, nct, true);
+  hipStream_t st = (hipStream_t)stream;
+  return dim == 2 ? volume_match_gradient<2>(st, sigma, prec, L, vs, nvs, cs, ncs, offsets,
+                                             incidences, vt, nvt, ct, nct, grad)
+                  : volume_match_gradient<3>(st, sigma, prec, L, vs, nvs, cs, ncs, offsets,
+                                             incidences, vt, nvt, ct, nct, grad);
 }
 
 size_t dfm_samples_workspace(int kind, long ns, long nt, int precision, int gradient) {

@@ -1,4 +1,5 @@
-"""ASCII legacy VTK POLYDATA: the meshes and polylines the reference's landmark objects hold.
+"""ASCII legacy VTK POLYDATA and UNSTRUCTURED_GRID: the meshes, polylines and volume meshes
+the reference's landmark objects hold.
 
 The reference reads them with VTK's vtkPolyDataReader (DeformableObjectReader.cxx:58-145)
 into vtkPoints of the file's declared type: ``POINTS n float`` coordinates are single
@@ -12,7 +13,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-_CELL_SECTIONS = ("VERTICES", "LINES", "POLYGONS", "TRIANGLE_STRIPS")
+_CELL_SECTIONS = ("VERTICES", "LINES", "POLYGONS", "TRIANGLE_STRIPS", "CELLS")
 _TYPES = {"float": np.float32, "double": np.float64, "int": np.int32, "long": np.int64,
           "short": np.int16, "unsigned_char": np.uint8, "char": np.int8,
           "unsigned_int": np.uint32, "unsigned_short": np.uint16, "unsigned_long": np.uint64}
@@ -36,6 +37,15 @@ class PolyData:
         segs = [(c[i], c[i + 1]) for c in self.cells.get("LINES", []) for i in range(len(c) - 1)]
         return np.array(segs, dtype=np.int32).reshape(-1, 2)
 
+    def volume_cells(self, dimension):
+        """(F, dimension + 1) int32 cells of a volume mesh: the triangles of a 2D mesh, or the
+        tetrahedra (unstructured-grid CELLS of 4 points) of a 3D one."""
+        key = "CELLS" if "CELLS" in self.cells else "POLYGONS"
+        cl = self.cells.get(key, [])
+        if any(len(c) != dimension + 1 for c in cl):
+            raise ValueError("Not an admissible cell!")
+        return np.array(cl, dtype=np.int32).reshape(-1, dimension + 1)
+
     def coordinates(self, dimension):
         """The first ``dimension`` coordinates of each point, as the reference's 2D objects
         take them."""
@@ -49,8 +59,8 @@ def read_vtk(path):
         raise ValueError(f"{path}: not a legacy VTK file")
     if lines[2].strip().upper() != "ASCII":
         raise ValueError(f"{path}: only ASCII legacy VTK files are read")
-    if lines[3].split()[:2] != ["DATASET", "POLYDATA"]:
-        raise ValueError(f"{path}: not a POLYDATA dataset")
+    if lines[3].split()[:2] not in (["DATASET", "POLYDATA"], ["DATASET", "UNSTRUCTURED_GRID"]):
+        raise ValueError(f"{path}: not a POLYDATA or UNSTRUCTURED_GRID dataset")
     tok = " ".join(lines[4:]).split()
     i = 0
     points = None
@@ -80,6 +90,8 @@ def read_vtk(path):
                 raise ValueError(f"{path}: {key} size mismatch")
             cells[key] = out
             i += 3 + size
+        elif key == "CELL_TYPES":
+            i += 2 + int(tok[i + 1])
         elif key in ("POINT_DATA", "CELL_DATA", "FIELD", "METADATA"):
             break
         else:

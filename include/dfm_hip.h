@@ -405,7 +405,7 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int precisi
  * p1 - p0), DFM_VARIFOLD is NonOrientedPolyLine (tangents scaled by |p1 - p0|^-1/2,
  * upper-triangular T T^T weights). */
 
-/* Vertex -> corner incidence list of any cell list of `corners` (1..3) points per cell
+/* Vertex -> corner incidence list of any cell list of `corners` (1..4) points per cell
  * (host pointers): offsets (nv + 1), incidences (corners * ncells) holding
  * corners * cell + corner, grouped by vertex in increasing corner order.  corners = 2 gives
  * the segment list dfm_curve_match_gradient gathers with, in place of the reference's
@@ -436,6 +436,33 @@ int dfm_curve_match_gradient(void *stream, int kind, int dim, double sigma, int 
                              const int *offsets, const int *incidences, const double *vt,
                              long nvt, const int *et, long net, double *grad, void *work,
                              size_t work_bytes);
+
+/* ---- volume-mesh data terms (device pointers) --------------------------------------------
+ * OrientedVolumeMesh: points (nv x dim, dim = 2 or 3) and cells of dim + 1 points (int32;
+ * triangles in 2D, tetrahedra in 3D), each a Dirac at its centre weighted by its volume
+ * |det(p1 - p0, ..., pD - p0)| / D! (OrientedVolumeMesh.cxx:297-334). */
+
+/* Workspace bytes for a volume match (gradient = 0) or match gradient (gradient = 1). */
+size_t dfm_volume_workspace(int dim, long ncs, long nct, int precision, int gradient);
+
+/* *match (device, one double) = |S|^2 + |T|^2 - 2 <S, T>.  Replaces
+ * OrientedVolumeMesh::ComputeMatch (OrientedVolumeMesh.cxx:114-152).  DFM_ERR_DIM for dim
+ * outside {2, 3}; DFM_ERR_ARG for a cell index outside [0, nv). */
+int dfm_volume_match(void *stream, int dim, double sigma, int precision, const double *vs,
+                     long nvs, const int *cs, long ncs, const double *vt, long nvt,
+                     const int *ct, long nct, double *match, void *work, size_t work_bytes);
+
+/* grad (device, nvs x dim) with the template's point -> corner incidence list from
+ * dfm_cell_incidence (corners = dim + 1; device copies).  Replaces
+ * OrientedVolumeMesh::ComputeMatchGradient (OrientedVolumeMesh.cxx:156-286) as written: the
+ * determinant's cofactor vectors carry no sign of the determinant (the derivative of the
+ * match for positively oriented cells), and the 3D corner factor is the single-precision
+ * 1/3 of the reference. */
+int dfm_volume_match_gradient(void *stream, int dim, double sigma, int precision,
+                              const double *vs, long nvs, const int *cs, long ncs,
+                              const int *offsets, const int *incidences, const double *vt,
+                              long nvt, const int *ct, long nct, double *grad, void *work,
+                              size_t work_bytes);
 
 /* Data term between two sets of oriented samples (centre c, area-weighted normal N; the
  * reference's m_Centers / m_Normals of a mesh, OrientedSurfaceMesh.cxx:298-326 and
