@@ -307,7 +307,8 @@ class HIPMultiObjectAtlasModel:
     every kernel sum on the GPU: each subject's shooting, flow and the reference's continuous
     adjoint through HIPDiffeos (Diffeos::Update, ::IntegrateAdjointEquations), the data
     terms through the objects' own ComputeMatch / ComputeMatchGradient (HIPOrientedPolyLine,
-    HIPNonOrientedPolyLine, HIPOrientedSurfaceMesh, HIPNonOrientedSurfaceMesh), the RKHS
+    HIPNonOrientedPolyLine, HIPOrientedSurfaceMesh, HIPNonOrientedSurfaceMesh,
+    HIPOrientedVolumeMesh), the RKHS
     regularity and the template smoothing through HIPExactKernel.
 
     ``objects`` are the template's data-term objects (their kernel widths set),
