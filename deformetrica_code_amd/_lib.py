@@ -141,9 +141,20 @@ def load():
     L.dfm_volume_match.argtypes = [vp, i, d, i, vp, l, vp, l, vp, l, vp, l, vp, vp, sz]
     L.dfm_volume_match_gradient.argtypes = [vp, i, d, i, vp, l, vp, l, vp, vp, vp, l, vp, l, vp,
                                             vp, sz]
+    # point-cloud and landmark data terms
+    L.dfm_pointcloud_workspace.argtypes = [i, l, l, i, i]
+    L.dfm_pointcloud_workspace.restype = sz
+    L.dfm_pointcloud_match.argtypes = [vp, i, d, i, vp, vp, l, vp, vp, l, vp, vp, sz]
+    L.dfm_pointcloud_match_gradient.argtypes = [vp, i, d, i, vp, vp, l, vp, vp, l, vp, vp, sz]
+    L.dfm_landmark_workspace.argtypes = [i, l]
+    L.dfm_landmark_workspace.restype = sz
+    L.dfm_landmark_match.argtypes = [vp, i, vp, vp, l, vp, vp, sz]
+    L.dfm_landmark_match_gradient.argtypes = [vp, i, vp, vp, l, vp]
     for name in ("dfm_mesh_incidence", "dfm_surface_match", "dfm_surface_match_gradient",
                  "dfm_samples_match_gradient", "dfm_cell_incidence", "dfm_curve_match",
-                 "dfm_curve_match_gradient", "dfm_volume_match", "dfm_volume_match_gradient"):
+                 "dfm_curve_match_gradient", "dfm_volume_match", "dfm_volume_match_gradient",
+                 "dfm_pointcloud_match", "dfm_pointcloud_match_gradient", "dfm_landmark_match",
+                 "dfm_landmark_match_gradient"):
         getattr(L, name).restype = i
     L.dfm_autotune.argtypes = [i]
     L.dfm_autotune.restype = i

@@ -308,7 +308,7 @@ class HIPMultiObjectAtlasModel:
     adjoint through HIPDiffeos (Diffeos::Update, ::IntegrateAdjointEquations), the data
     terms through the objects' own ComputeMatch / ComputeMatchGradient (HIPOrientedPolyLine,
     HIPNonOrientedPolyLine, HIPOrientedSurfaceMesh, HIPNonOrientedSurfaceMesh,
-    HIPOrientedVolumeMesh), the RKHS
+    HIPOrientedVolumeMesh, HIPPointCloud, HIPLandmark), the RKHS
     regularity and the template smoothing through HIPExactKernel.
 
     ``objects`` are the template's data-term objects (their kernel widths set),

@@ -1,6 +1,7 @@
-// surfaces.hip — triangle-surface, polyline and volume-mesh data terms on the GPU:
-// currents (OrientedSurfaceMesh, OrientedPolyLine), varifolds (NonOrientedSurfaceMesh,
-// NonOrientedPolyLine) and volume measures (OrientedVolumeMesh), their match values and
+// surfaces.hip — triangle-surface, polyline, volume-mesh, point-cloud and landmark data terms
+// on the GPU: currents (OrientedSurfaceMesh, OrientedPolyLine), varifolds
+// (NonOrientedSurfaceMesh, NonOrientedPolyLine), volume measures (OrientedVolumeMesh),
+// weighted Diracs (PointCloud) and squared distances (Landmark), their match values and
 // their gradients with respect to the deformed template's vertices.  The polyline path (segments in 2D or 3D) follows the same plan with
 // k_seg_geometry / k_seg_contrib in place of the face kernels.
 //
@@ -789,6 +790,131 @@ int volume_match_gradient(hipStream_t st, double sigma, int prec, const VolumeLa
   return report_bad(st, L.bad, "cell");
 }
 
+// ---- point clouds and landmarks -----------------------------------------------------------
+
+// out = 2 (g1 + g2): PointCloud's (grad_SdotS + grad_SdotT) * 2 (PointCloud.cxx:189)
+__global__ __launch_bounds__(256) void k_twice_sum(const double *g1, const double *g2, long n,
+                                                   double *out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = (g1[i] + g2[i]) * 2.0;
+}
+
+// out = 2 (X - Y): Landmark::ComputeMatchGradient (Landmark.cxx:318-319)
+__global__ __launch_bounds__(256) void k_twice_diff(const double *X, const double *Y, long n,
+                                                    double *out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = (X[i] - Y[i]) * 2.0;
+}
+
+// partial[b] = sum over the block's entries of (X - Y)^2
+__global__ __launch_bounds__(256) void k_sqdiff_partial(const double *X, const double *Y, long n,
+                                                        double *partial) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double d = X[i] - Y[i];
+    s += d * d;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out = sum of kRedBlocks partials, in a fixed order
+__global__ __launch_bounds__(256) void k_total(const double *p, double *out) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  sh[t] = p[t];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) sh[t] += sh[t + st];
+    __syncthreads();
+  }
+  if (t == 0) *out = sh[0];
+}
+
+// Point clouds are Diracs at the points weighted by one scalar each: the kernel sums take
+// the caller's points and weights as they are, with no geometry pass.
+struct PointLayout {
+  double *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *g1 = nullptr, *g2 = nullptr;
+  double *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+  void *pw;
+  size_t pw_bytes, total;
+  PointLayout(char *base, int D, int prec, long ns, long nt, bool gradient) {
+    size_t off = 0;
+    auto take = [&](size_t count) {
+      double *p = base ? (double *)(base + off) : nullptr;
+      off += a256(count * sizeof(double));
+      return p;
+    };
+    if (gradient) {
+      g1 = take(D * ns); g2 = take(D * ns);
+      pw_bytes = std::max(pairsum_workspace(OP_GRAD1, prec, D, 1, ns, ns),
+                          pairsum_workspace(OP_GRAD1, prec, D, 1, ns, nt));
+    } else {
+      k1 = take(ns); k2 = take(nt); k3 = take(nt);
+      p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
+      pw_bytes = std::max({pairsum_workspace(OP_CONV, prec, D, 1, ns, ns),
+                           pairsum_workspace(OP_CONV, prec, D, 1, nt, nt),
+                           pairsum_workspace(OP_CONV, prec, D, 1, nt, ns)});
+    }
+    pw = base ? (void *)(base + off) : nullptr;
+    off += a256(pw_bytes);
+    total = off;
+  }
+};
+
+int check_point_args(int dim, int prec, double sigma, long ns, long nt) {
+  if (dim != 2 && dim != 3)
+    return set_error(DFM_ERR_DIM, "point clouds need dimension 2 or 3, got %d", dim);
+  return check_args(DFM_CURRENT, prec, sigma, ns, ns, nt, nt);
+}
+
+int pointcloud_match(hipStream_t st, int D, double sigma, int prec, const PointLayout &L,
+                     const double *xs, const double *ws, long ns, const double *xt,
+                     const double *wt, long nt, double *match) {
+  // |S|^2 = sum_i w_i (K w)(x_i) (UpdateSelfNorm, PointCloud.cxx:231-250), |T|^2 likewise,
+  // <S, T> = sum_j wt_j (K w)(xt_j) (ComputeMatch, :137-143)
+  KParams p = blank();
+  int rc;
+  p.x = xs; p.nx = ns; p.y = xs; p.w = ws; p.ny = ns; p.out0 = L.k1;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, ns, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = xt; p.nx = nt; p.y = xt; p.w = wt; p.ny = nt; p.out0 = L.k2;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, nt, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = xt; p.nx = nt; p.y = xs; p.w = ws; p.ny = ns; p.out0 = L.k3;
+  if ((rc = pairsum(st, OP_CONV, prec, D, 1, sigma, p, nt, 0, L.pw, L.pw_bytes))) return rc;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, ws, ns, 1, L.p1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, wt, nt, 1, L.p2);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, wt, nt, 1, L.p3);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, st, L.p2, 1.0, L.p1, 1.0, L.p3, -2.0, match);
+  return launched("match reduction");
+}
+
+int pointcloud_match_gradient(hipStream_t st, int D, double sigma, int prec,
+                              const PointLayout &L, const double *xs, const double *ws, long ns,
+                              const double *xt, const double *wt, long nt, double *grad) {
+  // ConvolveGradient(x, w) with the template's points and weights as sources, then with the
+  // target's, summed and doubled as the reference writes it (PointCloud.cxx:179-189): the
+  // cross term enters with +, so this is d/dx |S + T|^2, not d/dx |S - T|^2
+  KParams p = blank();
+  int rc;
+  p.x = xs; p.ta = ws; p.nx = ns; p.y = xs; p.w = ws; p.ny = ns; p.out0 = L.g1;
+  if ((rc = pairsum(st, OP_GRAD1, prec, D, 1, sigma, p, ns * D, 0, L.pw, L.pw_bytes))) return rc;
+  p.y = xt; p.w = wt; p.ny = nt; p.out0 = L.g2;
+  if ((rc = pairsum(st, OP_GRAD1, prec, D, 1, sigma, p, ns * D, 0, L.pw, L.pw_bytes))) return rc;
+  if (ns > 0)
+    hipLaunchKernelGGL(k_twice_sum, dim3(blocks_for(ns * D)), dim3(256), 0, st, L.g1, L.g2,
+                       ns * D, grad);
+  return launched("k_twice_sum");
+}
+
 }  // namespace
 
 int surf_geometry(hipStream_t st, int kind, const double *V, long nv, const int *F, long nf,
@@ -1061,6 +1187,80 @@ int dfm_volume_match_gradient(void *stream, int dim, double sigma, int prec, con
                                              incidences, vt, nvt, ct, nct, grad)
                   : volume_match_gradient<3>(st, sigma, prec, L, vs, nvs, cs, ncs, offsets,
                                              incidences, vt, nvt, ct, nct, grad);
+}
+
+size_t dfm_pointcloud_workspace(int dim, long ns, long nt, int precision, int gradient) {
+  if (check_point_args(dim, precision, 1.0, ns, nt)) return 0;
+  return PointLayout(nullptr, dim, precision, ns, nt, gradient != 0).total;
+}
+
+int dfm_pointcloud_match(void *stream, int dim, double sigma, int prec, const double *xs,
+                         const double *ws, long ns, const double *xt, const double *wt, long nt,
+                         double *match, void *work, size_t work_bytes) {
+  clear_error();
+  int rc = check_point_args(dim, prec, sigma, ns, nt);
+  if (rc) return rc;
+  if (!match || (ns > 0 && (!xs || !ws)) || (nt > 0 && (!xt || !wt)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  PointLayout need(nullptr, dim, prec, ns, nt, false);
+  if (!work || work_bytes < need.total)
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  PointLayout L((char *)work, dim, prec, ns, nt, false);
+  return pointcloud_match((hipStream_t)stream, dim, sigma, prec, L, xs, ws, ns, xt, wt, nt,
+                          match);
+}
+
+int dfm_pointcloud_match_gradient(void *stream, int dim, double sigma, int prec,
+                                  const double *xs, const double *ws, long ns, const double *xt,
+                                  const double *wt, long nt, double *grad, void *work,
+                                  size_t work_bytes) {
+  clear_error();
+  int rc = check_point_args(dim, prec, sigma, ns, nt);
+  if (rc) return rc;
+  if ((ns > 0 && (!xs || !ws || !grad)) || (nt > 0 && (!xt || !wt)))
+    return set_error(DFM_ERR_ARG, "NULL argument");
+  PointLayout need(nullptr, dim, prec, ns, nt, true);
+  if (need.total > 0 && (!work || work_bytes < need.total))  // an empty template needs none
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
+  PointLayout L((char *)work, dim, prec, ns, nt, true);
+  return pointcloud_match_gradient((hipStream_t)stream, dim, sigma, prec, L, xs, ws, ns, xt, wt,
+                                   nt, grad);
+}
+
+size_t dfm_landmark_workspace(int dim, long n) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffffL) return 0;
+  return a256(kRedBlocks * sizeof(double));
+}
+
+int dfm_landmark_match(void *stream, int dim, const double *x, const double *y, long n,
+                       double *match, void *work, size_t work_bytes) {
+  clear_error();
+  if (dim != 2 && dim != 3)
+    return set_error(DFM_ERR_DIM, "landmarks need dimension 2 or 3, got %d", dim);
+  if (n < 0 || n > 0x7fffffffL) return set_error(DFM_ERR_ARG, "counts must be in [0, 2^31)");
+  if (!match || (n > 0 && (!x || !y))) return set_error(DFM_ERR_ARG, "NULL argument");
+  if (!work || work_bytes < a256(kRedBlocks * sizeof(double)))
+    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes",
+                     a256(kRedBlocks * sizeof(double)));
+  hipStream_t st = (hipStream_t)stream;
+  // |X - Y|_F^2 (Landmark.cxx:298-303), a deterministic two-stage FP64 sum
+  hipLaunchKernelGGL(k_sqdiff_partial, dim3(kRedBlocks), dim3(256), 0, st, x, y, n * dim,
+                     (double *)work);
+  hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, (const double *)work, match);
+  return launched("landmark match");
+}
+
+int dfm_landmark_match_gradient(void *stream, int dim, const double *x, const double *y, long n,
+                                double *grad) {
+  clear_error();
+  if (dim != 2 && dim != 3)
+    return set_error(DFM_ERR_DIM, "landmarks need dimension 2 or 3, got %d", dim);
+  if (n < 0 || n > 0x7fffffffL) return set_error(DFM_ERR_ARG, "counts must be in [0, 2^31)");
+  if (n == 0) return DFM_OK;
+  if (!x || !y || !grad) return set_error(DFM_ERR_ARG, "NULL argument");
+  hipLaunchKernelGGL(k_twice_diff, dim3(blocks_for(n * dim)), dim3(256), 0, (hipStream_t)stream,
+                     x, y, n * dim, grad);
+  return launched("landmark gradient");
 }
 
 size_t dfm_samples_workspace(int kind, long ns, long nt, int precision, int gradient) {

@@ -4,8 +4,9 @@ the reference's landmark objects hold.
 The reference reads them with VTK's vtkPolyDataReader (DeformableObjectReader.cxx:58-145)
 into vtkPoints of the file's declared type: ``POINTS n float`` coordinates are single
 precision values, which the objects then hold as doubles.  ``read_vtk`` keeps that rounding
-so that a template read here starts from the same numbers.  Attributes (POINT_DATA,
-CELL_DATA, FIELD) are skipped.  Binary legacy files are refused.
+so that a template read here starts from the same numbers.  The point attributes SCALARS,
+VECTORS and NORMALS are kept (PointCloud takes its weights from the point scalars); any
+other attribute ends the read.  Binary legacy files are refused.
 """
 from __future__ import annotations
 
@@ -23,6 +24,20 @@ _TYPES = {"float": np.float32, "double": np.float64, "int": np.int32, "long": np
 class PolyData:
     points: np.ndarray                       # (n, 3) float64
     cells: dict = field(default_factory=dict)  # section -> list of index arrays
+    point_data: list = field(default_factory=list)  # (kind, name, (n, ncomp) float64)
+    point_data_complete: bool = True         # False when a point attribute was not parsed
+
+    def point_weights(self):
+        """One weight per point as PointCloud::UpdatePointWeights (PointCloud.cxx:203-226)
+        takes them: the point scalars when the point data hold one component in all, else 1
+        (a scalar count other than the point count cannot occur in a legacy file)."""
+        n = self.points.shape[0]
+        if not self.point_data_complete:
+            raise ValueError("point attributes this reader does not parse")
+        ncomp = sum(a.shape[1] for _, _, a in self.point_data)
+        if ncomp == 1 and self.point_data[0][0] == "SCALARS":
+            return self.point_data[0][2][:, 0].copy()
+        return np.ones(n)
 
     def triangles(self):
         """(F, 3) int32 vertex indices of the polygons, which must all be triangles."""
@@ -65,6 +80,7 @@ def read_vtk(path):
     i = 0
     points = None
     cells = {}
+    point_data = []
     while i < len(tok):
         key = tok[i].upper()
         if key == "POINTS":
@@ -92,7 +108,10 @@ def read_vtk(path):
             i += 3 + size
         elif key == "CELL_TYPES":
             i += 2 + int(tok[i + 1])
-        elif key in ("POINT_DATA", "CELL_DATA", "FIELD", "METADATA"):
+        elif key == "POINT_DATA":
+            i = _point_data(path, tok, i, point_data)
+            break
+        elif key in ("CELL_DATA", "FIELD", "METADATA"):
             break
         else:
             raise ValueError(f"{path}: unexpected token {tok[i]!r}")
@@ -102,7 +121,41 @@ def read_vtk(path):
         for c in cl:
             if c.size and (c.min() < 0 or c.max() >= points.shape[0]):
                 raise ValueError(f"{path}: {key} index out of range")
-    return PolyData(points, cells)
+    complete = not (point_data and point_data[-1] is None)
+    return PolyData(points, cells, [a for a in point_data if a is not None], complete)
+
+
+def _point_data(path, tok, i, out):
+    """Parse the POINT_DATA section from tok[i]; appends (kind, name, values) per attribute
+    and None when an attribute this reader does not parse ends it."""
+    n = int(tok[i + 1])
+    i += 2
+    while i < len(tok):
+        key = tok[i].upper()
+        if key == "SCALARS":
+            name, typ = tok[i + 1], tok[i + 2].lower()
+            i += 3
+            ncomp = 1
+            if i < len(tok) and tok[i].isdigit():
+                ncomp = int(tok[i])
+                i += 1
+            if i < len(tok) and tok[i].upper() == "LOOKUP_TABLE":
+                i += 2
+        elif key in ("VECTORS", "NORMALS"):
+            name, typ, ncomp = tok[i + 1], tok[i + 2].lower(), 3
+            i += 3
+        else:
+            if key != "CELL_DATA":
+                out.append(None)
+            return i
+        if typ not in _TYPES:
+            raise ValueError(f"{path}: attribute type {typ!r}")
+        vals = np.array(tok[i:i + n * ncomp], dtype=np.float64)
+        if vals.size != n * ncomp:
+            raise ValueError(f"{path}: {key} {name} shorter than declared")
+        out.append((key, name, vals.astype(_TYPES[typ]).astype(np.float64).reshape(n, ncomp)))
+        i += n * ncomp
+    return i
 
 
 def write_vtk(path, points, triangles=None, lines=None, title="vtk output"):

@@ -1,5 +1,6 @@
 """Geodesic regression on the GPU: Deformetrica's Regression model (Regression.cxx) for a
-template of landmark objects (polylines and triangle surfaces), as the estimators see it.
+template of landmark objects (polylines, triangle surfaces, volume meshes, point clouds and
+landmarks), as the estimators see it.
 
 One geodesic is shot from the control points and the initial momenta; the template flows
 along it and each observation is matched at its time index of the time grid.  Every

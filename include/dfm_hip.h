@@ -464,6 +464,45 @@ int dfm_volume_match_gradient(void *stream, int dim, double sigma, int precision
                               long nvt, const int *ct, long nct, double *grad, void *work,
                               size_t work_bytes);
 
+/* ---- point-cloud and landmark data terms (device pointers) -------------------------------
+ * PointCloud: points (n x dim, dim = 2 or 3), each a Dirac weighted by one scalar (the
+ * VTK point scalars, else 1; PointCloud.cxx:203-226).  Landmark: corresponding points, no
+ * kernel. */
+
+/* Workspace bytes for a point-cloud match (gradient = 0) or match gradient (gradient = 1)
+ * between a template of ns points and a target of nt points. */
+size_t dfm_pointcloud_workspace(int dim, long ns, long nt, int precision, int gradient);
+
+/* *match (device, one double) = |S|^2 + |T|^2 - 2 <S, T> with |S|^2 = sum_i ws_i (K ws)(xs_i)
+ * and <S, T> = sum_j wt_j (K ws)(xt_j).  Replaces PointCloud::ComputeMatch
+ * (PointCloud.cxx:108-148; its print of the value is not made) with UpdateSelfNorm
+ * (:231-250).  DFM_ERR_DIM for dim outside {2, 3}. */
+int dfm_pointcloud_match(void *stream, int dim, double sigma, int precision, const double *xs,
+                         const double *ws, long ns, const double *xt, const double *wt, long nt,
+                         double *match, void *work, size_t work_bytes);
+
+/* grad (device, ns x dim) = 2 (ConvolveGradient(xs, ws) from (xs, ws) + ConvolveGradient(xs,
+ * ws) from (xt, wt)).  Replaces PointCloud::ComputeMatchGradient (PointCloud.cxx:152-194)
+ * as written: the cross term enters with +, so the result is d/dxs |S + T|^2, not the
+ * derivative of the match. */
+int dfm_pointcloud_match_gradient(void *stream, int dim, double sigma, int precision,
+                                  const double *xs, const double *ws, long ns, const double *xt,
+                                  const double *wt, long nt, double *grad, void *work,
+                                  size_t work_bytes);
+
+/* Workspace bytes for dfm_landmark_match (0 for dim outside {2, 3} or a bad count). */
+size_t dfm_landmark_workspace(int dim, long n);
+
+/* *match (device, one double) = |X - Y|_F^2 of n corresponding points.  Replaces
+ * Landmark::ComputeMatch (Landmark.cxx:287-304). */
+int dfm_landmark_match(void *stream, int dim, const double *x, const double *y, long n,
+                       double *match, void *work, size_t work_bytes);
+
+/* grad (device, n x dim) = 2 (X - Y).  Replaces Landmark::ComputeMatchGradient
+ * (Landmark.cxx:306-322). */
+int dfm_landmark_match_gradient(void *stream, int dim, const double *x, const double *y, long n,
+                                double *grad);
+
 /* Data term between two sets of oriented samples (centre c, area-weighted normal N; the
  * reference's m_Centers / m_Normals of a mesh, OrientedSurfaceMesh.cxx:298-326 and
  * NonOrientedSurfaceMesh.cxx:269-308, given directly): *match (device double) =

@@ -13,8 +13,10 @@ pinned pieces; the GPU tests check the log-likelihood's gradient against central
 differences as well.
 
 An object is (kind, cells, attachment_width, data_sigma_squared): kind "current" or
-"varifold", cells (f x 3) triangles or (e x 2) segments.  A subject's target is one
-(points, cells) pair per object.
+"varifold" with cells (f x 3) triangles or (e x 2) segments, "pointcloud" with the
+template's point weights in place of the cells, or "landmark" (cells and width unused).  A
+subject's target is one (points, cells) pair per object, the cells being the target's point
+weights for a point cloud.
 """
 from __future__ import annotations
 
@@ -22,12 +24,29 @@ import numpy as np
 
 from . import curves as oc
 from . import deformations as dfo
+from . import points as po
 from . import surfaces as so
 from .oracle import ExactKernel
 
 
 def _terms(cells):
     return so if np.asarray(cells).shape[1] == 3 else oc
+
+
+def _match(kind, cells, x, tv, tf, width):
+    if kind == "pointcloud":
+        return po.pointcloud_match(x, cells, tv, tf, width)
+    if kind == "landmark":
+        return po.landmark_match(x, tv)
+    return getattr(_terms(cells), kind + "_match")(x, cells, tv, tf, width)
+
+
+def _match_gradient(kind, cells, x, tv, tf, width):
+    if kind == "pointcloud":
+        return po.pointcloud_match_gradient(x, cells, tv, tf, width)
+    if kind == "landmark":
+        return po.landmark_match_gradient(x, tv)
+    return getattr(_terms(cells), kind + "_match_gradient")(x, cells, tv, tf, width)
 
 
 def _split(X, counts):
@@ -45,7 +64,7 @@ def multi_loglik(cp, momenta, template, objects, targets, *, deformation_width, 
     for a, target in zip(momenta, targets):
         P, A = dfo.shoot(cp, a, deformation_width, T, scheme=scheme)
         X = dfo.flow(P, A, X0, deformation_width, T, improved_euler=improved_euler)
-        res.append([getattr(_terms(cells), kind + "_match")(x, cells, tv, tf, width)
+        res.append([_match(kind, cells, x, tv, tf, width)
                     for (kind, cells, width, _), x, (tv, tf)
                     in zip(objects, _split(X[-1], counts), target)])
     data = 0.0
@@ -81,7 +100,7 @@ def multi_gradient_reference(cp, momenta, template, objects, targets, *, deforma
         P, A = dfo.shoot(cp, a, deformation_width, T, scheme=scheme)
         X = dfo.flow(P, A, X0, deformation_width, T, improved_euler=improved_euler)
         theta = np.concatenate([
-            getattr(_terms(cells), kind + "_match_gradient")(x, cells, tv, tf, width) / (2.0 * s2)
+            _match_gradient(kind, cells, x, tv, tf, width) / (2.0 * s2)
             for (kind, cells, width, s2), x, (tv, tf) in zip(objects, _split(X[-1], counts), target)])
         xp, xm, th0 = dfo.adjoint_reference(P, A, X, theta, deformation_width, T,
                                             improved_euler=improved_euler)

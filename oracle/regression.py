@@ -24,7 +24,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import deformations as dfo
-from .atlas import _split, _terms, smooth_template_gradient
+from .atlas import _match, _match_gradient, _split, smooth_template_gradient
 from .oracle import ExactKernel
 
 
@@ -46,7 +46,7 @@ def loglik(cp, mom, template, objects, observations, time_indices, *, deformatio
     """(data term, regularity term, residuals (observations x objects))."""
     counts = [len(v) for v in template]
     _, _, X = _trajectory(cp, mom, template, deformation_width, T, t0, tN, improved_euler)
-    res = [[getattr(_terms(cells), kind + "_match")(x, cells, tv, tf, width)
+    res = [[_match(kind, cells, x, tv, tf, width)
             for (kind, cells, width, _), x, (tv, tf)
             in zip(objects, _split(X[ti], counts), obs)]
            for obs, ti in zip(observations, time_indices)]
@@ -70,7 +70,7 @@ def gradient_reference(cp, mom, template, objects, observations, time_indices, *
     counts = [len(v) for v in template]
     P, A, X = _trajectory(cp, mom, template, deformation_width, T, t0, tN, improved_euler)
     conditions = [np.concatenate([
-        getattr(_terms(cells), kind + "_match_gradient")(x, cells, tv, tf, width) / (2.0 * s2)
+        _match_gradient(kind, cells, x, tv, tf, width) / (2.0 * s2)
         for (kind, cells, width, s2), x, (tv, tf) in zip(objects, _split(X[ti], counts), obs)])
         for obs, ti in zip(observations, time_indices)]
     xp, xm, th0 = dfo.adjoint_reference(P, A, X, conditions, deformation_width, T, t0=t0, tN=tN,

@@ -1,0 +1,110 @@
+"""Oracle pins of the point-cloud and landmark data terms (CPU only), and the VTK point
+data that PointCloud takes its weights from.
+
+- match against a brute-force double sum over point pairs of |S - T|^2;
+- gradient as the reference writes it (PointCloud.cxx:189) against central differences of
+  the brute-force |S + T|^2, whose derivative it is; turning the cross term's sign gives
+  central differences of the match itself, which pins the cross term's own sign;
+- landmark match against numpy's Frobenius norm, its gradient against central differences.
+"""
+import numpy as np
+import pytest
+
+from oracle import points as po
+from deformetrica_code_amd.io.vtk import read_vtk
+
+
+def cloud(d, seed, n=23, m=19):
+    rng = np.random.default_rng(seed)
+    Xs = rng.uniform(0, 1, (n, d))
+    Xt = Xs[:m] + 0.1 * rng.standard_normal((m, d))
+    return Xs, rng.uniform(0.5, 2.0, n), Xt, rng.uniform(0.5, 2.0, m)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_pointcloud_match_equals_brute_force(d):
+    Xs, ws, Xt, wt = cloud(d, 10 + d)
+    for width in (0.2, 0.7):
+        ref = po.brute_force_pointcloud(Xs, ws, Xt, wt, width)
+        assert abs(po.pointcloud_match(Xs, ws, Xt, wt, width) - ref) <= 1e-12 * max(1.0, abs(ref))
+        n2 = po.brute_force_pointcloud(Xs, ws, Xs[:0], ws[:0], width)
+        assert abs(po.pointcloud_norm2(Xs, ws, width) - n2) <= 1e-12 * n2
+
+
+def central(f, X, h):
+    g = np.zeros_like(X)
+    for i in range(X.shape[0]):
+        for k in range(X.shape[1]):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[i, k] += h
+            Xm[i, k] -= h
+            g[i, k] = (f(Xp) - f(Xm)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_pointcloud_gradient_is_the_derivative_of_the_sum(d):
+    Xs, ws, Xt, wt = cloud(d, 20 + d)
+    width = 0.3
+    g = po.pointcloud_match_gradient(Xs, ws, Xt, wt, width)
+    fd_sum = central(lambda X: po.brute_force_pointcloud(X, ws, Xt, wt, width, sign=1.0), Xs, 1e-5)
+    assert np.max(np.abs(g - fd_sum)) <= 1e-6 * np.max(np.abs(fd_sum))
+    # the derivative of the match itself differs by twice the cross term
+    fd_match = central(lambda X: po.brute_force_pointcloud(X, ws, Xt, wt, width), Xs, 1e-5)
+    g_match = g - 4.0 * po.cross_gradient(Xs, ws, Xt, wt, width)
+    assert np.max(np.abs(g_match - fd_match)) <= 1e-6 * np.max(np.abs(fd_match))
+    assert np.max(np.abs(g - fd_match)) > 1e-2 * np.max(np.abs(fd_match))
+
+
+def test_pointcloud_empty_sets():
+    Xs, ws, Xt, wt = cloud(3, 5)
+    assert po.pointcloud_match(Xs, ws, Xt[:0], wt[:0], 0.4) == pytest.approx(
+        po.pointcloud_norm2(Xs, ws, 0.4), rel=1e-15)
+    assert po.pointcloud_match(Xs[:0], ws[:0], Xt, wt, 0.4) == pytest.approx(
+        po.pointcloud_norm2(Xt, wt, 0.4), rel=1e-15)
+    assert po.pointcloud_match_gradient(Xs[:0], ws[:0], Xt, wt, 0.4).shape == (0, 3)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_landmark_match_and_gradient(d):
+    rng = np.random.default_rng(30 + d)
+    X, Y = rng.standard_normal((11, d)), rng.standard_normal((11, d))
+    assert po.landmark_match(X, Y) == pytest.approx(np.linalg.norm(X - Y, "fro") ** 2, rel=1e-14)
+    fd = central(lambda Z: po.landmark_match(Z, Y), X, 1e-6)
+    assert np.max(np.abs(po.landmark_match_gradient(X, Y) - fd)) <= 1e-7
+
+
+VTK_HEAD = "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET POLYDATA\nPOINTS 3 float\n" \
+           "0 0 0 1 0 0 0 1 0\n"
+
+
+def write(tmp_path, body):
+    p = tmp_path / "cloud.vtk"
+    p.write_text(VTK_HEAD + body)
+    return str(p)
+
+
+def test_point_scalars_become_weights(tmp_path):
+    pd = read_vtk(write(tmp_path, "POINT_DATA 3\nSCALARS w float\nLOOKUP_TABLE default\n"
+                                  "0.1 2 3.5\n"))
+    w = pd.point_weights()
+    assert w.tolist() == [float(np.float32(0.1)), 2.0, 3.5]   # vtkFloatArray values
+    pd = read_vtk(write(tmp_path, "POINT_DATA 3\nSCALARS w double 1\nLOOKUP_TABLE default\n"
+                                  "0.1 2 3.5\n"))
+    assert pd.point_weights().tolist() == [0.1, 2.0, 3.5]
+
+
+def test_point_data_with_more_components_gives_unit_weights(tmp_path):
+    pd = read_vtk(write(tmp_path, "POINT_DATA 3\nSCALARS w float\nLOOKUP_TABLE default\n"
+                                  "0.1 2 3.5\nNORMALS n float\n1 0 0 0 1 0 0 0 1\n"))
+    assert pd.point_weights().tolist() == [1.0, 1.0, 1.0]
+    pd = read_vtk(write(tmp_path, "POINT_DATA 3\nSCALARS w float 2\nLOOKUP_TABLE default\n"
+                                  "1 2 3 4 5 6\n"))
+    assert pd.point_weights().tolist() == [1.0, 1.0, 1.0]
+    assert read_vtk(write(tmp_path, "")).point_weights().tolist() == [1.0, 1.0, 1.0]
+
+
+def test_unparsed_point_attributes_are_refused(tmp_path):
+    pd = read_vtk(write(tmp_path, "POINT_DATA 3\nFIELD FieldData 1\nf 1 3 float\n1 2 3\n"))
+    with pytest.raises(ValueError, match="does not parse"):
+        pd.point_weights()
