@@ -201,6 +201,13 @@ def curve_bounding_box(points, segments):
     return np.stack([c.min(axis=0), c.max(axis=0)], axis=1)
 
 
+def point_bounding_box(points):
+    """Bounding box of a Landmark or PointCloud object: over its points
+    (Landmark::UpdateBoundingBox, Landmark.cxx:371-387)."""
+    v = np.asarray(points, dtype=np.float64)
+    return np.stack([v.min(axis=0), v.max(axis=0)], axis=1)
+
+
 def union_bounding_box(boxes):
     """Bounding box of a multi-object template: the union of its objects' boxes
     (DeformableMultiObject::UpdateBoundingBox, DeformableMultiObject.cxx:410-429)."""

@@ -58,6 +58,14 @@ PARABOLA_STATE = os.path.join(STATES, "atlas_landmark_2d_parabola.bin")
 REGRESSION = os.path.join(FUNCTIONAL, "regression_skulls")
 REGRESSION_STATE = os.path.join(STATES, "regression_landmark_2d_skulls.bin")
 
+# examples/registration/landmark/2d/simple_point_cloud: Registration of four 2D points as a
+# Landmark object (object kernel width 30, unused by Landmark; data sigma 0.1), deformation
+# kernel width 20 (control-point spacing defaults to it), 10 time points, GradientAscent from
+# step 1 with the default settings.  The example keeps no output.
+SIMPLE_CLOUD = os.path.join(FUNCTIONAL, "simple_point_cloud")
+SIMPLE_CLOUD_PARAMS = dict(deformation_width=20.0, T=10, improved_euler=False)
+SIMPLE_CLOUD_ESTIMATOR = dict(initial_step_size=1.0)
+
 # model.xml / optimization_parameters.xml of the tests
 AMYG_PARAMS = dict(kind="current", deformation_width=10.0, attachment_width=10.0,
                    data_sigma_squared=0.1 ** 2, T=10, improved_euler=False)
@@ -129,6 +137,16 @@ def parabola():
     return [v for v, _ in tpl], [("current", e, 0.2, 1.0) for _, e in tpl], targets
 
 
+def simple_point_cloud():
+    """(template [points], objects [("landmark", None, 30, 0.1^2)], targets) of the
+    simple_point_cloud registration example."""
+    from deformetrica_code_amd.io import read_vtk
+    D = os.path.join(SIMPLE_CLOUD, "data")
+    src = read_vtk(os.path.join(D, "sourcePoints.vtk")).coordinates(2)
+    tgt = read_vtk(os.path.join(D, "targetPoints.vtk")).coordinates(2)
+    return [src], [("landmark", None, 30.0, 0.1 ** 2)], [[(tgt, None)]]
+
+
 def regression_skulls():
     """(template [points], objects, observations (one list of per-object targets per
     visit), time indices) of the skulls regression."""
@@ -153,9 +171,15 @@ def multi_initial_effects(template, objects, spacing, nsubjects):
     """Control points on the lattice over the union of the objects' bounding boxes, the
     template points, zero momenta."""
     from deformetrica_code_amd.atlas import (curve_bounding_box, initialize_control_points,
-                                             surface_bounding_box, union_bounding_box)
-    boxes = [(surface_bounding_box if np.asarray(c).shape[1] == 3 else curve_bounding_box)(v, c)
-             for v, (_, c, _, _) in zip(template, objects)]
+                                             point_bounding_box, surface_bounding_box,
+                                             union_bounding_box)
+
+    def box(v, kind, c):
+        if kind in ("landmark", "pointcloud"):
+            return point_bounding_box(v)
+        return (surface_bounding_box if np.asarray(c).shape[1] == 3 else curve_bounding_box)(v, c)
+
+    boxes = [box(v, k, c) for v, (k, c, _, _) in zip(template, objects)]
     cp0 = initialize_control_points(union_bounding_box(boxes), spacing)
     return ({"ControlPoints": cp0, "TemplateData": [np.array(v) for v in template]},
             {"Momenta": [np.zeros_like(cp0) for _ in range(nsubjects)]})

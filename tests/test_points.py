@@ -108,3 +108,19 @@ def test_unparsed_point_attributes_are_refused(tmp_path):
     pd = read_vtk(write(tmp_path, "POINT_DATA 3\nFIELD FieldData 1\nf 1 3 float\n1 2 3\n"))
     with pytest.raises(ValueError, match="does not parse"):
         pd.point_weights()
+
+
+def test_simple_point_cloud_example_on_the_oracle():
+    """The reference's simple_point_cloud registration example (a Landmark template) through
+    GradientAscent over the oracle model: control points on the 20-spaced lattice over the
+    points' box, the data term from -9e4 (two points 30 away, data sigma 0.1) above -50."""
+    import functional_cases as fc
+    from deformetrica_code_amd.estimators import GradientAscent
+    template, objects, targets = fc.simple_point_cloud()
+    p = fc.SIMPLE_CLOUD_PARAMS
+    fixed, ind = fc.multi_initial_effects(template, objects, p["deformation_width"], 1)
+    assert fixed["ControlPoints"].tolist() == [[20.0, 25.0], [40.0, 25.0], [60.0, 25.0]]
+    model = fc.OracleMultiObjectAtlasModel(objects, targets, p, fixed, freeze_template=True)
+    out = GradientAscent(model, **fc.SIMPLE_CLOUD_ESTIMATOR).Update({}, ind)
+    hist = [h for h in out["log_likelihood_history"] if np.size(h)]
+    assert hist[0][0] == pytest.approx(-9e4, rel=1e-12) and hist[-1][0] > -50.0
