@@ -66,6 +66,24 @@ SIMPLE_CLOUD = os.path.join(FUNCTIONAL, "simple_point_cloud")
 SIMPLE_CLOUD_PARAMS = dict(deformation_width=20.0, T=10, improved_euler=False)
 SIMPLE_CLOUD_ESTIMATOR = dict(initial_step_size=1.0)
 
+# Three more of the reference's examples (examples/), inputs in golden/examples; none keeps
+# an output.
+EXAMPLES = os.path.join(HERE, "golden", "examples")
+# registration/landmark/2d/starfish: Registration of a NonOrientedPolyline (kernel width 30,
+# data sigma 0.1), deformation kernel width 40, 10 time points, GradientAscent from step
+# 1e-3.  The model names data/starfish2.vtk for the file Starfish2.vtk.
+STARFISH_PARAMS = dict(deformation_width=40.0, T=10, improved_euler=False)
+STARFISH_ESTIMATOR = dict(initial_step_size=1e-3)
+# registration/landmark/3d/bundles_cortico: Registration of a NonOrientedPolyline fibre
+# bundle (kernel width 11, data sigma 1) and a NonOrientedSurfaceMesh putamen (kernel width
+# 11, data sigma 0.1), deformation kernel width 15, 10 time points, GradientAscent from step
+# 1e-5.
+BUNDLES_PARAMS = dict(deformation_width=15.0, T=10, improved_euler=False)
+BUNDLES_ESTIMATOR = dict(initial_step_size=1e-5)
+# shooting/landmark/2d/grid: Shooting of 24 control points with given momenta (exact kernel
+# of width 500, 20 time points) and the flow of the 783 grid points.
+GRID_PARAMS = dict(deformation_width=500.0, T=20)
+
 # model.xml / optimization_parameters.xml of the tests
 AMYG_PARAMS = dict(kind="current", deformation_width=10.0, attachment_width=10.0,
                    data_sigma_squared=0.1 ** 2, T=10, improved_euler=False)
@@ -145,6 +163,33 @@ def simple_point_cloud():
     src = read_vtk(os.path.join(D, "sourcePoints.vtk")).coordinates(2)
     tgt = read_vtk(os.path.join(D, "targetPoints.vtk")).coordinates(2)
     return [src], [("landmark", None, 30.0, 0.1 ** 2)], [[(tgt, None)]]
+
+
+def starfish():
+    """(template [points], objects, targets) of the starfish registration example."""
+    D = os.path.join(EXAMPLES, "starfish", "data")
+    V, E = _polyline(os.path.join(D, "Starfish2.vtk"))
+    return [V], [("varifold", E, 30.0, 0.1 ** 2)], [[_polyline(os.path.join(D, "Starfish1.vtk"))]]
+
+
+def bundles_cortico():
+    """(template [bundle, putamen], objects, targets) of the bundles_cortico example."""
+    from deformetrica_code_amd.io import read_vtk
+    D = os.path.join(EXAMPLES, "bundles_cortico", "data")
+    b, p = (read_vtk(os.path.join(D, f)) for f in ("bundle_prototype.vtk", "putamen_prototype.vtk"))
+    tb, tp = (read_vtk(os.path.join(D, f)) for f in ("subject_bundle.vtk", "subject_putamen.vtk"))
+    objects = [("varifold", b.segments(), 11.0, 1.0), ("varifold", p.triangles(), 11.0, 0.1 ** 2)]
+    targets = [[(tb.coordinates(3), tb.segments()), (tp.coordinates(3), tp.triangles())]]
+    return [b.coordinates(3), p.coordinates(3)], objects, targets
+
+
+def grid():
+    """(control points, momenta, grid points) of the grid shooting example."""
+    from deformetrica_code_amd.io import read_vtk
+    D = os.path.join(EXAMPLES, "grid", "data")
+    cp = np.loadtxt(os.path.join(D, "INITIALISATIONCP.txt"))
+    mom = np.loadtxt(os.path.join(D, "INITIALISATIONMOM.txt"))
+    return cp, mom, read_vtk(os.path.join(D, "grilleCP.vtk")).coordinates(2)
 
 
 def regression_skulls():
