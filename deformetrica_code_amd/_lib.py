@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG, "libdfm_hip.so")
 
 DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, DFM_ERR_NOMEM = range(7)
 FP64, FP32C, FP32X = 0, 1, 2
-OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP = range(7)
+OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP, OP_HESSIAN = range(8)
 EULER, HEUN = 0, 1
 CURRENT, VARIFOLD = 0, 1
 ADJOINT_DISCRETE, ADJOINT_CONTINUOUS = 0, 1
@@ -173,6 +173,7 @@ def load():
     L.dfm_varlin.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
     L.dfm_graddim.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, i, _P, _P, sz]
     L.dfm_graddiff.argtypes = [_P, i, i, d, i, _P, _P, _P, l, _P, _P, sz]
+    L.dfm_hessian.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
     # host-pointer wrappers
     L.dfm_gauss_conv.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
     L.dfm_gauss_grad1conv.argtypes = [i, i, d, _D, _D, _D, _D, _D, i, i, i]
@@ -180,9 +181,11 @@ def load():
     L.dfm_gauss_gradconv_varlin.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
     L.dfm_gauss_graddiffconv.argtypes = [i, i, d, _D, _D, _D, _D, i, i]
     L.dfm_gauss_conv_grad1.argtypes = [i, i, d, _D, _D, _D, _D, _D, _D, i, i, i]
+    L.dfm_gauss_hessianconv.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
     for name in ("dfm_conv", "dfm_grad1", "dfm_conv_grad1", "dfm_varlin", "dfm_graddim",
-                 "dfm_graddiff", "dfm_gauss_conv", "dfm_gauss_grad1conv", "dfm_gauss_gradconv",
-                 "dfm_gauss_gradconv_varlin", "dfm_gauss_graddiffconv", "dfm_gauss_conv_grad1"):
+                 "dfm_graddiff", "dfm_hessian", "dfm_gauss_conv", "dfm_gauss_grad1conv",
+                 "dfm_gauss_gradconv", "dfm_gauss_gradconv_varlin", "dfm_gauss_graddiffconv",
+                 "dfm_gauss_conv_grad1", "dfm_gauss_hessianconv"):
         getattr(L, name).restype = i
     _lib = L
     return L

@@ -203,7 +203,7 @@ int dfm_profile_read(double *total_ms, long *launches) {
 
 size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision) {
   if (check_common(dimpoint, dimvect, precision, 1.0, nx, ny) != DFM_OK) return 0;
-  if (op < DFM_OP_CONV || op > DFM_OP_SHOOT_VJP) return 0;
+  if (op < DFM_OP_CONV || op > DFM_OP_HESSIAN) return 0;
   if (op == DFM_OP_GRADDIFF || op == DFM_OP_SHOOT_VJP) ny = nx;
   if (nx == 0 || ny == 0) return 0;
   return make_plan(op, precision, dimpoint, dimvect, nx, ny).work_bytes;
@@ -264,6 +264,18 @@ int dfm_varlin(void *stream, int D, int DV, double sigma, int prec, const double
   KParams p = blank();
   p.x = x; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny;
   return run(stream, OP_VARLIN, D, DV, sigma, prec, p, nx * DV * D, 0, work, work_bytes);
+}
+
+int dfm_hessian(void *stream, int D, int DV, double sigma, int prec, const double *x, long nx,
+                const double *y, const double *beta, long ny, double *gamma, void *work,
+                size_t work_bytes) {
+  int rc = check_common(D, DV, prec, sigma, nx, ny);
+  if (!rc && nx > 0) rc = need(x, "x") ? DFM_ERR_ARG : need(gamma, "gamma");
+  if (!rc && nx > 0 && ny > 0) rc = need(y, "y") ? DFM_ERR_ARG : need(beta, "beta");
+  if (rc) return rc;
+  KParams p = blank();
+  p.x = x; p.y = y; p.w = beta; p.out0 = gamma; p.nx = nx; p.ny = ny;
+  return run(stream, OP_HESS, D, DV, sigma, prec, p, nx * DV * D * D, 0, work, work_bytes);
 }
 
 int dfm_graddim(void *stream, int D, int DV, double sigma, int prec, const double *x, long nx,
@@ -430,6 +442,25 @@ int dfm_gauss_gradconv_varlin(int D, int DV, double sigma, const double *x, cons
   TRY(dfm_varlin(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
                  (double *)db.p, ny, (double *)dg.p, dw.p, ws));
   TRY(download(gamma, dg, (size_t)nx * DV * D));
+  return finish();
+}
+
+int dfm_gauss_hessianconv(int D, int DV, double sigma, const double *x, const double *y,
+                          const double *beta, double *gamma, int nx, int ny, int prec) {
+  clear_error();
+  TRY(check_common(D, DV, prec, sigma, nx, ny));
+  if (nx == 0) return DFM_OK;
+  if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
+  DevBuf dx, dy, db, dg, dw;
+  TRY(upload(dx, x, (size_t)nx * D));
+  TRY(upload(dy, y, (size_t)ny * D));
+  TRY(upload(db, beta, (size_t)ny * DV));
+  TRY(dev_alloc(dg, (size_t)nx * DV * D * D * sizeof(double)));
+  size_t ws = dfm_workspace_size(DFM_OP_HESSIAN, D, DV, nx, ny, prec);
+  TRY(dev_alloc(dw, ws));
+  TRY(dfm_hessian(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
+                  (double *)db.p, ny, (double *)dg.p, dw.p, ws));
+  TRY(download(gamma, dg, (size_t)nx * DV * D * D));
   return finish();
 }
 

@@ -15,7 +15,8 @@ enum : int {
   OP_VARLIN = 3,
   OP_GRADDIM = 4,
   OP_GRADDIFF = 5,
-  OP_SHOOTVJP = 6
+  OP_SHOOTVJP = 6,
+  OP_HESS = 7
 };
 
 constexpr int kBBoxParts = 256;

@@ -187,6 +187,7 @@ static int targets_per_lane(int op, int prec, int D, int DV) {
     case OP_VARLIN: return DV * D > 9 ? 1 : 2;
     case OP_GRADDIFF: return 2;
     case OP_SHOOTVJP: return 1;
+    case OP_HESS: return 1;
     default: return 4;
   }
 }
@@ -201,6 +202,7 @@ static int nacc_of(int op, int D, int DV) {
     case OP_VARLIN: return DV * D;
     case OP_GRADDIM: return DV;
     case OP_GRADDIFF: return 2 * D;
+    case OP_HESS: return DV * (1 + D * (D + 1) / 2);
     default: return 5 * D;
   }
 }
@@ -449,7 +451,7 @@ static Choice tune_choose(const TuneKey &k, int op, int prec, int D, int DV, int
 int tune_report(char *buf, size_t len) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   static const char *ops[] = {"conv", "grad1", "conv_grad1", "varlin", "graddim", "graddiff",
-                              "shoot_vjp"};
+                              "shoot_vjp", "hessian"};
   std::string out;
   for (auto &kv : g_tune) {
     resolve_pending(kv.second);

@@ -97,6 +97,7 @@ struct Shape {
                               : OP == OP_VARLIN    ? DV * D
                               : OP == OP_GRADDIM   ? DV
                               : OP == OP_GRADDIFF  ? 2 * D
+                              : OP == OP_HESS      ? DV * (1 + D * (D + 1) / 2)
                                                    : 5 * D;
 };
 
@@ -152,6 +153,23 @@ __device__ __forceinline__ void pair_update(const T *xi, const T *xl, const T *t
     const T t = K * d[AUX];
 #pragma unroll
     for (int k = 0; k < DV; k++) acc[k] += t * w[k];
+  } else if constexpr (OP == OP_HESS) {
+    // per weight k: K w_k and K w_k d d^T (upper triangle); the Hessian's two terms are
+    // combined in FP64 at the end
+    constexpr int NQ = D * (D + 1) / 2;
+    T q[NQ];
+    int m = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++)
+#pragma unroll
+      for (int b = a; b < D; b++) q[m++] = d[a] * d[b];
+#pragma unroll
+    for (int k = 0; k < DV; k++) {
+      const T t = K * w[k];
+      acc[k * (NQ + 1)] += t;
+#pragma unroll
+      for (int j = 0; j < NQ; j++) acc[k * (NQ + 1) + 1 + j] += t * q[j];
+    }
   } else if constexpr (OP == OP_SHOOTVJP) {
     // transpose of the shooting field's Jacobian at (p, a) applied to (mu_p, mu_a):
     //   conv:  K mu_p_j                                   -> Convolve(mu_p)
@@ -236,6 +254,24 @@ __device__ __forceinline__ void finalize(const double *a, const KParams &p, long
   } else if constexpr (OP == OP_GRADDIFF) {
 #pragma unroll
     for (int k = 0; k < D; k++) p.out0[o * D + k] = p.ch1 * a[k] + p.ch2 * a[D + k];
+  } else if constexpr (OP == OP_HESS) {
+    // H_k = 4 / sigma^4 sum K w_k dx dx^T - 2 / sigma^2 sum K w_k I, with dx = d / s:
+    // 4 / (sigma^4 s^2) = -ch2 and 2 / sigma^2 = ch1; out0 (nx x DV x D x D), both triangles
+    constexpr int NQ = D * (D + 1) / 2;
+#pragma unroll
+    for (int k = 0; k < DV; k++) {
+      const double *h = a + k * (NQ + 1);
+      double *out = p.out0 + (o * DV + k) * D * D;
+      int m = 1;
+#pragma unroll
+      for (int r = 0; r < D; r++)
+#pragma unroll
+        for (int c = r; c < D; c++) {
+          const double v = -p.ch2 * h[m++] - (r == c ? p.ch1 * h[0] : 0.0);
+          out[r * D + c] = v;
+          out[c * D + r] = v;
+        }
+    }
   } else {  // OP_SHOOTVJP: out0 = J_p, out1 = J_a
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -558,6 +594,7 @@ static int launch_op(hipStream_t st, int op, int D, int DV, const KParams &p, co
     case OP_GRADDIM: return launch_d<T, MODE, OP_GRADDIM>(st, D, DV, p, pl);
     case OP_GRADDIFF: return launch_d<T, MODE, OP_GRADDIFF>(st, D, DV, p, pl);
     case OP_SHOOTVJP: return launch_d<T, MODE, OP_SHOOTVJP>(st, D, DV, p, pl);
+    case OP_HESS: return launch_d<T, MODE, OP_HESS>(st, D, DV, p, pl);
   }
   return set_error(DFM_ERR_ARG, "unknown op %d", op);
 }

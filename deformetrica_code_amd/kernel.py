@@ -4,7 +4,9 @@
 Deformetrica's ``CUDAExactKernel<double, D>`` (src/support/kernels/CUDAExactKernel.h:29-81),
 the exact-kernel type it replaces: the kernel owns copies of its sources and weights
 (AbstractKernel.h:93-108), clamps the width at 1e-10 (AbstractKernel.h:85-91), and the
-GPU ops are Convolve, the three ConvolveGradient overloads and ConvolveSpecialHessian.
+GPU ops are Convolve, the three ConvolveGradient overloads, the three ConvolveHessian
+overloads (which CUDAExactKernel inherits from ExactKernel on the CPU) and
+ConvolveSpecialHessian.
 Matrices are numpy float64 arrays (rows x cols).  Every op runs on the gfx950 library
 through its synchronous host-pointer entry points; nothing is computed in Python.
 """
@@ -137,6 +139,32 @@ class HIPExactKernel:
                                                     _p(self._W), _p(conv), _p(grad), n, ny,
                                                     self.precision))
         return conv, grad
+
+    def ConvolveHessian(self, X, *args):
+        """(X) -> (N, Dv, d, d), the list form; (X, row, col) -> (N, Dv); (X, k, dp, dq) ->
+        (N,) (ExactKernel.cxx:365-468, with its error messages).  One pass over the pairs
+        gives every entry; the overloads with indices are slices of it."""
+        self._check_rows()
+        X = self._targets(X)
+        n, dv, d, ny = X.shape[0], self._W.shape[1], self.dim, self._Y.shape[0]
+        if len(args) == 2 and (args[0] >= d or args[1] >= d or min(args) < 0):
+            raise RuntimeError("Dimension index out of bounds")
+        if len(args) == 3:
+            if not 0 <= args[0] < dv:
+                raise RuntimeError("Invalid weight index")
+            if not (0 <= args[1] < d and 0 <= args[2] < d):
+                raise RuntimeError("Invalid derivative direction")
+        if len(args) not in (0, 2, 3):
+            raise TypeError("ConvolveHessian(X), (X, row, col) or (X, k, dp, dq)")
+        out = np.empty((n, dv, d, d))
+        _lib.check(_lib.load().dfm_gauss_hessianconv(d, dv, self._h, _p(X), _p(self._Y),
+                                                     _p(self._W), _p(out), n, ny,
+                                                     self.precision))
+        if len(args) == 2:
+            return np.ascontiguousarray(out[:, :, args[0], args[1]])
+        if len(args) == 3:
+            return np.ascontiguousarray(out[:, args[0], args[1], args[2]])
+        return out
 
     # CUDAExactKernel.cxx:196-226
     def ConvolveSpecialHessian(self, xi) -> np.ndarray:

@@ -141,6 +141,16 @@ int dfm_gauss_graddiffconv(int dimpoint, int dimvect, double sigma, const double
                            const double *beta, const double *eta, double *gamma, int nx,
                            int precision);
 
+/* gamma (nx x dv x dp x dp), gamma[i][k] = sum_j beta_j[k] H_x K(x_i, y_j) with
+ * H_x K = K (4 (x - y)(x - y)^T / s^4 - 2 I / s^2) (ExactKernel.h:113-133); index
+ * ((i*dv + k)*dp + r)*dp + c, both triangles written.  Replaces
+ * ExactKernel::ConvolveHessian(X) (ExactKernel.cxx:365-397), which CUDAExactKernel
+ * inherits on the CPU; the (X, row, col) and (X, k, dp, dq) overloads (:399-468) are
+ * slices of it. */
+int dfm_gauss_hessianconv(int dimpoint, int dimvect, double sigma, const double *x,
+                          const double *y, const double *beta, double *gamma, int nx, int ny,
+                          int precision);
+
 /* Fused Convolve + ConvolveGradient(X, alpha) over one pass of the pairs: conv (nx x dv)
  * and grad (nx x dp); either output may be NULL.  No single reference counterpart: it
  * replaces the back-to-back GaussGpuEvalConv1D + GaussGpuGrad1Conv1D calls that the
@@ -162,7 +172,8 @@ enum dfm_op {
   DFM_OP_VARLIN = 3,
   DFM_OP_GRADDIM = 4,
   DFM_OP_GRADDIFF = 5,
-  DFM_OP_SHOOT_VJP = 6
+  DFM_OP_SHOOT_VJP = 6,
+  DFM_OP_HESSIAN = 7
 };
 
 /* Bytes of workspace the device entry point for `op` needs at these sizes. */
@@ -188,6 +199,10 @@ int dfm_varlin(void *stream, int dimpoint, int dimvect, double sigma, int precis
 int dfm_graddim(void *stream, int dimpoint, int dimvect, double sigma, int precision,
                 const double *x, long nx, const double *y, const double *beta, long ny,
                 int dim, double *gamma, void *work, size_t work_bytes);
+
+int dfm_hessian(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                const double *x, long nx, const double *y, const double *beta, long ny,
+                double *gamma, void *work, size_t work_bytes);
 
 int dfm_graddiff(void *stream, int dimpoint, int dimvect, double sigma, int precision,
                  const double *x, const double *beta, const double *eta, long n, double *gamma,
