@@ -220,15 +220,19 @@ struct Choice {
 // per lane (64-target wavefronts: the fewest registers, the most wavefronts per SIMD and
 // the tightest wavefront boxes); source splits aiming for 262144 wavefronts on grids of
 // >= 4096 target workgroups and 16384 below (the short tail of many small wavefronts
-// balances the CUs); splits of at least 256 FP32 sources (4 tiles), so that the split
-// reduction stays short on small grids, or 128 FP64 sources (one tile) on grids of fewer
-// than 1024 target workgroups.  At N = M = 1M the fused FP32C sum runs in 363 ms with this
+// balances the CUs); splits of at least 4 source tiles, so that the split reduction
+// stays short, except on the smallest grids (one tile: FP32 below 64 target workgroups,
+// FP64 below 1024).  At N = M = 1M the fused FP32C sum runs in 363 ms with this
 // plan (366 ms aiming for 65536 wavefronts, same box) against 483 ms with 4 targets per
 // lane, 4096 wavefronts and 256-source tiles.
 static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
   (void)op; (void)D; (void)DV;
   const long blocks = (max_nx + 63) / 64 * nseg;
-  const int min_tiles = (prec == DFM_FP64 && blocks < 1024) ? 1 : 4;
+  // at 1,000 x 1,000 (16 target workgroups) the fused FP32C sum takes 0.017 ms with
+  // one-tile splits against 0.030 ms with four-tile ones (FP64: 0.021 against 0.053 ms);
+  // from 64 workgroups up (a 4,096-point registration's sums) one-tile FP32 splits cost
+  // more in split partials than they gain: 16.4 against 11.8 ms for that gradient
+  const int min_tiles = (blocks < 64 || (prec == DFM_FP64 && blocks < 1024)) ? 1 : 4;
   return Choice{1, blocks >= 4096 ? 262144 : 16384, min_tiles};
 }
 
