@@ -51,7 +51,11 @@ enum dfm_status {
 
 enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2 };
 
-/* ABI version of this header (bumped on any signature change). */
+/* ABI version of this header (bumped on any signature or struct-layout change).
+ *   1  first release
+ *   2  dfm_atlas_desc gained dimension and cell_corners (appended); new entry points for
+ *      polylines, volume meshes, point clouds, landmarks, ConvolveHessian and the
+ *      adjoint's jump conditions; DFM_OP_HESSIAN */
 int dfm_abi_version(void);
 /* Last error message of the calling thread ("" when none). */
 const char *dfm_last_error(void);

@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.dfm_abi_version() == 1
+    assert lib.dfm_abi_version() == 2
 
 
 def test_argument_errors_without_gpu(lib):
