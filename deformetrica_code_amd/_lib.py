@@ -21,6 +21,9 @@ ADJOINTS = {"discrete": ADJOINT_DISCRETE, "continuous": ADJOINT_CONTINUOUS}
 
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X}
 
+# DFM_ABI_VERSION of include/dfm_hip.h, the header these bindings follow
+ABI_VERSION = 2
+
 
 class DfmError(RuntimeError):
     def __init__(self, code: int, message: str):
@@ -46,6 +49,9 @@ def load():
     i, l, d, sz = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
     L.dfm_abi_version.restype = i
     L.dfm_last_error.restype = ctypes.c_char_p
+    if L.dfm_abi_version() != ABI_VERSION:
+        raise RuntimeError("libdfm_hip.so has ABI version %d, these bindings expect %d; rebuild it"
+                           % (L.dfm_abi_version(), ABI_VERSION))
     L.dfm_workspace_size.argtypes = [i, i, i, l, l, i]
     L.dfm_workspace_size.restype = sz
     vp = ctypes.c_void_p

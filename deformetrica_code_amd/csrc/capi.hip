@@ -59,7 +59,7 @@ KParams blank() {
 
 extern "C" {
 
-int dfm_abi_version(void) { return 2; }
+int dfm_abi_version(void) { return DFM_ABI_VERSION; }
 
 const char *dfm_last_error(void) { return last_error(); }
 

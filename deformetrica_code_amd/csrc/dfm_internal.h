@@ -107,8 +107,10 @@ int fill(hipStream_t st, double *dst, long n, double v);
 
 // One pair-sum call on device pointers: plans the work, checks the workspace, clamps the
 // kernel width and launches (empty source sets give zero outputs).
+// `ws_label` names the workspace in the error of a call that does not fit it.
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
-            long out0_n, long out1_n, void *work, size_t work_bytes);
+            long out0_n, long out1_n, void *work, size_t work_bytes,
+            const char *ws_label = "workspace");
 size_t pairsum_workspace(int op, int prec, int D, int DV, long nx, long ny);
 
 // Segments of a ragged batch: host offsets (n + 1 entries, from 0, non-decreasing) and

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -359,15 +360,38 @@ Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, l
 
 // ------------------------------------------------------------------------------------
 // autotuner: per (op, precision, d, dv, batched, log2 of the largest target and source
-// counts), time every candidate twice on real calls (HIP events around the pair-sum
-// launch, read back at the next call of the same key) and keep the fastest.
+// counts, grid class), time every candidate twice on real calls (HIP events around the
+// pair-sum launch, read back at a later call of the same key) and keep the fastest.
+//
+// The grid class (target workgroups of the whole grid: < 64, < 1024, < 4096, >= 4096) is
+// part of the key because the candidate set and the default plan depend on it: a winner
+// tuned on a large batch is then never reused by a smaller batch of the same largest
+// segment, whose workspace (sized over its own candidates) would not hold it.
 
 struct TuneKey {
-  int op, prec, D, DV, batch, lx, ly;
+  int op, prec, D, DV, batch, lx, ly, grid;
   bool operator<(const TuneKey &o) const {
-    const int a[7] = {op, prec, D, DV, batch, lx, ly};
-    const int b[7] = {o.op, o.prec, o.D, o.DV, o.batch, o.lx, o.ly};
-    return std::lexicographical_compare(a, a + 7, b, b + 7);
+    const int a[8] = {op, prec, D, DV, batch, lx, ly, grid};
+    const int b[8] = {o.op, o.prec, o.D, o.DV, o.batch, o.lx, o.ly, o.grid};
+    return std::lexicographical_compare(a, a + 8, b, b + 8);
+  }
+};
+
+static int grid_class(int nseg, long max_nx) {
+  const long blocks = (max_nx + 63) / 64 * nseg;
+  return blocks < 64 ? 0 : blocks < 1024 ? 1 : blocks < 4096 ? 2 : 3;
+}
+
+// One timed launch of a candidate.  The events are created under the tuner's lock and
+// shared between the launch that records them and the entry that reads them, so neither
+// a concurrent call of the same key nor dfm_autotune_reset can free them under the other.
+struct TimedLaunch {
+  int cand = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::atomic<int> state{0};  // 0 being issued, 1 recorded, 2 launch failed
+  ~TimedLaunch() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
   }
 };
 
@@ -375,8 +399,7 @@ struct TuneEntry {
   std::vector<Choice> cand;
   std::vector<float> best_ms;
   int next = 0;
-  int pending = -1;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<std::shared_ptr<TimedLaunch>> pending;
   bool done = false;
   int winner = 0;
 };
@@ -399,39 +422,38 @@ void tune_set_mode(int mode) { g_tune_mode = mode; }
 
 void tune_reset() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  for (auto &kv : g_tune) {
-    if (kv.second.e0) (void)hipEventDestroy(kv.second.e0);
-    if (kv.second.e1) (void)hipEventDestroy(kv.second.e1);
-  }
-  g_tune.clear();
+  g_tune.clear();  // launches still in flight keep their own TimedLaunch alive
 }
 
+// Read back every timed launch of the entry that has been recorded (caller holds the lock).
 static void resolve_pending(TuneEntry &t) {
-  if (t.pending < 0) return;
-  if (!t.e1) {  // the timed launch recorded nothing (failed or still being issued)
-    t.pending = -1;
-    return;
+  std::vector<std::shared_ptr<TimedLaunch>> keep;
+  for (auto &tl : t.pending) {
+    const int st = tl->state.load();
+    if (st == 0) {  // another thread is still issuing it
+      keep.push_back(tl);
+      continue;
+    }
+    float ms = 0.f;
+    if (st == 1 && hipEventSynchronize(tl->e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, tl->e0, tl->e1) == hipSuccess)
+      t.best_ms[tl->cand] = std::min(t.best_ms[tl->cand], ms);
   }
-  float ms = 0.f;
-  if (hipEventSynchronize(t.e1) == hipSuccess && hipEventElapsedTime(&ms, t.e0, t.e1) == hipSuccess)
-    t.best_ms[t.pending] = std::min(t.best_ms[t.pending], ms);
-  (void)hipEventDestroy(t.e0);
-  (void)hipEventDestroy(t.e1);
-  t.e0 = t.e1 = nullptr;
-  t.pending = -1;
-  if (t.next >= (int)t.cand.size() * kTuneReps && !t.done) {
+  t.pending.swap(keep);
+  if (t.next >= (int)t.cand.size() * kTuneReps && t.pending.empty() && !t.done) {
     t.done = true;
     t.winner = (int)(std::min_element(t.best_ms.begin(), t.best_ms.end()) - t.best_ms.begin());
   }
 }
 
-// The plan choice for a call, and whether to time it (explore mode).
-static Choice tune_choose(const TuneKey &k, int op, int prec, int D, int DV, int nseg,
-                          long max_nx, TuneEntry **timed) {
-  *timed = nullptr;
+// The plan choice for a call and, in explore mode, the timed launch to record.
+static Choice tune_choose(int op, int prec, int D, int DV, int batch, int nseg, long max_nx,
+                          long max_ny, std::shared_ptr<TimedLaunch> *timed) {
+  timed->reset();
   const int mode = g_tune_mode;
   const Choice d = default_choice(op, prec, D, DV, nseg, max_nx);
   if (mode == 0) return d;
+  const TuneKey k{op, prec, D, DV, batch, ilog2(max_nx), ilog2(max_ny), grid_class(nseg, max_nx)};
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(k);
   if (it == g_tune.end()) {
@@ -444,12 +466,15 @@ static Choice tune_choose(const TuneKey &k, int op, int prec, int D, int DV, int
   TuneEntry &t = it->second;
   if (mode == 1) resolve_pending(t);
   if (t.done) return t.cand[t.winner];
-  if (mode == 2) return d;
-  const int c = t.next / kTuneReps;
-  t.pending = c;
+  if (mode == 2 || t.next >= (int)t.cand.size() * kTuneReps) return d;
+  auto tl = std::make_shared<TimedLaunch>();
+  tl->cand = t.next / kTuneReps;
+  if (hipEventCreate(&tl->e0) != hipSuccess || hipEventCreate(&tl->e1) != hipSuccess)
+    return d;  // untimed: this call runs the default plan
   t.next++;
-  *timed = &t;
-  return t.cand[c];
+  t.pending.push_back(tl);
+  *timed = tl;
+  return t.cand[tl->cand];
 }
 
 int tune_report(char *buf, size_t len) {
@@ -464,9 +489,10 @@ int tune_report(char *buf, size_t len) {
     char line[256];
     const Choice &w = t.cand[t.done ? t.winner : 0];
     snprintf(line, sizeof(line),
-             "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d -> R=%d waves=%d min_tiles=%d "
-             "mx=%d %s %.4f ms (default %.4f ms) |",
-             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, w.R, w.waves, w.min_tiles, w.mx,
+             "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d grid=%d -> R=%d waves=%d "
+             "min_tiles=%d mx=%d %s %.4f ms (default %.4f ms) |",
+             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, k.grid, w.R, w.waves, w.min_tiles,
+             w.mx,
              t.done ? "tuned" : "exploring", t.done ? t.best_ms[t.winner] : 0.0,
              t.best_ms.empty() ? 0.0 : t.best_ms[0]);
     out += line;
@@ -558,7 +584,7 @@ int seg_from_host(Seg &s, int n, const long *h, const long *d) {
 }
 
 static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
-                        const Plan &pl, char *work, TuneEntry *timed);
+                        const Plan &pl, char *work, TimedLaunch *timed);
 
 int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
                    const Plan &pl, char *work) {
@@ -566,7 +592,7 @@ int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
 }
 
 static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
-                        const Plan &pl, char *work, TuneEntry *timed) {
+                        const Plan &pl, char *work, TimedLaunch *timed) {
   const double sigma = p.sigma;
   // FP64: K = exp(-|x' - y'|^2) with x' = x / sigma; FP32: K = exp2(-|x' - y'|^2) with
   // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
@@ -582,31 +608,30 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
   p.partial = pl.nsplit > 1 ? (double *)(work + pl.off_partial) : nullptr;
   if (pl.sort) {
     int rc = sort_targets(st, p.x, p.nx, D, work, pl);
-    if (rc != DFM_OK) return rc;
+    if (rc != DFM_OK) {
+      if (timed) timed->state = 2;
+      return rc;
+    }
     p.perm = (const uint32_t *)(work + pl.off_perm);
   }
+  // the tuner's events (created by tune_choose) or, when profiling, a pair of our own
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (g_profile.on || timed) {
+  if (timed) {
+    ev0 = timed->e0;
+    ev1 = timed->e1;
+  } else if (g_profile.on) {
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
-    (void)hipEventRecord(ev0, st);
   }
+  if (ev0) (void)hipEventRecord(ev0, st);
   int rc = prec == DFM_FP64    ? launch_op_f64(st, op, D, DV, p, pl)
            : prec == DFM_FP32X ? launch_op_f32x(st, op, D, DV, p, pl)
                                : launch_op_f32c(st, op, D, DV, p, pl);
-  if (ev0) {
-    (void)hipEventRecord(ev1, st);
-    if (timed) {
-      // the tuner owns the events; profiling gets its own pair
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      timed->e0 = ev0;
-      timed->e1 = ev1;
-    } else {
-      g_profile.events.push_back({ev0, ev1});
-    }
-  }
+  hipError_t e = rc == DFM_OK ? hipGetLastError() : hipSuccess;
+  if (ev1) (void)hipEventRecord(ev1, st);
+  if (timed) timed->state = (rc == DFM_OK && e == hipSuccess) ? 1 : 2;
+  else if (ev0) g_profile.events.push_back({ev0, ev1});
   if (rc != DFM_OK) return rc;
-  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
   return DFM_OK;
 }
@@ -640,19 +665,20 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
   }
-  TuneEntry *timed = nullptr;
-  const Choice c = tune_choose(TuneKey{op, prec, D, DV, 1, ilog2(xs.max), ilog2(ys.max)}, op,
-                               prec, D, DV, xs.n, xs.max, &timed);
+  std::shared_ptr<TimedLaunch> timed;
+  const Choice c = tune_choose(op, prec, D, DV, 1, xs.n, xs.max, ys.max, &timed);
   Plan pl = batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max, c);
-  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
+  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes)) {
+    if (timed) timed->state = 2;
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
                      work ? work_bytes : (size_t)0);
+  }
   p.sigma = clamp_width(sigma);
-  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed);
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get());
 }
 
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
-            long out0_n, long out1_n, void *work, size_t work_bytes) {
+            long out0_n, long out1_n, void *work, size_t work_bytes, const char *ws_label) {
   if (p.nx == 0) return DFM_OK;
   if (p.ny == 0) {  // empty sums
     fill(st, p.out0, out0_n, 0.0);
@@ -660,15 +686,16 @@ int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KPara
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
   }
-  TuneEntry *timed = nullptr;
-  const Choice c = tune_choose(TuneKey{op, prec, D, DV, 0, ilog2(p.nx), ilog2(p.ny)}, op, prec,
-                               D, DV, 1, p.nx, &timed);
+  std::shared_ptr<TimedLaunch> timed;
+  const Choice c = tune_choose(op, prec, D, DV, 0, 1, p.nx, p.ny, &timed);
   Plan pl = single_plan(op, prec, D, DV, p.nx, p.ny, c);
-  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes))
-    return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes, got %zu", pl.work_bytes,
-                     work ? work_bytes : (size_t)0);
+  if (pl.work_bytes > 0 && (!work || work_bytes < pl.work_bytes)) {
+    if (timed) timed->state = 2;
+    return set_error(DFM_ERR_ARG, "%s too small: need %zu bytes, got %zu", ws_label,
+                     pl.work_bytes, work ? work_bytes : (size_t)0);
+  }
   p.sigma = clamp_width(sigma);
-  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed);
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get());
 }
 
 }  // namespace dfm

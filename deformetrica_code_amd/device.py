@@ -65,6 +65,8 @@ class Stream:
     def __init__(self):
         self._h = ctypes.c_void_p()
         _lib.check(_lib.load().dfm_stream_create(ctypes.byref(self._h)))
+        # the default workspace of calls ordered on this stream (see _ws)
+        self.workspace = Workspace()
 
     @property
     def handle(self):
@@ -108,6 +110,15 @@ def _p(a):
 _default_ws = Workspace()
 
 
+def _ws(workspace, stream):
+    """The workspace of a call: the caller's, else the default one of the call's stream.
+    Calls on different streams may run concurrently, so each stream has its own default
+    scratch; calls without a stream share the null stream's, which orders them."""
+    if workspace is not None:
+        return workspace
+    return stream.workspace if stream is not None else _default_ws
+
+
 def device_count() -> int:
     n = ctypes.c_int()
     rc = _lib.load().dfm_device_count(ctypes.byref(n))
@@ -125,7 +136,7 @@ def conv_grad1(x, alpha, y, w, sigma, precision="fp32c", conv=True, grad=True, s
     gout = DeviceArray((n, d)) if grad is True else (grad or None)
     op = _lib.OP_CONV_GRAD1 if (cout is not None and gout is not None) else (
         _lib.OP_CONV if cout is not None else _lib.OP_GRAD1)
-    ws = workspace or _default_ws
+    ws = _ws(workspace, stream)
     wp, wb = ws.get(L.dfm_workspace_size(op, d, dv, n, y.shape[0], prec))
     _lib.check(L.dfm_conv_grad1(_sp(stream), d, dv, float(sigma), prec, x.ptr,
                                 _p(alpha) if gout is not None else None, n, y.ptr, w.ptr,
@@ -139,7 +150,7 @@ def convolve(x, y, w, sigma, precision="fp32c", stream=None, workspace=None):
     n, d = x.shape
     dv = w.shape[1]
     out = DeviceArray((n, dv))
-    wp, wb = (workspace or _default_ws).get(
+    wp, wb = _ws(workspace, stream).get(
         L.dfm_workspace_size(_lib.OP_CONV, d, dv, n, y.shape[0], prec))
     _lib.check(L.dfm_conv(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
                           y.shape[0], out.ptr, wp, wb))
@@ -152,7 +163,7 @@ def grad1(x, alpha, y, w, sigma, precision="fp32c", stream=None, workspace=None)
     n, d = x.shape
     dv = w.shape[1]
     out = DeviceArray((n, d))
-    wp, wb = (workspace or _default_ws).get(
+    wp, wb = _ws(workspace, stream).get(
         L.dfm_workspace_size(_lib.OP_GRAD1, d, dv, n, y.shape[0], prec))
     _lib.check(L.dfm_grad1(_sp(stream), d, dv, float(sigma), prec, x.ptr, alpha.ptr, n, y.ptr,
                            w.ptr, y.shape[0], out.ptr, wp, wb))
@@ -165,7 +176,7 @@ def varlin(x, y, w, sigma, precision="fp32c", stream=None, workspace=None):
     n, d = x.shape
     dv = w.shape[1]
     out = DeviceArray((n, dv, d))
-    wp, wb = (workspace or _default_ws).get(
+    wp, wb = _ws(workspace, stream).get(
         L.dfm_workspace_size(_lib.OP_VARLIN, d, dv, n, y.shape[0], prec))
     _lib.check(L.dfm_varlin(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
                             y.shape[0], out.ptr, wp, wb))
@@ -178,7 +189,7 @@ def graddim(x, y, w, sigma, dim, precision="fp32c", stream=None, workspace=None)
     n, d = x.shape
     dv = w.shape[1]
     out = DeviceArray((n, dv))
-    wp, wb = (workspace or _default_ws).get(
+    wp, wb = _ws(workspace, stream).get(
         L.dfm_workspace_size(_lib.OP_GRADDIM, d, dv, n, y.shape[0], prec))
     _lib.check(L.dfm_graddim(_sp(stream), d, dv, float(sigma), prec, x.ptr, n, y.ptr, w.ptr,
                              y.shape[0], int(dim), out.ptr, wp, wb))
@@ -192,7 +203,7 @@ def graddiff(x, beta, eta, sigma, precision="fp32c", stream=None, workspace=None
     n, d = x.shape
     dv = beta.shape[1]
     out = DeviceArray((n, d))
-    wp, wb = (workspace or _default_ws).get(
+    wp, wb = _ws(workspace, stream).get(
         L.dfm_workspace_size(_lib.OP_GRADDIFF, d, dv, n, n, prec))
     _lib.check(L.dfm_graddiff(_sp(stream), d, dv, float(sigma), prec, x.ptr, beta.ptr, eta.ptr,
                               n, out.ptr, wp, wb))
@@ -214,7 +225,7 @@ def batch_conv_grad1(x, alpha, y, w, x_offsets, y_offsets, sigma, precision="fp3
     gout = DeviceArray((n, d)) if grad is True else (grad or None)
     op = _lib.OP_CONV_GRAD1 if (cout is not None and gout is not None) else (
         _lib.OP_CONV if cout is not None else _lib.OP_GRAD1)
-    ws = workspace or _default_ws
+    ws = _ws(workspace, stream)
     xp, yp = xo.ctypes.data_as(ctypes.c_void_p), yo.ctypes.data_as(ctypes.c_void_p)
     wp, wb = ws.get(L.dfm_batch_workspace(op, d, dv, S, xp, yp, prec))
     _lib.check(L.dfm_batch_conv_grad1(_sp(stream), d, dv, float(sigma), prec, S, xp, yp, x.ptr,

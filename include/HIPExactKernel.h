@@ -55,10 +55,21 @@ class HIPExactKernel : public ExactKernel<ScalarType, PointDim> {
   int GetPrecision() const { return m_Precision; }
 
   static int DefaultPrecision() {
+    CheckAbi();
     const char *env = std::getenv("DFM_PRECISION");
     if (env && std::strcmp(env, "fp32c") == 0) return DFM_FP32C;
     return DFM_FP64;
   }
+  /// Throws when the loaded libdfm_hip.so was built from another ABI version of dfm_hip.h
+  /// than this adapter (struct layouts and signatures would not match).
+  static void CheckAbi() {
+    const int v = dfm_abi_version();
+    if (v != DFM_ABI_VERSION)
+      throw std::runtime_error("HIPExactKernel: libdfm_hip.so has ABI version " +
+                               std::to_string(v) + ", this header expects " +
+                               std::to_string(DFM_ABI_VERSION));
+  }
+
 
   /// CUDAExactKernel::Convolve (CUDAExactKernel.cxx:41-90).
   virtual MatrixType Convolve(const MatrixType &X) {

@@ -51,11 +51,14 @@ enum dfm_status {
 
 enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2 };
 
-/* ABI version of this header (bumped on any signature or struct-layout change).
+/* ABI version of this header (bumped on any signature or struct-layout change); callers
+ * compare dfm_abi_version() of the loaded library with DFM_ABI_VERSION of the header they
+ * were built against.
  *   1  first release
  *   2  dfm_atlas_desc gained dimension and cell_corners (appended); new entry points for
  *      polylines, volume meshes, point clouds, landmarks, ConvolveHessian and the
  *      adjoint's jump conditions; DFM_OP_HESSIAN */
+#define DFM_ABI_VERSION 2
 int dfm_abi_version(void);
 /* Last error message of the calling thread ("" when none). */
 const char *dfm_last_error(void);
@@ -85,15 +88,18 @@ int dfm_profile_read(double *total_ms, long *launches);
 
 /* Autotuner of the pair-sum work split (no reference counterpart: the reference launches
  * one thread per target in blocks of 192, GpuConv1D.cu:98-132).  Per (op, precision,
- * dimpoint, dimvect, batched or not, log2 of the target and source counts) it chooses
+ * dimpoint, dimvect, batched or not, log2 of the target and source counts, and the grid's
+ * class of target workgroups: < 64, < 1024, < 4096, >= 4096) it chooses
  * the targets per lane (1, 2 or 4), the wavefront count the source splits aim for (4096,
  * 16384 or 65536), the fewest source tiles per split (4 or 1) and, for the FP32C fused
  * Convolve + Grad1 in 3D, whether the per-pair accumulation runs on the matrix cores.
  *   mode 0  heuristics only (default; results deterministic across calls)
  *   mode 1  explore: the first calls of each key run every candidate twice, timed with HIP
- *           events on the call's stream (read back at the next call of that key), then the
+ *           events on the call's stream (read back at a later call of that key), then the
  *           fastest is kept.  Plans differ in summation order, so results differ across
  *           candidates by rounding.  Not for use while a stream is being captured.
+ *           Threads may explore concurrently; dfm_autotune_reset may run at any time (a
+ *           timed launch in flight keeps its events alive and its timing is dropped).
  *   mode 2  use the tuned choices (default where a key was not tuned); capture-safe.
  * Workspace queries always return the maximum over every candidate, so a buffer sized
  * in any mode fits every plan.  dfm_autotune_report writes one line per key (truncated to

@@ -6,6 +6,7 @@ engine, on top of the oracle ExactKernel (oracle/exact_kernel.c):
   shoot            Diffeos::Shoot — explicit Euler (Diffeos.cxx:589-659), with the
                    commented-out Heun variant (Diffeos.cxx:634-652) as scheme="heun"
   flow             Diffeos::FlowLandmarkPointsTrajectory (Diffeos.cxx:661-710)
+  shoot_vjp        the shooting field's transpose Jacobian, from the reference's kernel ops
   parallel_transport  Diffeos::ParallelTransport (Diffeos.cxx:293-582)
   adjoint_reference   AdjointEquationsIntegrator (AdjointEquationsIntegrator.cxx:33-136,
                    414-567): the reference's continuous adjoint, discretised with Heun
@@ -65,6 +66,30 @@ def shoot(p0, a0, width, T, t0=0.0, tN=1.0, scheme="euler"):
             P[t + 1] = P[t] + (dpos + dpos2) * (dt * 0.5)
             A[t + 1] = A[t] - (dmom + dmom2) * (dt * 0.5)
     return P, A
+
+
+def shoot_vjp(p, a, mu_p, mu_a, width):
+    """Transpose Jacobian of the shooting field F(p, a) = (K(p,p) a, -G(p,a)^T a) (the
+    rates of Diffeos::Shoot, Diffeos.cxx:617-632) applied to (mu_p, mu_a), composed from
+    the kernel operations the reference's adjoint uses for the same terms
+    (AdjointEquationsIntegrator.cxx:476-567):
+      d/dp = ConvolveGradient(p, mu_p) [weights a] + ConvolveGradient(p, a) [weights mu_p]
+             + ConvolveSpecialHessian(mu_a) [weights a]
+      d/da = Convolve(p) [weights mu_p] - (G(p,a) mu_a - sum_l ConvolveGradient(p, l)
+             [weights a * mu_a_l])
+    Pinned against central differences of F in tests/test_deformations.py."""
+    p, a, mu_p, mu_a = (np.asarray(v, dtype=np.float64) for v in (p, a, mu_p, mu_a))
+    n, d = p.shape
+    ka = _kernel(d, width, p, a)
+    kmp = _kernel(d, width, p, mu_p)
+    jp = (ka.ConvolveGradient(p, mu_p) + kmp.ConvolveGradient(p, a)
+          + ka.ConvolveSpecialHessianGradDiff(mu_a))
+    G = ka.ConvolveGradient(p)  # (n, d, d): G[i, k, l] = sum_j a_j[k] d/dp_l K
+    diff = np.einsum("ikl,il->ik", G, mu_a)
+    for l in range(d):
+        diff -= _kernel(d, width, p, a * mu_a[:, l:l + 1]).ConvolveGradient(p, l)
+    ja = kmp.Convolve(p) - diff
+    return jp, ja
 
 
 def flow(P, A, x0, width, T, t0=0.0, tN=1.0, improved_euler=True):

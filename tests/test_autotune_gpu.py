@@ -1,6 +1,7 @@
 """The plan autotuner: every candidate plan it explores gives correct results (each call in
-explore mode against the default plan's FP64 result), exploration ends in a tuned choice
-per shape, and the tuned mode is deterministic."""
+explore mode against the FP64 oracle: ExactKernel.cxx:51-74, 166-202, 264-323 and the
+SpecialHessian closed form of CUDAExactKernel.cxx:196-226), exploration ends in a tuned
+choice per shape, and the tuned mode is deterministic."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,35 @@ def tuner():
         _lib.load().dfm_autotune_reset()
 
 
+def _oracle_refs(oracle_mod, data):
+    from oracle import deformations as dfo
+    X, Y, W, A, E, An, W6, A6, s = (data[k] for k in ("X", "Y", "W", "A", "E", "An", "W6", "A6",
+                                                      "s"))
+    n, m = X.shape[0], Y.shape[0]
+
+    def K(Ys, Ws):
+        k = oracle_mod.ExactKernel(3, s)
+        k.SetSources(Ys)
+        k.SetWeights(Ws)
+        return k
+
+    k, k6 = K(Y, W), K(Y, W6)
+    kb = [K(Y[:m // 2], W[:m // 2]), K(Y[m // 2:], W[m // 2:])]
+    xs = [slice(0, n // 3), slice(n // 3, n)]
+    return {
+        "conv_grad1": [k.Convolve(X), k.ConvolveGradient(X, A)],
+        "conv_grad1_dv6": [k6.Convolve(X), k6.ConvolveGradient(X, A6)],
+        "grad1": k.ConvolveGradient(X, A),
+        "conv": k.Convolve(X),
+        "varlin": k.ConvolveGradient(X),
+        "graddim": k.ConvolveGradient(X, 1),
+        "graddiff": K(X, A).ConvolveSpecialHessianGradDiff(E),
+        "shoot_vjp": list(dfo.shoot_vjp(X, A, E, An, s)),
+        "batch": [np.concatenate([kb[i].Convolve(X[xs[i]]) for i in range(2)]),
+                  np.concatenate([kb[i].ConvolveGradient(X[xs[i]], A[xs[i]]) for i in range(2)])],
+    }
+
+
 def _ops(n, m, seed):
     from deformetrica_code_amd import device
     from deformetrica_code_amd.diffeos import shoot_vjp
@@ -36,7 +66,9 @@ def _ops(n, m, seed):
     dAn = D(rng.normal(size=(n, 3)))
     dW6, dA6 = D(rng.normal(size=(m, 6))), D(rng.normal(size=(n, 6)))
     s = 0.1
-    return {
+    data = {"X": X, "Y": Y, "W": W, "A": A, "E": E, "An": dAn.numpy(), "W6": dW6.numpy(),
+            "A6": dA6.numpy(), "s": s}
+    return data, {
         "conv_grad1": lambda p: [a.numpy() for a in device.conv_grad1(dX, dA, dY, dW, s, p)],
         "conv_grad1_dv6": lambda p: [a.numpy() for a in device.conv_grad1(dX, dA6, dY, dW6, s, p)],
         "grad1": lambda p: device.grad1(dX, dA, dY, dW, s, p).numpy(),
@@ -51,9 +83,10 @@ def _ops(n, m, seed):
 
 
 @pytest.mark.parametrize("n,m", [(1000, 1000), (5000, 20000)])
-def test_every_explored_plan_is_correct(tuner, n, m):
-    ops = _ops(n, m, n)
-    refs = {k: f("fp64") for k, f in ops.items()}
+def test_every_explored_plan_is_correct(tuner, oracle_mod, n, m):
+    data, ops = _ops(n, m, n)
+    refs = _oracle_refs(oracle_mod, data)
+    assert set(refs) == set(ops)
     tuner.autotune(1)
     for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6)):
         for k, f in ops.items():
@@ -70,7 +103,7 @@ def test_every_explored_plan_is_correct(tuner, n, m):
 
 
 def test_tuned_mode_is_deterministic(tuner):
-    ops = _ops(3000, 3000, 3)
+    _, ops = _ops(3000, 3000, 3)
     tuner.autotune(1)
     for _ in range(50):
         ops["conv_grad1"]("fp32c")

@@ -195,3 +195,37 @@ def test_jump_schedule_walks_the_list_as_the_reference_loop():
     assert jump_schedule([1, 3], 6, True) == (None, {3: 1, 1: 0})
     assert jump_schedule([2], 6, True) == (0, {})
     assert jump_schedule(None, 6, True) == (0, {})
+
+
+def test_shoot_vjp_is_the_transpose_jacobian_of_the_shooting_field():
+    """oracle shoot_vjp against central differences of mu . F(p, a), F the Euler rates of
+    Diffeos::Shoot (Diffeos.cxx:617-632)."""
+    from oracle import deformations
+    from oracle.deformations import _kernel
+    rng = np.random.default_rng(21)
+    n, d, width = 12, 3, 0.45
+    p, a = rng.uniform(size=(n, d)), rng.normal(size=(n, d))
+    mp, ma = rng.normal(size=(n, d)), rng.normal(size=(n, d))
+
+    def objective(pp, aa):
+        k = _kernel(d, width, pp, aa)
+        fp = k.Convolve(pp)
+        fa = -np.einsum("ikl,ik->il", k.ConvolveGradient(pp), aa)
+        return float(np.sum(mp * fp) + np.sum(ma * fa))
+
+    jp, ja = deformations.shoot_vjp(p, a, mp, ma, width)
+    eps = 1e-6
+    for arr, got in ((p, jp), (a, ja)):
+        fd = np.zeros_like(arr)
+        for i in range(n):
+            for k in range(d):
+                plus, minus = arr.copy(), arr.copy()
+                plus[i, k] += eps
+                minus[i, k] -= eps
+                args_p = (plus, a) if arr is p else (p, plus)
+                args_m = (minus, a) if arr is p else (p, minus)
+                fd[i, k] = (objective(*args_p) - objective(*args_m)) / (2 * eps)
+        assert np.linalg.norm(got - fd) / np.linalg.norm(fd) < 1e-7
+    # a transposed operand or a dropped term fails: swapping the roles of mu_p and a does
+    jp_bad, _ = deformations.shoot_vjp(p, mp, a, ma, width)
+    assert np.linalg.norm(jp_bad - jp) / np.linalg.norm(jp) > 1e-2

@@ -33,7 +33,11 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.dfm_abi_version() == 2
+    from deformetrica_code_amd import _lib
+    header = open(os.path.join(ROOT, "include", "dfm_hip.h")).read()
+    m = re.search(r"#define DFM_ABI_VERSION (\d+)", header)
+    assert m, "dfm_hip.h defines DFM_ABI_VERSION"
+    assert lib.dfm_abi_version() == int(m.group(1)) == _lib.ABI_VERSION
 
 
 def test_argument_errors_without_gpu(lib):
