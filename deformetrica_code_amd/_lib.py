@@ -176,6 +176,7 @@ def load():
     L.dfm_conv.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
     L.dfm_grad1.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P, sz]
     L.dfm_conv_grad1.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P, _P, sz]
+    L.dfm_conv_grad1_noalloc.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P]
     L.dfm_varlin.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
     L.dfm_graddim.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, i, _P, _P, sz]
     L.dfm_graddiff.argtypes = [_P, i, i, d, i, _P, _P, _P, l, _P, _P, sz]
@@ -188,7 +189,8 @@ def load():
     L.dfm_gauss_graddiffconv.argtypes = [i, i, d, _D, _D, _D, _D, i, i]
     L.dfm_gauss_conv_grad1.argtypes = [i, i, d, _D, _D, _D, _D, _D, _D, i, i, i]
     L.dfm_gauss_hessianconv.argtypes = [i, i, d, _D, _D, _D, _D, i, i, i]
-    for name in ("dfm_conv", "dfm_grad1", "dfm_conv_grad1", "dfm_varlin", "dfm_graddim",
+    for name in ("dfm_conv", "dfm_grad1", "dfm_conv_grad1", "dfm_conv_grad1_noalloc",
+                 "dfm_varlin", "dfm_graddim",
                  "dfm_graddiff", "dfm_hessian", "dfm_gauss_conv", "dfm_gauss_grad1conv",
                  "dfm_gauss_gradconv", "dfm_gauss_gradconv_varlin", "dfm_gauss_graddiffconv",
                  "dfm_gauss_conv_grad1", "dfm_gauss_hessianconv"):

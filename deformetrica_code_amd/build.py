@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "libdfm_hip.so")
 
 SOURCES = [
     "pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip",
-    "integrators.hip", "surfaces.hip", "atlas.hip", "transport.hip",
+    "integrators.hip", "surfaces.hip", "atlas.hip", "transport.hip", "scratch.hip",
 ]
 
 HIPCC_FLAGS = [

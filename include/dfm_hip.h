@@ -202,6 +202,20 @@ int dfm_conv_grad1(void *stream, int dimpoint, int dimvect, double sigma, int pr
                    const double *beta, long ny, double *conv, double *grad, void *work,
                    size_t work_bytes);
 
+/* dfm_conv_grad1 without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Its workspace is a static device array of
+ * DFM_STATIC_SCRATCH_BYTES per GPU, held until the process ends; the call itself
+ * allocates nothing.  It runs the plan dfm_conv_grad1 runs at the same sizes, so results
+ * are bitwise equal to dfm_conv_grad1's; a call whose plan needs more scratch than the
+ * array holds fails with DFM_ERR_ARG ("static scratch ... too small: need N bytes").
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be ordered by the caller. */
+#define DFM_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int dfm_conv_grad1_noalloc(void *stream, int dimpoint, int dimvect, double sigma, int precision,
+                           const double *x, const double *alpha, long nx, const double *y,
+                           const double *beta, long ny, double *conv, double *grad);
+
 int dfm_varlin(void *stream, int dimpoint, int dimvect, double sigma, int precision,
                const double *x, long nx, const double *y, const double *beta, long ny,
                double *gamma, void *work, size_t work_bytes);
