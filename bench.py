@@ -10,6 +10,8 @@ plugin boundary.
 
 Beside the headline the run reports:
   * the same step in plain FP64 (the reference's arithmetic on the GPU) and the speed-up;
+  * the same step with exact FP32 differences everywhere (fp32x) and with the distances'
+    cross term on the matrix cores (bf16mfma), each with its time and error;
   * the FP32-compensated result's error against the FP64 GPU result over all N rows;
   * `roofline`: the pair-sum kernel's duration from HIP events on its stream,
     29 algorithmic flops per pair (d=3, Dv=3) against the FP32 peak of 157.3 TFLOP/s;
@@ -146,6 +148,13 @@ def main():
     gx = grad.numpy(stream)
     print("fp32x: %.4f s/step, kernel %.4f s" % (s_x, kern_x), file=sys.stderr, flush=True)
 
+    # ---- DFM_BF16MFMA: the distances' cross term on the matrix cores
+    step("bf16mfma")
+    s_b, kern_b = timed("bf16mfma", max(1, args.steps // 4))
+    cb = conv.numpy(stream)
+    gb = grad.numpy(stream)
+    print("bf16mfma: %.4f s/step, kernel %.4f s" % (s_b, kern_b), file=sys.stderr, flush=True)
+
     # ---- plain FP64 form on the GPU
     step("fp64")
     s64, kern64 = timed("fp64", args.fp64_steps)
@@ -167,6 +176,9 @@ def main():
         "conv_max_over_rms": maxrms(c64, c32), "grad_max_over_rms": maxrms(g64, g32),
         "fp32x_conv_normwise": nrel(c64, cx), "fp32x_grad_normwise": nrel(g64, gx),
         "fp32x_conv_max_over_rms": maxrms(c64, cx), "fp32x_grad_max_over_rms": maxrms(g64, gx),
+        "bf16mfma_conv_normwise": nrel(c64, cb), "bf16mfma_grad_normwise": nrel(g64, gb),
+        "bf16mfma_conv_max_over_rms": maxrms(c64, cb),
+        "bf16mfma_grad_max_over_rms": maxrms(g64, gb),
     }
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of target rows
@@ -204,6 +216,9 @@ def main():
             "conv_max_over_rms": maxrms(oc, c32[idx]), "grad_max_over_rms": maxrms(og, g32[idx]),
             "fp64_gpu_conv_normwise": nrel(oc, c64[idx]),
             "fp64_gpu_grad_normwise": nrel(og, g64[idx]),
+            "bf16mfma_conv_normwise": nrel(oc, cb[idx]), "bf16mfma_grad_normwise": nrel(og, gb[idx]),
+            "bf16mfma_conv_max_over_rms": maxrms(oc, cb[idx]),
+            "bf16mfma_grad_max_over_rms": maxrms(og, gb[idx]),
         }
 
     pairs = float(N) * float(M)
@@ -250,6 +265,9 @@ def main():
                            "speedup_fp32c_over_fp64": s64 / s_per_step},
         "fp32x_exact_differences": {"s_per_step": s_x, "kernel_s": kern_x,
                                     "pair_evals_per_s": pairs / s_x},
+        "bf16mfma_cross_term": {"s_per_step": s_b, "kernel_s": kern_b,
+                                "pair_evals_per_s": pairs / s_b,
+                                "kernel": "k_pairsum_bf<CONVGRAD1, 3, 3, 4> + k_reduce_splits"},
         "error_vs_fp64_gpu": err_fp64gpu,
         "error_vs_fp64_oracle": err_oracle,
     }

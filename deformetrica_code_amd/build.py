@@ -22,7 +22,8 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libdfm_hip.so")
 
 SOURCES = [
-    "pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum.hip", "capi.hip",
+    "pairsum_f64.hip", "pairsum_f32c.hip", "pairsum_f32x.hip", "pairsum_bf16.hip", "pairsum.hip",
+    "capi.hip",
     "integrators.hip", "surfaces.hip", "atlas.hip", "transport.hip", "scratch.hip",
 ]
 

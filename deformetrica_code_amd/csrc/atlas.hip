@@ -190,7 +190,7 @@ int check_desc(const dfm_atlas_desc *d) {
   if (!d) return set_error(DFM_ERR_ARG, "descriptor is NULL");
   if (d->kind != DFM_CURRENT && d->kind != DFM_VARIFOLD)
     return set_error(DFM_ERR_ARG, "unknown data term %d", d->kind);
-  if (d->precision != DFM_FP64 && d->precision != DFM_FP32C && d->precision != DFM_FP32X)
+  if (!valid_precision(d->precision))
     return set_error(DFM_ERR_ARG, "unknown precision %d", d->precision);
   if (d->scheme != DFM_EULER && d->scheme != DFM_HEUN)
     return set_error(DFM_ERR_ARG, "unknown scheme %d", d->scheme);

@@ -28,7 +28,7 @@ int check_dims(int D, int DV) {
 int check_common(int D, int DV, int prec, double sigma, long nx, long ny) {
   int rc = check_dims(D, DV);
   if (rc) return rc;
-  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+  if (!valid_precision(prec))
     return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
   if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
   if (nx < 0 || ny < 0 || nx > kMaxCount || ny > kMaxCount)

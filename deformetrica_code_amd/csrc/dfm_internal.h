@@ -6,6 +6,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "dfm_hip.h"
+
 namespace dfm {
 
 enum : int {
@@ -36,6 +38,15 @@ constexpr int kFoldF32 = 64;
 // sort's launches (bounding box, keys, radix passes) cost more than the compact wavefronts
 // save, and wide wavefronts take the exact-difference path instead.
 constexpr long kSortMinTargets = 8192;
+
+// DFM_BF16MFMA: targets per workgroup of the matrix-core kernel (4 wavefronts x 4 tiles of
+// 16), which also decides, per group of this many sorted targets, which kernel runs them;
+// and sources per LDS tile (one staging thread each)
+constexpr int kBfGroup = 256;
+constexpr int kBfTile = 256;
+// DFM_BF16MFMA: a group of kBfGroup targets spanning more than this many scaled kernel units
+// runs on k_pairsum instead (its coordinates, and so the cross term's rounding, would grow)
+constexpr double kBfExtent = 6.0;
 
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
 // (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
@@ -75,7 +86,9 @@ struct KParams {
   double ch1;  // GradDiff:  2 / sigma^2
   double ch2;  // GradDiff: -4 / (sigma^4 s^2)
   double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
-  int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores)
+  double exact_extent_bf;  // DFM_BF16MFMA: group extent above which k_pairsum runs the group
+  int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores);
+                        // 2 only those of wide groups of kBfGroup targets (DFM_BF16MFMA)
 };
 
 // Work decomposition and workspace layout of one call.
@@ -91,6 +104,8 @@ struct Plan {
   size_t sort_tmp_bytes;
   size_t work_bytes;
 };
+
+inline bool valid_precision(int prec) { return prec >= DFM_FP64 && prec <= DFM_BF16MFMA; }
 
 int set_error(int code, const char *fmt, ...);
 const char *last_error();
@@ -176,6 +191,9 @@ size_t batch_adjoint_workspace(int D, int nseg, const long *cp_off, const long *
 
 // the matrix-core form exists for this op and shape (pairsum_f32c.hip)
 bool mx_supported(int op, int D, int DV);
+// the BF16 cross-term kernel exists for this op and shape (pairsum_bf16.hip)
+bool bf_supported(int op, int D, int DV);
+int launch_op_bf16(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);
 
 // one per precision (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip)
 int launch_op_f64(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl);

@@ -193,7 +193,16 @@ static int targets_per_lane(int op, int prec, int D, int DV) {
   }
 }
 
-static int tile_of(int prec) { return prec == DFM_FP64 ? kTileF64 : kTileF32; }
+static int tile_of(int prec) {
+  return prec == DFM_FP64 ? kTileF64 : prec == DFM_BF16MFMA ? kBfTile : kTileF32;
+}
+
+// DFM_BF16MFMA runs the matrix-core kernel only for single Convolve / Grad1 / fused calls
+// large enough to be Morton-sorted; everything else runs as DFM_FP32C.
+static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx) {
+  if (prec != DFM_BF16MFMA) return prec;
+  return (nseg == 1 && nx >= kSortMinTargets && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
+}
 
 static int nacc_of(int op, int D, int DV) {
   switch (op) {
@@ -227,8 +236,18 @@ struct Choice {
 // plan (366 ms aiming for 65536 wavefronts, same box) against 483 ms with 4 targets per
 // lane, 4096 wavefronts and 256-source tiles.
 static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
-  (void)op; (void)D; (void)DV;
+  (void)D; (void)DV;
+  // DFM_BF16MFMA: workgroups of 256 targets (R = 4 in 64-target units) and 256-source tiles;
+  // the split count aims for 16384 workgroups
+  if (prec == DFM_BF16MFMA) return Choice{4, 16384, 1};
   const long blocks = (max_nx + 63) / 64 * nseg;
+  // FP32 Convolve / Grad1 / fused sums (the expanded-distance form) on large single grids:
+  // two targets per lane share each LDS source record, halving the LDS reads that bound
+  // the one-target plan.  At N = M = 1M the fused FP32C sum runs in 318 ms with two
+  // targets per lane and 65536 wavefronts against 354 ms with the one-target plan below
+  // (same box, every candidate timed by the autotuner).
+  const bool expandable = op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1;
+  if (prec != DFM_FP64 && expandable && nseg == 1 && blocks >= 4096) return Choice{2, 65536, 4};
   // at 1,000 x 1,000 (16 target workgroups) the fused FP32C sum takes 0.017 ms with
   // one-tile splits against 0.030 ms with four-tile ones (FP64: 0.021 against 0.053 ms);
   // from 64 workgroups up (a 4,096-point registration's sums) one-tile FP32 splits cost
@@ -241,6 +260,13 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
 // register budget of the accumulators).
 static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg, long max_nx) {
   const Choice d = default_choice(op, prec, D, DV, nseg, max_nx);
+  if (prec == DFM_BF16MFMA) {
+    std::vector<Choice> out{d};
+    for (int waves : {4096, 16384, 65536})
+      for (int mt : {1, 2})
+        if (!(waves == d.waves && mt == d.min_tiles)) out.push_back(Choice{4, waves, mt});
+    return out;
+  }
   const int Rmax = targets_per_lane(op, prec, D, DV);
   std::vector<int> Rs{Rmax};
   if (Rmax >= 4) Rs.push_back(2);
@@ -339,6 +365,7 @@ static Plan single_plan(int op, int prec, int D, int DV, long nx, long ny, const
 // Workspace bound over every candidate, so that a caller's buffer fits whatever plan the
 // autotuner picks.
 Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
+  prec = effective_prec(op, prec, D, DV, 1, nx);
   Plan best = single_plan(op, prec, D, DV, nx, ny, default_choice(op, prec, D, DV, 1, nx));
   for (const Choice &c : candidates(op, prec, D, DV, 1, nx)) {
     Plan p = single_plan(op, prec, D, DV, nx, ny, c);
@@ -349,6 +376,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
 
 Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
                      long max_ny) {
+  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Morton-sorted
   Plan best = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny,
                          default_choice(op, prec, D, DV, nseg, max_nx));
   for (const Choice &c : candidates(op, prec, D, DV, nseg, max_nx)) {
@@ -598,6 +626,7 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
   // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
   p.s = prec == DFM_FP64 ? 1.0 / sigma : std::sqrt(1.4426950408889634074) / sigma;
   p.exact_extent = kExactExtent;
+  p.exact_extent_bf = kBfExtent;
   const double s2 = sigma * sigma;
   p.cg = -2.0 / (s2 * p.s);
   p.ch1 = 2.0 / s2;
@@ -624,9 +653,10 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
   }
   if (ev0) (void)hipEventRecord(ev0, st);
-  int rc = prec == DFM_FP64    ? launch_op_f64(st, op, D, DV, p, pl)
-           : prec == DFM_FP32X ? launch_op_f32x(st, op, D, DV, p, pl)
-                               : launch_op_f32c(st, op, D, DV, p, pl);
+  int rc = prec == DFM_FP64       ? launch_op_f64(st, op, D, DV, p, pl)
+           : prec == DFM_FP32X    ? launch_op_f32x(st, op, D, DV, p, pl)
+           : prec == DFM_BF16MFMA ? launch_op_bf16(st, op, D, DV, p, pl)
+                                  : launch_op_f32c(st, op, D, DV, p, pl);
   hipError_t e = rc == DFM_OK ? hipGetLastError() : hipSuccess;
   if (ev1) (void)hipEventRecord(ev1, st);
   if (timed) timed->state = (rc == DFM_OK && e == hipSuccess) ? 1 : 2;
@@ -653,6 +683,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
                   const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
                   size_t work_bytes) {
   if (xs.n != ys.n) return set_error(DFM_ERR_ARG, "target and source segment counts differ");
+  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Morton-sorted
   p.nx = xs.total;
   p.ny = ys.total;
   p.xoff = xs.n > 1 ? xs.d : nullptr;
@@ -680,6 +711,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
             long out0_n, long out1_n, void *work, size_t work_bytes, const char *ws_label) {
   if (p.nx == 0) return DFM_OK;
+  prec = effective_prec(op, prec, D, DV, 1, p.nx);
   if (p.ny == 0) {  // empty sums
     fill(st, p.out0, out0_n, 0.0);
     fill(st, p.out1, out1_n, 0.0);

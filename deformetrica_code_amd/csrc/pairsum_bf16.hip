@@ -1,0 +1,59 @@
+// pairsum_bf16.hip — launches of the DFM_BF16MFMA pair sums: the matrix-core kernel
+// (k_pairsum_bf) over the compact groups of 256 sorted targets, k_pairsum (FP32C, one
+// target per lane) over the wavefronts of the wide groups, then the split reduction.
+#include "pairsum_bf16.h"
+
+namespace dfm {
+namespace {
+
+template <int OP, int D, int DV>
+void launch_bf(hipStream_t st, const KParams &p, const Plan &pl) {
+  const dim3 gbf((unsigned)((p.nx + kBfGroup - 1) / kBfGroup), (unsigned)pl.nsplit, 1);
+  hipLaunchKernelGGL((k_pairsum_bf<OP, D, DV, kBfGroup / 64>), gbf, dim3(256), 0, st, p);
+  KParams q = p;
+  q.wave_filter = 2;
+  const dim3 gw((unsigned)((p.nx + 63) / 64), (unsigned)pl.nsplit, 1);
+  hipLaunchKernelGGL((k_pairsum<float, OP, D, DV, 0, 1, MODE_ADAPTIVE>), gw, dim3(64), 0, st, q);
+  if (pl.nsplit > 1)
+    hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
+                       dim3(256), 0, st, p);
+}
+
+template <int OP, int D>
+int launch_bf_dv(hipStream_t st, int DV, const KParams &p, const Plan &pl) {
+  switch (DV) {
+    case 1: launch_bf<OP, D, 1>(st, p, pl); return DFM_OK;
+    case 2: launch_bf<OP, D, 2>(st, p, pl); return DFM_OK;
+    case 3: launch_bf<OP, D, 3>(st, p, pl); return DFM_OK;
+    case 4: launch_bf<OP, D, 4>(st, p, pl); return DFM_OK;
+    case 5: launch_bf<OP, D, 5>(st, p, pl); return DFM_OK;
+    case 6: launch_bf<OP, D, 6>(st, p, pl); return DFM_OK;
+  }
+  return set_error(DFM_ERR_DIM, "unsupported dimvect %d", DV);
+}
+
+template <int OP>
+int launch_bf_d(hipStream_t st, int D, int DV, const KParams &p, const Plan &pl) {
+  if (D == 2) return launch_bf_dv<OP, 2>(st, DV, p, pl);
+  if (D == 3) return launch_bf_dv<OP, 3>(st, DV, p, pl);
+  return set_error(DFM_ERR_DIM, "unsupported dimpoint %d", D);
+}
+
+}  // namespace
+
+bool bf_supported(int op, int D, int DV) {
+  return (op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1) && (D == 2 || D == 3) &&
+         DV >= 1 && DV <= 6;
+}
+
+int launch_op_bf16(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl) {
+  if (p.xoff || p.yoff || !bf_supported(op, D, DV))
+    return set_error(DFM_ERR_ARG, "the BF16 cross-term kernel takes single Convolve / Grad1 calls");
+  switch (op) {
+    case OP_CONV: return launch_bf_d<OP_CONV>(st, D, DV, p, pl);
+    case OP_GRAD1: return launch_bf_d<OP_GRAD1>(st, D, DV, p, pl);
+    default: return launch_bf_d<OP_CONVGRAD1>(st, D, DV, p, pl);
+  }
+}
+
+}  // namespace dfm

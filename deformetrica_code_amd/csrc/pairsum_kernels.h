@@ -99,6 +99,13 @@ struct Shape {
                               : OP == OP_GRADDIFF  ? 2 * D
                               : OP == OP_HESS      ? DV * (1 + D * (D + 1) / 2)
                                                    : 5 * D;
+  // FP32 compact wavefronts of CONV, GRAD1 and CONVGRAD1 use the expanded distance (see
+  // pair_update_expand): record [y (D) | w (DV) | |y|^2], and GRAD1 accumulates sum_j t_ij
+  // beside sum_j t_ij y_j
+  static constexpr bool EXPANDABLE = OP == OP_CONV || OP == OP_GRAD1 || OP == OP_CONVGRAD1;
+  static constexpr int NSPE = (D + DV + 1 + 3) / 4 * 4;
+  static constexpr int NACCE = OP == OP_GRAD1 ? D + 1 : NACC;
+  static constexpr int NACCMAX = NACCE > NACC ? NACCE : NACC;
 };
 
 // ------------------------------------------------------------------------------------
@@ -227,6 +234,71 @@ __device__ __forceinline__ void pair_update(const T *xi, const T *xl, const T *t
   }
 }
 
+// Per-pair update of the expanded-distance form (FP32 compact wavefronts; CONV, GRAD1,
+// CONVGRAD1).  With xt, yt the FP32 centred scaled points,
+//   |xt - yt|^2 = |xt|^2 + (|yt|^2 - 2 xt . yt)
+// and the bracket is three FMAs from the staged |yt|^2 and xm2 = -2 xt, so
+//   K = exp2(-|xt|^2) K',  K' = exp2(-(|yt|^2 - 2 xt . yt)),
+// with the per-target factor exp2(-|xt|^2) applied once, in FP64, after the sum.  The
+// gradient sum_j t_ij (xt_i - yt_j), t_ij = K_ij (a_i . w_j), is formed at the end as
+// xt_i sum_j t_ij - sum_j t_ij yt_j, with sum_j t_ij = a_i . conv_i for CONVGRAD1 and an
+// accumulator of its own for GRAD1.  Three FMAs replace the three differences and three
+// multiply-adds of |xt - yt|^2; the error stays that of the FP32 differences (the
+// bracket's rounding is of the order of the coordinates' own FP32 rounding).
+template <int OP, int D, int DV>
+__device__ __forceinline__ void pair_update_expand(const float *xm2, const float *te,
+                                                   const float *rec, float *acc) {
+  float e = rec[D + DV];
+#pragma unroll
+  for (int k = 0; k < D; k++) e = __builtin_fmaf(xm2[k], rec[k], e);
+  const float K = Prec<float>::kexp(e);
+  const float *w = rec + D;
+  if constexpr (OP == OP_CONV || OP == OP_CONVGRAD1) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) acc[k] += K * w[k];
+  }
+  if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
+    constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
+    float sa = te[0] * w[0];
+#pragma unroll
+    for (int k = 1; k < DV; k++) sa += te[k] * w[k];
+    const float t = K * sa;
+    if constexpr (OP == OP_GRAD1) acc[D] += t;
+#pragma unroll
+    for (int l = 0; l < D; l++) acc[G + l] += t * rec[l];
+  }
+}
+
+// The expanded form's sums of one target -> the sums pair_update accumulates (the conv
+// part, then sum_j t_ij (xt_i - yt_j) for the gradient), in FP64.
+template <int OP, int D, int DV, int NA>
+__device__ __forceinline__ void expand_finish(double (&a)[NA], const float *xt, const float *te) {
+  double x[D];
+  double n2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    x[k] = (double)xt[k];
+    n2 += x[k] * x[k];
+  }
+  const double f = exp2(-n2);
+  if constexpr (OP == OP_CONV) {
+#pragma unroll
+    for (int k = 0; k < DV; k++) a[k] *= f;
+  } else if constexpr (OP == OP_GRAD1) {
+    const double st = a[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) a[l] = f * (x[l] * st - a[l]);
+  } else {  // OP_CONVGRAD1
+    double st = 0.0;
+#pragma unroll
+    for (int k = 0; k < DV; k++) st += (double)te[k] * a[k];
+#pragma unroll
+    for (int l = 0; l < D; l++) a[DV + l] = f * (x[l] * st - a[DV + l]);
+#pragma unroll
+    for (int k = 0; k < DV; k++) a[k] *= f;
+  }
+}
+
 // Scale the FP64 accumulators of one target and store them at original index `o`.
 template <int OP, int D, int DV>
 __device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
@@ -295,14 +367,19 @@ __device__ __forceinline__ double wave_max(double v) {
 // ------------------------------------------------------------------------------------
 // the pair-sum kernel
 
-// Stream the sources [js, je) through the wavefront's LDS tile and accumulate.
-template <typename T, int OP, int D, int DV, int AUX, int R, bool EXACT, int NTE, int NACC>
+// Stream the sources [js, je) through the wavefront's LDS tile and accumulate.  EXPAND
+// selects the expanded-distance form (pair_update_expand; FP32 only).
+template <typename T, int OP, int D, int DV, int AUX, int R, bool EXACT, bool EXPAND, int NTE,
+          int NA>
 __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, long js, long je,
                                           const double *c, double s, const T (&xt)[R][D],
                                           const T (&xl)[R][D], const T (&te)[R][NTE],
-                                          double (&accd)[R][NACC]) {
+                                          double (&accd)[R][NA]) {
   using S = Shape<OP, D, DV>;
-  constexpr int NSP = EXACT ? S::NSPX : S::NSP;
+  static_assert(!EXPAND || (S::EXPANDABLE && !EXACT), "expanded form: CONV / GRAD1 / CONVGRAD1");
+  constexpr int NSP = EXPAND ? S::NSPE : EXACT ? S::NSPX : S::NSP;
+  constexpr int NACC = EXPAND ? S::NACCE : S::NACC;  // accumulators in use
+  static_assert(NACC <= NA, "accumulator array holds the form's sums");
   constexpr int LO = S::NSRC;
   constexpr int TS = Prec<T>::kTile;
   constexpr int FOLD = Prec<T>::kFold;
@@ -316,6 +393,11 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
   for (int r = 0; r < R; r++)
 #pragma unroll
     for (int a = 0; a < NACC; a++) acc[r][a] = T(0);
+  T xm2[R][D];  // -2 xt, the expanded form's cross-term factor
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int k = 0; k < D; k++) xm2[r][k] = T(-2) * xt[r][k];
 
   for (long j0 = js; j0 < je; j0 += TS) {
     __syncthreads();
@@ -346,6 +428,12 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
 #pragma unroll
         for (int k = 0; k < D; k++) rec[k] = far;
       }
+      if constexpr (EXPAND) {  // |yt|^2 of the FP32 coordinates, rounded once
+        double n2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) n2 += (double)rec[k] * (double)rec[k];
+        rec[D + DV] = (T)n2;
+      }
       T *dst = sm + (q * 64 + lane) * NSP;
 #pragma unroll
       for (int k = 0; k < NSP; k++) dst[k] = rec[k];
@@ -364,8 +452,12 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
 #pragma unroll
           for (int k = 0; k < NSP; k++) rec[k] = srcp[k];
 #pragma unroll
-          for (int r = 0; r < R; r++)
-            pair_update<OP, D, DV, AUX, EXACT, T>(xt[r], xl[r], te[r], rec, acc[r]);
+          for (int r = 0; r < R; r++) {
+            if constexpr (EXPAND)
+              pair_update_expand<OP, D, DV>(xm2[r], te[r], rec, acc[r]);
+            else
+              pair_update<OP, D, DV, AUX, EXACT, T>(xt[r], xl[r], te[r], rec, acc[r]);
+          }
         }
       }
       // fold the FOLD-source partial sums into the FP64 accumulators
@@ -384,6 +476,7 @@ template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
 __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   using S = Shape<OP, D, DV>;
   constexpr int NACC = S::NACC;
+  constexpr int NA = S::NACCMAX;
   constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
   constexpr int TS = Prec<T>::kTile;
   constexpr int FOLD = Prec<T>::kFold;
@@ -447,6 +540,33 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   if constexpr (Prec<T>::kCenter) {
     // the compact wavefronts of this call run on the matrix cores (k_pairsum_mx)
     if (p.wave_filter == 1 && !(extent * s > p.exact_extent)) return;
+    if (p.wave_filter == 2) {
+      // DFM_BF16MFMA: k_pairsum_bf ran this wavefront's group of kBfGroup sorted targets
+      // unless the group is wide; the same exact FP64 min / max over the same targets
+      // decides in both kernels (single segment, one target per lane)
+      const long gb = base & ~(long)(kBfGroup - 1);
+      double glo[D], ghi[D];
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        glo[k] = INFINITY;
+        ghi[k] = -INFINITY;
+      }
+      for (int q = 0; q < kBfGroup / 64; q++) {
+        const long gp = gb + 64 * q + lane;
+        if (gp < xe) {
+          const long o = p.perm ? (long)p.perm[gp] : gp;
+#pragma unroll
+          for (int k = 0; k < D; k++) {
+            glo[k] = fmin(glo[k], p.x[o * D + k]);
+            ghi[k] = fmax(ghi[k], p.x[o * D + k]);
+          }
+        }
+      }
+      double gext = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; k++) gext = fmax(gext, wave_max(ghi[k]) - wave_min(glo[k]));
+      if (gext * s <= p.exact_extent_bf) return;
+    }
   }
   T xt[R][D];
   T xl[R][D];
@@ -455,7 +575,8 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   for (int r = 0; r < R; r++) {
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      const double v = (xd[r][k] - c[k]) * s;
+      // lanes past the end sit at the centre (their sums are discarded)
+      const double v = valid[r] ? (xd[r][k] - c[k]) * s : 0.0;
       xt[r][k] = (T)v;
       xl[r][k] = (T)(v - (double)xt[r][k]);
     }
@@ -476,11 +597,11 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
     }
   }
 
-  double accd[R][NACC];
+  double accd[R][NA];
 #pragma unroll
   for (int r = 0; r < R; r++)
 #pragma unroll
-    for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
+    for (int a = 0; a < NA; a++) accd[r][a] = 0.0;
 
   const long js = yb + (long)split * p.split_len;
   const long je = std::min(ye, js + p.split_len);
@@ -488,12 +609,21 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
     // wavefront-uniform choice: coordinates re-centred on a box wider than a few kernel
     // widths lose too much to FP32 rounding, so such wavefronts form exact differences
     const bool exact = MODE == MODE_EXACT || extent * s > p.exact_extent;
-    if (exact)
-      run_tiles<T, OP, D, DV, AUX, R, true, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
-    else
-      run_tiles<T, OP, D, DV, AUX, R, false, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
+    if (exact) {
+      run_tiles<T, OP, D, DV, AUX, R, true, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
+                                                            accd);
+    } else if constexpr (S::EXPANDABLE) {
+      run_tiles<T, OP, D, DV, AUX, R, false, true, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
+                                                            accd);
+#pragma unroll
+      for (int r = 0; r < R; r++) expand_finish<OP, D, DV, NA>(accd[r], xt[r], te[r]);
+    } else {
+      run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
+                                                             accd);
+    }
   } else {
-    run_tiles<T, OP, D, DV, AUX, R, false, NTE, NACC>(p, sm, lane, js, je, c, s, xt, xl, te, accd);
+    run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
+                                                           accd);
   }
 
   // ---- results

@@ -60,7 +60,7 @@ extern "C" int dfm_conv_grad1_noalloc(void *stream, int D, int DV, double sigma,
   // the argument checks of dfm_conv_grad1
   if (D != 2 && D != 3) return set_error(DFM_ERR_DIM, "unsupported dimpoint %d (2 or 3)", D);
   if (DV < 1 || DV > 6) return set_error(DFM_ERR_DIM, "unsupported dimvect %d (1..6)", DV);
-  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+  if (!valid_precision(prec))
     return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
   if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
   if (nx < 0 || ny < 0 || nx > 0x7fffffffL || ny > 0x7fffffffL)

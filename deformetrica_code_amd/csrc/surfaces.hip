@@ -612,7 +612,7 @@ KParams blank() {
 int check_args(int kind, int prec, double sigma, long nvs, long nfs, long nvt, long nft) {
   if (kind != DFM_CURRENT && kind != DFM_VARIFOLD)
     return set_error(DFM_ERR_ARG, "unknown data term %d", kind);
-  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+  if (!valid_precision(prec))
     return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
   if (!(sigma == sigma)) return set_error(DFM_ERR_ARG, "kernel width is NaN");
   if (nvs < 0 || nfs < 0 || nvt < 0 || nft < 0 || nfs > 0x7fffffffL || nft > 0x7fffffffL ||

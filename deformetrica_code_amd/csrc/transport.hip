@@ -199,7 +199,7 @@ struct Transport {
 
 int check_args(int D, int prec, double sigma, long n, long m, int T) {
   if (D != 2 && D != 3) return set_error(DFM_ERR_DIM, "unsupported dimpoint %d (2 or 3)", D);
-  if (prec != DFM_FP64 && prec != DFM_FP32C && prec != DFM_FP32X)
+  if (!valid_precision(prec))
     return set_error(DFM_ERR_ARG, "unknown precision %d", prec);
   if (!(sigma > 0)) return set_error(DFM_ERR_ARG, "kernel width must be positive");
   if (n < 0 || m < 0 || n > 46340 || m > 0x7fffffffL)

@@ -23,6 +23,13 @@
  *              summation).  A wavefront whose targets span more than 4 scaled kernel
  *              widths forms its coordinate differences exactly (FP32 hi + lo parts).
  *   DFM_FP32X  as DFM_FP32C with exact coordinate differences in every wavefront.
+ *   DFM_BF16MFMA  Convolve, ConvolveGradient(X, alpha) and the fused sum of single calls
+ *              from 8,192 targets on: the distances' cross term -2 x.y and |y|^2 on the
+ *              matrix cores (v_mfma_f32_16x16x32_bf16 over three-part BF16 splits of the
+ *              FP32 centred coordinates, FP32 accumulation), the target norm applied
+ *              exactly per target, exp2 and accumulation as DFM_FP32C.  Groups of 256
+ *              sorted targets spanning more than 4 scaled kernel widths, every other
+ *              operation, ragged batches and smaller calls run as DFM_FP32C.
  *
  * Supported shapes: dimpoint in {2, 3}; dimvect in {1, ..., 6} (the reference supports
  * dimvect in {D, 2D, D(D+1)/2}, CUDAExactKernel.cxx:56-82).  Counts are non-negative and
@@ -49,7 +56,7 @@ enum dfm_status {
   DFM_ERR_NOMEM = 6     /* device allocation failed (host-pointer wrappers only) */
 };
 
-enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2 };
+enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2, DFM_BF16MFMA = 3 };
 
 /* ABI version of this header (bumped on any signature or struct-layout change); callers
  * compare dfm_abi_version() of the loaded library with DFM_ABI_VERSION of the header they
