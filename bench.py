@@ -3,23 +3,24 @@ ConvolveGradient(X, alpha) on one MI355X.
 
 Workload (raw kernel sums at scale): N = M = 1,000,000 points in 3D, targets X and
 sources Y i.i.d. uniform in [0, 1]^3, weights W (M x 3) and alpha (N x 3) i.i.d. N(0, 1)
-(separate seeds), sigma = 0.05.  One step = one call of the FP32-compensated fused
-pair-sum (Morton sort of the targets + the pair-sum kernel) with every input already
-resident in HBM; value = N*M / (seconds per step).  FP64 inputs and outputs, as at the
-plugin boundary.
+(separate seeds), sigma = 0.05.  One step = one call of the fused pair-sum in the headline
+precision (default bf16mfma: the distances' cross term on the matrix cores, FP32 exp2 and
+accumulation with FP64 folds; Morton sort of the targets + the pair-sum kernel) with every
+input already resident in HBM; value = N*M / (seconds per step).  FP64 inputs and outputs,
+as at the plugin boundary.
 
 Beside the headline the run reports:
-  * the same step in plain FP64 (the reference's arithmetic on the GPU) and the speed-up;
-  * the same step with exact FP32 differences everywhere (fp32x) and with the distances'
-    cross term on the matrix cores (bf16mfma), each with its time and error;
-  * the FP32-compensated result's error against the FP64 GPU result over all N rows;
+  * the same step in the other FP32 forms (fp32c: FP32 VALU distances; fp32x: exact FP32
+    differences everywhere) and in plain FP64 (the reference's arithmetic on the GPU);
+  * each FP32 form's error against the FP64 GPU result over all N rows, and
+    `error_bound_met`: the headline's norm-wise error <= 1e-6;
   * `roofline`: the pair-sum kernel's duration from HIP events on its stream,
     29 algorithmic flops per pair (d=3, Dv=3) against the FP32 peak of 157.3 TFLOP/s;
   * `cpu_baseline`: the FP64 oracle (a C restatement of ExactKernel) timed on a bounded
     sample of target rows against all M sources on the host's cores, whose outputs also
-    give the FP32-compensated path's error against the FP64 oracle on those rows.
+    give each FP32 form's error against the FP64 oracle on those rows.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--m M]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--m M] [--precision P]
 
 Other workloads (one JSON line each, their own metric; the default is the headline):
     --workload shooting      landmark geodesic shooting, 1,000 control points, sigma 0.2,
@@ -47,12 +48,41 @@ import numpy as np
 
 FLOPS_PER_PAIR = 29           # fused Convolve (Dv=3) + ConvolveGradient(X, alpha), d = 3
 FP32_PEAK = 157.3e12          # MI355X FP32 vector peak, FLOP/s
-# HBM bytes per launch of the headline k_pairsum dispatch at N = M = 1M: rocprofv3 --pmc
-# FETCH_SIZE (5.97 GB) and WRITE_SIZE (0.816 GB, the FP64 split partials), each scaled by
-# the same counters on a 24 MB device copy (profiles/k_pairsum_convgrad1_N1M_sigma0.05.md).
-# Counter runs cannot be made from inside this process, so the figure is the profiled one.
-PROFILED_TRAFFIC_BYTES = 5.97e9 + 0.816e9
-PROFILED_TRAFFIC_SOURCE = "profiles/k_pairsum_convgrad1_N1M_sigma0.05.md"
+KERNELS = {
+    "bf16mfma": "k_pairsum_bf<CONVGRAD1, 3, 3, 4> + k_reduce_splits",
+    "fp32c": "k_pairsum<float, CONVGRAD1, 3, 3, 0, 2, ADAPTIVE> + k_reduce_splits",
+    "fp32x": "k_pairsum<float, CONVGRAD1, 3, 3, 0, 2, EXACT> + k_reduce_splits",
+}
+FORMS = {
+    "bf16mfma": "squared distances' cross term -2 x.y + |y|^2 on the matrix cores "
+                "(v_mfma_f32_16x16x32_bf16 over exact three-part BF16 splits of the FP32 "
+                "group-centred coordinates, FP32 accumulation), target norm applied exactly, "
+                "FP32 exp2 and accumulation with FP64 folds per 256-source tile",
+    "fp32c": "FP32 exp2/FMA on FP64-re-centred coordinates (expanded distance), FP64 folds "
+             "per 64-source tile, exact differences in wide wavefronts",
+    "fp32x": "FP32 exp2/FMA with exact (hi + lo) coordinate differences in every wavefront",
+}
+# HBM bytes per launch of the pair-sum dispatch at N = M = 1M, sigma 0.05: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a 24 MB device copy.
+# Counter runs cannot be made from inside this process, so the figures are the profiled ones.
+PROFILED = {
+    # k_pairsum_bf: FETCH_SIZE 6,270,480 KiB x 1.998 (the copy kernel's 24 MB read counts
+    # 11,731 KiB) = 12.83 GB, WRITE_SIZE 785,452 KiB x 1.0 = 0.80 GB (the FP64 split partials)
+    "bf16mfma": {"traffic": 12.83e9 + 0.80e9,
+                 "source": "profiles/k_pairsum_bf_convgrad1_N1M_sigma0.05.md",
+                 "note": "VALU-issue bound with matrix-core co-issue: two "
+                         "v_mfma_f32_16x16x32_bf16 per 256 pairs form the distances' cross "
+                         "term and alpha.w, the VALU keeps exp2 and the FP32 accumulation "
+                         "(9.7 VALU instructions per pair against 15.0 for fp32c); 13.6 GB per "
+                         "launch is 54 GB/s, under 1% of HBM bandwidth; the peak is the FP32 "
+                         "vector peak of the 29 algorithmic flops per pair"},
+    # k_pairsum<float, ..., R=2>: FETCH_SIZE 13,446,624 KiB x 1.998 = 27.51 GB, WRITE_SIZE
+    # 421,875 KiB = 0.43 GB
+    "fp32c": {"traffic": 27.51e9 + 0.43e9,
+              "source": "profiles/k_pairsum_convgrad1_N1M_sigma0.05.md",
+              "note": "VALU-issue bound, no matrix-core work (15.0 VALU instructions per pair); "
+                      "28 GB per launch is 89 GB/s, about 1% of HBM bandwidth"},
+}
 METRIC = "kernel pair-evals/s (Convolve+Gradient), FP32-comp vs FP64, at ≤1e-6 rel err"
 
 
@@ -68,6 +98,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="bf16mfma", choices=["bf16mfma", "fp32c", "fp32x"],
+                    help="the headline leg's precision")
     ap.add_argument("--workload", default="kernel",
                     choices=["kernel", "shooting", "registration", "atlas", "varifold",
                              "curves"])
@@ -133,29 +165,20 @@ def main():
         _lib.check(L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt)))
         return (t1 - t0) / steps, tot.value / 1e3 / max(cnt.value, 1)
 
-    # ---- FP32-compensated (headline)
+    # ---- the headline precision, then the other FP32 forms and plain FP64
+    head = args.precision
+    others = [p for p in ("fp32c", "bf16mfma", "fp32x") if p != head]
     for _ in range(args.warmup):
-        step("fp32c")
-    s_per_step, kern_s = timed("fp32c", args.steps)
-    c32 = conv.numpy(stream)
-    g32 = grad.numpy(stream)
-    print("fp32c: %.4f s/step, kernel %.4f s" % (s_per_step, kern_s), file=sys.stderr, flush=True)
-
-    # ---- FP32 with exact differences in every wavefront
-    step("fp32x")
-    s_x, kern_x = timed("fp32x", max(1, args.steps // 4))
-    cx = conv.numpy(stream)
-    gx = grad.numpy(stream)
-    print("fp32x: %.4f s/step, kernel %.4f s" % (s_x, kern_x), file=sys.stderr, flush=True)
-
-    # ---- DFM_BF16MFMA: the distances' cross term on the matrix cores
-    step("bf16mfma")
-    s_b, kern_b = timed("bf16mfma", max(1, args.steps // 4))
-    cb = conv.numpy(stream)
-    gb = grad.numpy(stream)
-    print("bf16mfma: %.4f s/step, kernel %.4f s" % (s_b, kern_b), file=sys.stderr, flush=True)
-
-    # ---- plain FP64 form on the GPU
+        step(head)
+    s_per_step, kern_s = timed(head, args.steps)
+    res = {head: (s_per_step, kern_s, conv.numpy(stream), grad.numpy(stream))}
+    print("%s: %.4f s/step, kernel %.4f s" % (head, s_per_step, kern_s), file=sys.stderr,
+          flush=True)
+    for prec in others:
+        step(prec)
+        s_o, k_o = timed(prec, max(1, args.steps // 4))
+        res[prec] = (s_o, k_o, conv.numpy(stream), grad.numpy(stream))
+        print("%s: %.4f s/step, kernel %.4f s" % (prec, s_o, k_o), file=sys.stderr, flush=True)
     step("fp64")
     s64, kern64 = timed("fp64", args.fp64_steps)
     c64 = conv.numpy(stream)
@@ -166,20 +189,22 @@ def main():
         return float(np.linalg.norm(ref - got) / np.linalg.norm(ref))
 
     def maxrms(ref, got):
-        """largest element error relative to max(|element|, RMS of the output)"""
-        scale = np.maximum(np.abs(ref), np.sqrt(np.mean(ref ** 2)))
-        return float(np.max(np.abs(ref - got) / scale))
+        """largest element error relative to the RMS of the output"""
+        return float(np.max(np.abs(ref - got)) / np.sqrt(np.mean(ref ** 2)))
 
-    err_fp64gpu = {
-        "rows": N,
-        "conv_normwise": nrel(c64, c32), "grad_normwise": nrel(g64, g32),
-        "conv_max_over_rms": maxrms(c64, c32), "grad_max_over_rms": maxrms(g64, g32),
-        "fp32x_conv_normwise": nrel(c64, cx), "fp32x_grad_normwise": nrel(g64, gx),
-        "fp32x_conv_max_over_rms": maxrms(c64, cx), "fp32x_grad_max_over_rms": maxrms(g64, gx),
-        "bf16mfma_conv_normwise": nrel(c64, cb), "bf16mfma_grad_normwise": nrel(g64, gb),
-        "bf16mfma_conv_max_over_rms": maxrms(c64, cb),
-        "bf16mfma_grad_max_over_rms": maxrms(g64, gb),
-    }
+    def errors(cref, gref, rows=None):
+        out = {}
+        for prec in [head] + others:
+            c, g = res[prec][2], res[prec][3]
+            if rows is not None:
+                c, g = c[rows], g[rows]
+            key = "" if prec == head else prec + "_"
+            out.update({key + "conv_normwise": nrel(cref, c), key + "grad_normwise": nrel(gref, g),
+                        key + "conv_max_over_rms": maxrms(cref, c),
+                        key + "grad_max_over_rms": maxrms(gref, g)})
+        return out
+
+    err_fp64gpu = {"rows": N, **errors(c64, g64)}
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of target rows
     cpu = None
@@ -210,19 +235,16 @@ def main():
                       "reference), %.1f s on %d threads" % (rows, M, tb - ta, threads),
             "scan": _cpu_scan(oracle, sigma, threads),
         }
-        err_oracle = {
-            "rows": rows,
-            "conv_normwise": nrel(oc, c32[idx]), "grad_normwise": nrel(og, g32[idx]),
-            "conv_max_over_rms": maxrms(oc, c32[idx]), "grad_max_over_rms": maxrms(og, g32[idx]),
-            "fp64_gpu_conv_normwise": nrel(oc, c64[idx]),
-            "fp64_gpu_grad_normwise": nrel(og, g64[idx]),
-            "bf16mfma_conv_normwise": nrel(oc, cb[idx]), "bf16mfma_grad_normwise": nrel(og, gb[idx]),
-            "bf16mfma_conv_max_over_rms": maxrms(oc, cb[idx]),
-            "bf16mfma_grad_max_over_rms": maxrms(og, gb[idx]),
-        }
+        err_oracle = {"rows": rows, **errors(oc, og, idx),
+                      "fp64_gpu_conv_normwise": nrel(oc, c64[idx]),
+                      "fp64_gpu_grad_normwise": nrel(og, g64[idx])}
+    within = (err_fp64gpu["conv_normwise"] <= 1e-6 and err_fp64gpu["grad_normwise"] <= 1e-6
+              and (err_oracle is None or (err_oracle["conv_normwise"] <= 1e-6
+                                          and err_oracle["grad_normwise"] <= 1e-6)))
 
     pairs = float(N) * float(M)
     achieved = FLOPS_PER_PAIR * pairs / kern_s
+    profiled = PROFILED[head] if (N, M, sigma) == (1000000, 1000000, 0.05) else None
     out = {
         "metric": METRIC,
         "value": pairs / s_per_step,
@@ -239,38 +261,37 @@ def main():
         "config": {
             "workload": "raw kernel sums: fused Convolve + ConvolveGradient(X, alpha), "
                         "N=M=%d points U[0,1]^3, d=3, Dv=3, W, alpha ~ N(0,1), sigma=%g; "
-                        "FP32 exp/FMA with FP64 re-centring and FP64 tile folds" % (N, sigma),
-            "N": N, "M": M, "d": 3, "dv": 3, "sigma": sigma, "precision": "fp32c",
+                        "%s" % (N, sigma, FORMS[head]),
+            "N": N, "M": M, "d": 3, "dv": 3, "sigma": sigma, "precision": head,
             "parallelism": "single GPU",
         },
+        "error_bound_met": within,
+        "error_bound": "normwise relative error <= 1e-6 of Convolve and of the gradient, "
+                       "against the FP64 GPU form (all rows) and the FP64 oracle (sampled rows)",
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK,
-            "traffic": PROFILED_TRAFFIC_BYTES if (N, M, sigma) == (1000000, 1000000, 0.05) else None,
+            "traffic": profiled["traffic"] if profiled else None,
             "traffic_unit": "bytes per launch",
-            "traffic_source": PROFILED_TRAFFIC_SOURCE,
-            "bound_note": "VALU-issue bound, neither HBM nor MFMA: the default plan issues no "
-                          "matrix-core instructions and reads 6.8 GB per launch (0.2% of HBM "
-                          "bandwidth); the peak is the FP32 vector peak (the dense FP32 MFMA "
-                          "figure is the same 157.3 TFLOP/s)",
-            "kernel": "k_pairsum<float, CONVGRAD1, 3, 3, 0, 1, ADAPTIVE> + k_reduce_splits",
+            "traffic_source": profiled["source"] if profiled else None,
+            "bound_note": profiled["note"] if profiled else None,
+            "kernel": KERNELS[head],
             "kernel_ms": kern_s * 1e3,
             "flops_per_pair": FLOPS_PER_PAIR,
         },
         "cpu_baseline": cpu,
         "fp64_plain_gpu": {"s_per_step": s64, "kernel_s": kern64, "pair_evals_per_s": pairs / s64,
-                           "speedup_fp32c_over_fp64": s64 / s_per_step},
-        "fp32x_exact_differences": {"s_per_step": s_x, "kernel_s": kern_x,
-                                    "pair_evals_per_s": pairs / s_x},
-        "bf16mfma_cross_term": {"s_per_step": s_b, "kernel_s": kern_b,
-                                "pair_evals_per_s": pairs / s_b,
-                                "kernel": "k_pairsum_bf<CONVGRAD1, 3, 3, 4> + k_reduce_splits"},
+                           "speedup_over_fp64": s64 / s_per_step},
         "error_vs_fp64_gpu": err_fp64gpu,
         "error_vs_fp64_oracle": err_oracle,
     }
+    for prec in others:
+        out[prec] = {"s_per_step": res[prec][0], "kernel_s": res[prec][1],
+                     "pair_evals_per_s": pairs / res[prec][0], "kernel": KERNELS[prec],
+                     "form": FORMS[prec]}
     print(json.dumps(out), flush=True)
 
 

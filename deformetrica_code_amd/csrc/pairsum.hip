@@ -238,8 +238,11 @@ struct Choice {
 static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max_nx) {
   (void)D; (void)DV;
   // DFM_BF16MFMA: workgroups of 256 targets (R = 4 in 64-target units) and 256-source tiles;
-  // the split count aims for 16384 workgroups
-  if (prec == DFM_BF16MFMA) return Choice{4, 16384, 1};
+  // the split count aims for 65536 workgroups.  At N = M = 1M (17 splits) the fused sum runs
+  // in 264 ms against 280 ms aiming for 16384 (4 splits) and 303 ms for 4096, same box,
+  // every candidate timed by the autotuner: the shorter splits also spread the FP32C
+  // wavefronts of the few wide groups over more of the chip.
+  if (prec == DFM_BF16MFMA) return Choice{4, 65536, 1};
   const long blocks = (max_nx + 63) / 64 * nseg;
   // FP32 Convolve / Grad1 / fused sums (the expanded-distance form) on large single grids:
   // two targets per lane share each LDS source record, halving the LDS reads that bound

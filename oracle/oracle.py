@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -64,6 +65,28 @@ def _c(a):
     return a
 
 
+# Host threads for the row-parallel sums below (each target row's sum is independent of the
+# others and computed by the same C loop, so the results do not depend on the split).
+_threads = max(1, min(16, os.cpu_count() or 1))
+
+
+def set_threads(n: int) -> None:
+    global _threads
+    _threads = max(1, int(n))
+
+
+def _rows(n, ny, call):
+    """call(r0, r1) fills target rows [r0, r1); large sums are split over host threads
+    (ctypes releases the GIL during the foreign call)."""
+    if _threads == 1 or n < 2 or n * ny < (1 << 22):
+        call(0, n)
+        return
+    parts = min(4 * _threads, n)
+    b = np.linspace(0, n, parts + 1).astype(np.int64)
+    with ThreadPoolExecutor(min(_threads, parts)) as ex:
+        list(ex.map(lambda k: call(int(b[k]), int(b[k + 1])), range(parts)))
+
+
 def _p(a):
     return a.ctypes.data_as(_D) if a is not None else None
 
@@ -107,8 +130,9 @@ class ExactKernel:
         X = _c(X)
         dv = self._W.shape[1]
         V = np.empty((X.shape[0], dv))
-        lib().orc_convolve(self.dim, dv, self.h, _p(X), X.shape[0], _p(self._Y), _p(self._W),
-                           self._Y.shape[0], _p(V))
+        Y, W, ny = self._Y, self._W, self._Y.shape[0]
+        _rows(X.shape[0], ny, lambda a, b: lib().orc_convolve(
+            self.dim, dv, self.h, _p(X[a:b]), b - a, _p(Y), _p(W), ny, _p(V[a:b])))
         return V
 
     def ConvolveGradient(self, X, arg=None):
@@ -117,20 +141,23 @@ class ExactKernel:
         X = _c(X)
         n, dv, d = X.shape[0], self._W.shape[1], self.dim
         ny = self._Y.shape[0]
+        Y, W = self._Y, self._W
         if arg is None:
             G = np.empty((n, dv, d))
-            lib().orc_varlin(d, dv, self.h, _p(X), n, _p(self._Y), _p(self._W), ny, _p(G))
+            _rows(n, ny, lambda a, b: lib().orc_varlin(
+                d, dv, self.h, _p(X[a:b]), b - a, _p(Y), _p(W), ny, _p(G[a:b])))
             return G
         if isinstance(arg, (int, np.integer)):
-            out = np.empty((n, dv))
-            if lib().orc_grad_dim(d, dv, self.h, _p(X), n, _p(self._Y), _p(self._W), ny,
-                                  int(arg), _p(out)) != 0:
+            if not 0 <= int(arg) < d:
                 raise RuntimeError("dimension index out of bounds")
+            out = np.empty((n, dv))
+            _rows(n, ny, lambda a, b: lib().orc_grad_dim(
+                d, dv, self.h, _p(X[a:b]), b - a, _p(Y), _p(W), ny, int(arg), _p(out[a:b])))
             return out
         alpha = _c(arg)
         out = np.empty((n, d))
-        lib().orc_grad1(d, dv, self.h, _p(X), _p(alpha), n, _p(self._Y), _p(self._W), ny,
-                        _p(out))
+        _rows(n, ny, lambda a, b: lib().orc_grad1(
+            d, dv, self.h, _p(X[a:b]), _p(alpha[a:b]), b - a, _p(Y), _p(W), ny, _p(out[a:b])))
         return out
 
     def ConvolveHessian(self, X, row=None, col=None):

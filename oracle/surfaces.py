@@ -183,3 +183,34 @@ def brute_force_samples(cs, Ns, ct, Nt, width, kind):
         return float(np.sum(K * (u1 @ u2.T) ** 2))
 
     return ip(cs, Ns, cs, Ns) + ip(ct, Nt, ct, Nt) - 2 * ip(cs, Ns, ct, Nt)
+
+
+def samples_varifold_gradient_rows(cs, Ns, ct, Nt, width, rows):
+    """d/d centres and d/d (area-weighted) normals of the varifold data term between oriented
+    samples, |S|^2 + |T|^2 - 2 <S, T> with <A, B> = sum_ij K(a_i, b_j) (u_i . v_j)^2 and
+    u = N / sqrt(|N| + 1e-20) (NonOrientedSurfaceMesh.cxx:269-308), at the source samples
+    ``rows`` only, from the kernel sums of NonOrientedSurfaceMesh.cxx:169-260 with every
+    sample of both sets as sources: Convolve and ConvolveGradient(X) with the Dv = 6 weights
+    M_j = u_j u_j^T (upper triangle), as varifold_match_gradient forms them for a mesh.
+      d/d c_i = 2 [u_i^T (dG_S - dG_T)[:, p] u_i]_p
+      d/d u_i = 4 (C_S - C_T) u_i,  d/d N_i = (g - u_i (u_i . g) / (2 |u_i|^2)) / |u_i|
+    (the chain rule through u = N / sqrt(|N|), as the reference applies it to a triangle's
+    normal).  Returns (rows x 3, rows x 3)."""
+    def unit_area(N):
+        u = N / np.sqrt(np.linalg.norm(N, axis=1, keepdims=True) + 1e-20)
+        return u, np.stack([u[:, p] * u[:, q] for p in range(3) for q in range(p, 3)], axis=1)
+
+    us, Ms = unit_area(np.asarray(Ns, dtype=np.float64))
+    _, Mt = unit_area(np.asarray(Nt, dtype=np.float64))
+    x = np.asarray(cs, dtype=np.float64)[rows]
+    u = us[rows]
+    kS, kT = _conv(cs, Ms, x, width), _conv(ct, Mt, x, width)
+    dC = kS.Convolve(x) - kT.Convolve(x)                    # (r, 6)
+    dG = kS.ConvolveGradient(x) - kT.ConvolveGradient(x)    # (r, 6, 3)
+    g = 4.0 * special_product(dC, u)
+    n2 = np.sum(u * u, axis=1, keepdims=True)
+    gN = (g - u * np.sum(u * g, axis=1, keepdims=True) / (2.0 * n2)) / np.sqrt(n2)
+    gc = np.stack([2.0 * np.sum(special_product(dG[:, :, p], u) * u, axis=1) for p in range(3)],
+                  axis=1)
+    return gc, gN
+

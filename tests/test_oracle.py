@@ -197,6 +197,29 @@ def test_threaded_matches_serial(oracle_mod):
     np.testing.assert_array_equal(g, k.ConvolveGradient(X, A))
 
 
+def test_row_parallel_sums_match_one_thread(oracle_mod):
+    """Sums split over host threads by target rows equal the one-thread sums bit for bit
+    (each row's sum is the same C loop whichever thread runs it)."""
+    rng = np.random.default_rng(51)
+    X = rng.uniform(size=(3000, 3))
+    A = rng.normal(size=(3000, 3))
+    Y = rng.uniform(size=(2000, 3))
+    W = rng.normal(size=(2000, 6))
+    k = oracle_mod.ExactKernel(3, 0.2)
+    k.SetSources(Y)
+    k.SetWeights(W)
+    out = []
+    for t in (1, 6):
+        oracle_mod.set_threads(t)
+        out.append((k.Convolve(X), k.ConvolveGradient(X), k.ConvolveGradient(X, A[:, :3].repeat(2, 1)),
+                    k.ConvolveGradient(X, 2)))
+    oracle_mod.set_threads(16)
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(RuntimeError, match="out of bounds"):
+        k.ConvolveGradient(X, 3)
+
+
 def test_mismatch_raises(oracle_mod):
     k = oracle_mod.ExactKernel(3, 1.0)
     k.SetSources(np.zeros((4, 3)))

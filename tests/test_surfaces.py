@@ -73,6 +73,32 @@ def test_sample_brute_force_equals_mesh_match(kind):
     assert abs(so.brute_force_samples(cs, ns, ct, nt, 0.3, kind) - m) <= 1e-12 * abs(m)
 
 
+def test_sample_varifold_gradient_rows_match_central_differences():
+    """The row-restricted varifold sample gradient (kernel sums with Dv = 6 weights) against
+    central differences of the definitional double sum, for centres and normals."""
+    from deformetrica_code_amd.meshes import sphere_samples
+    from oracle import surfaces as so
+    rng = np.random.default_rng(11)
+    cs, Ns = sphere_samples(60, rng, -3.0, 0.3)
+    ct, Nt = sphere_samples(50, rng, -3.0, 0.3)
+    ct = ct + 0.02
+    w = 0.3
+    rows = np.array([0, 7, 33, 59])
+    gc, gN = so.samples_varifold_gradient_rows(cs, Ns, ct, Nt, w, rows)
+    scale = max(np.abs(gc).max(), np.abs(gN).max())
+    for r, i in enumerate(rows):
+        for k in range(3):
+            for arr, g in ((cs, gc), (Ns, gN)):
+                e = 1e-6 * (1.0 if arr is cs else np.abs(Ns[i]).max())
+                p, q = arr.copy(), arr.copy()
+                p[i, k] += e
+                q[i, k] -= e
+                ap, aq = ((p, Ns), (q, Ns)) if arr is cs else ((cs, p), (cs, q))
+                fd = (so.brute_force_samples(*ap, ct, Nt, w, "varifold") -
+                      so.brute_force_samples(*aq, ct, Nt, w, "varifold")) / (2 * e)
+                assert abs(g[r, k] - fd) <= 1e-7 * scale, (i, k, g[r, k], fd)
+
+
 def test_icosphere_counts_and_orientation():
     from deformetrica_code_amd.meshes import icosphere, bump_field, decimated_icosphere
     from oracle import surfaces as so
