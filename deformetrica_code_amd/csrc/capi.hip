@@ -2,13 +2,19 @@
 //
 // Device entry points validate their arguments the way the reference plugin does
 // (CUDAExactKernel.cxx: row-count mismatch, unsupported dimvect, dim out of range) and
-// return a status code; host-pointer wrappers reproduce the reference's
-// GaussGpu*Conv1D host functions (GpuConv1D.cu:88-133 and siblings): allocate, copy in,
-// launch, synchronise, copy out, free — with every HIP error checked.
+// return a status code; host-pointer wrappers take the place of the reference's
+// GaussGpu*Conv1D host functions (GpuConv1D.cu:88-133 and siblings), which allocate, copy
+// in, launch, synchronise, copy out and free on every call: here the device buffers are
+// leased from a pool that keeps them resident between calls (grown when a call needs more,
+// never freed per call), so a call is copy in, launch, copy out and one synchronisation of
+// the calling thread's stream — with every HIP error checked.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
+#include <mutex>
+#include <vector>
 
 #include "dfm_hip.h"
 #include "dfm_internal.h"
@@ -315,44 +321,171 @@ int dfm_graddiff(void *stream, int D, int DV, double sigma, int prec, const doub
 
 namespace {
 
-struct DevBuf {
-  void *p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+// A set of device buffers and one pinned host staging buffer for one host-pointer call.
+// Arenas live in a per-device pool for the life of the process: a call leases one (another
+// thread's concurrent call leases another), and a buffer is reallocated only when a call
+// needs more than it holds, so a steady sequence of calls (a shoot, a subject loop)
+// allocates and frees nothing.  A call packs its inputs into the staging buffer for one
+// host-to-device copy, and its outputs come back in one device-to-host copy.
+struct Arena {
+  static constexpr int kSlots = 4;  // packed inputs, packed outputs, workspace
+  int device = 0;
+  void *p[kSlots] = {};
+  size_t cap[kSlots] = {};
+  void *host = nullptr;      // pinned and mapped (hipHostMalloc)
+  void *host_dev = nullptr;  // its device address
+  size_t host_cap = 0;
 };
 
-int dev_alloc(DevBuf &b, size_t bytes) {
-  if (bytes == 0) return DFM_OK;
-  hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess) {
-    b.p = nullptr;
-    return set_error(DFM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+
+std::mutex g_pool_mu;
+std::vector<Arena *> g_pool;  // idle arenas, any device
+
+constexpr size_t kAlign = 256;
+size_t aligned(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+struct Piece {
+  const double *h;
+  size_t n;  // doubles
+};
+
+struct Lease {
+  Arena *a = nullptr;
+  int next = 0;
+  bool synced = true;  // no copy or kernel of this call can still be in flight
+
+  int init() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(DFM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    for (size_t i = 0; i < g_pool.size(); i++)
+      if (g_pool[i]->device == dev) {
+        a = g_pool[i];
+        g_pool.erase(g_pool.begin() + (long)i);
+        return DFM_OK;
+      }
+    a = new Arena;
+    a->device = dev;
+    return DFM_OK;
   }
-  return DFM_OK;
-}
+  // Slot `next` holding at least `bytes` (grown by 1/4 beyond the request, so slowly
+  // growing calls reallocate seldom).
+  int buf(size_t bytes, void **out) {
+    *out = nullptr;
+    if (bytes == 0) return DFM_OK;
+    if (next >= Arena::kSlots) return set_error(DFM_ERR_ARG, "internal: arena slots exhausted");
+    const int i = next++;
+    if (a->cap[i] < bytes) {
+      if (a->p[i]) {
+        (void)hipFree(a->p[i]);
+        a->p[i] = nullptr;
+        a->cap[i] = 0;
+      }
+      const size_t want = bytes + bytes / 4;
+      hipError_t e = hipMalloc(&a->p[i], want);
+      if (e != hipSuccess) {
+        a->p[i] = nullptr;
+        return set_error(DFM_ERR_NOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+      }
+      a->cap[i] = want;
+    }
+    *out = a->p[i];
+    synced = false;
+    return DFM_OK;
+  }
+  int staging(size_t bytes) {
+    if (a->host_cap >= bytes) return DFM_OK;
+    if (a->host) {
+      // this call's input copy may still be reading the buffer about to be replaced
+      (void)hipStreamSynchronize(hipStreamPerThread);
+      (void)hipHostFree(a->host);
+      a->host = a->host_dev = nullptr;
+      a->host_cap = 0;
+    }
+    const size_t want = bytes + bytes / 4;
+    hipError_t e = hipHostMalloc(&a->host, want, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&a->host_dev, a->host, 0);
+    if (e != hipSuccess) {
+      if (a->host) (void)hipHostFree(a->host);
+      a->host = a->host_dev = nullptr;
+      return set_error(DFM_ERR_NOMEM, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    a->host_cap = want;
+    return DFM_OK;
+  }
+  // The inputs packed into one device buffer with one copy; d[i] receives piece i's device
+  // address (nullptr for an empty piece).
+  int upload(std::initializer_list<Piece> pieces, double **d) {
+    size_t total = 0;
+    for (const Piece &q : pieces) total += aligned(q.n * sizeof(double));
+    void *dev = nullptr;
+    int rc = buf(total, &dev);
+    if (!rc) rc = staging(total);
+    if (rc) return rc;
+    size_t off = 0;
+    int i = 0;
+    for (const Piece &q : pieces) {
+      d[i++] = q.n ? (double *)((char *)dev + off) : nullptr;
+      if (q.n) std::memcpy((char *)a->host + off, q.h, q.n * sizeof(double));
+      off += aligned(q.n * sizeof(double));
+    }
+    // the staging copies are copy kernels on the call's stream reading (or writing) the
+    // mapped pinned buffer: small DMA copies from several threads' streams queue on the few
+    // copy engines (8 threads: 84,000 calls/s against 28,000 with DMA copies)
+    return total ? copy_d2d(hipStreamPerThread, dev, a->host_dev, total) : DFM_OK;
+  }
+  // Device buffers for the outputs, packed in one allocation (counts in doubles).
+  int outputs(std::initializer_list<size_t> counts, double **d) {
+    size_t total = 0;
+    for (size_t n : counts) total += aligned(n * sizeof(double));
+    void *dev = nullptr;
+    int rc = buf(total, &dev);
+    if (rc) return rc;
+    size_t off = 0;
+    int i = 0;
+    for (size_t n : counts) {
+      d[i++] = n ? (double *)((char *)dev + off) : nullptr;
+      off += aligned(n * sizeof(double));
+    }
+    out_dev = dev;
+    out_total = total;
+    return DFM_OK;
+  }
+  // One copy of the packed outputs back, the call's single synchronisation, then the
+  // outputs to the caller's arrays (a null host pointer skips that output).
+  int download(std::initializer_list<double *> hosts, std::initializer_list<size_t> counts) {
+    int rc = staging(out_total);
+    if (rc) return rc;
+    if (out_total) {
+      rc = copy_d2d(hipStreamPerThread, a->host_dev, out_dev, out_total);
+      if (rc) return rc;
+    }
+    hipError_t e = hipStreamSynchronize(hipStreamPerThread);
+    if (e != hipSuccess) return set_error(DFM_ERR_HIP, "sync: %s", hipGetErrorString(e));
+    synced = true;
+    size_t off = 0;
+    auto h = hosts.begin();
+    for (size_t n : counts) {
+      if (*h && n) std::memcpy(*h, (char *)a->host + off, n * sizeof(double));
+      off += aligned(n * sizeof(double));
+      ++h;
+    }
+    return DFM_OK;
+  }
+  ~Lease() {
+    if (!a) return;
+    // an early error return can leave this call's copies queued: let them finish before
+    // another call reuses (or regrows) the buffers
+    if (!synced) (void)hipStreamSynchronize(hipStreamPerThread);
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    g_pool.push_back(a);
+  }
 
-int upload(DevBuf &b, const double *h, size_t n) {
-  if (n == 0) return DFM_OK;
-  int rc = dev_alloc(b, n * sizeof(double));
-  if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice,
-                                hipStreamPerThread);
-  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "H2D: %s", hipGetErrorString(e));
-}
-
-int download(double *h, const DevBuf &b, size_t n) {
-  if (n == 0) return DFM_OK;
-  hipError_t e = hipMemcpyAsync(h, b.p, n * sizeof(double), hipMemcpyDeviceToHost,
-                                hipStreamPerThread);
-  if (e == hipSuccess) e = hipStreamSynchronize(hipStreamPerThread);
-  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "D2H: %s", hipGetErrorString(e));
-}
-
-int finish() {
-  hipError_t e = hipStreamSynchronize(hipStreamPerThread);
-  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "sync: %s", hipGetErrorString(e));
-}
+ private:
+  void *out_dev = nullptr;
+  size_t out_total = 0;
+};
 
 #define TRY(expr)               \
   do {                          \
@@ -370,17 +503,17 @@ int dfm_gauss_conv(int D, int DV, double sigma, const double *x, const double *y
   TRY(check_common(D, DV, prec, sigma, nx, ny));
   if (nx == 0) return DFM_OK;
   if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, dy, db, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  TRY(dev_alloc(dg, (size_t)nx * DV * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[3], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {y, (size_t)ny * D}, {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({(size_t)nx * DV}, out));
   size_t ws = dfm_workspace_size(DFM_OP_CONV, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_conv(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
-               (double *)db.p, ny, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * DV));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_conv(hipStreamPerThread, D, DV, sigma, prec, in[0], nx, in[1], in[2], ny, out[0], dw,
+               ws));
+  return L.download({gamma}, {(size_t)nx * DV});
 }
 
 int dfm_gauss_grad1conv(int D, int DV, double sigma, const double *alpha, const double *x,
@@ -391,18 +524,18 @@ int dfm_gauss_grad1conv(int D, int DV, double sigma, const double *alpha, const 
   if (nx == 0) return DFM_OK;
   if (!x || !alpha || !gamma || (ny > 0 && (!y || !beta)))
     return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, da, dy, db, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(da, alpha, (size_t)nx * DV));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[4], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {alpha, (size_t)nx * DV}, {y, (size_t)ny * D},
+                {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({(size_t)nx * D}, out));
   size_t ws = dfm_workspace_size(DFM_OP_GRAD1, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_grad1(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)da.p, nx,
-                (double *)dy.p, (double *)db.p, ny, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * D));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_grad1(hipStreamPerThread, D, DV, sigma, prec, in[0], in[1], nx, in[2], in[3], ny,
+                out[0], dw, ws));
+  return L.download({gamma}, {(size_t)nx * D});
 }
 
 int dfm_gauss_gradconv(int D, int DV, double sigma, const double *x, const double *y,
@@ -413,17 +546,17 @@ int dfm_gauss_gradconv(int D, int DV, double sigma, const double *x, const doubl
   if (dim >= (unsigned)D) return set_error(DFM_ERR_INDEX, "dimension index out of bounds");
   if (nx == 0) return DFM_OK;
   if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, dy, db, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  TRY(dev_alloc(dg, (size_t)nx * DV * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[3], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {y, (size_t)ny * D}, {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({(size_t)nx * DV}, out));
   size_t ws = dfm_workspace_size(DFM_OP_GRADDIM, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_graddim(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
-                  (double *)db.p, ny, (int)dim, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * DV));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_graddim(hipStreamPerThread, D, DV, sigma, prec, in[0], nx, in[1], in[2], ny, (int)dim,
+                  out[0], dw, ws));
+  return L.download({gamma}, {(size_t)nx * DV});
 }
 
 int dfm_gauss_gradconv_varlin(int D, int DV, double sigma, const double *x, const double *y,
@@ -432,17 +565,17 @@ int dfm_gauss_gradconv_varlin(int D, int DV, double sigma, const double *x, cons
   TRY(check_common(D, DV, prec, sigma, nx, ny));
   if (nx == 0) return DFM_OK;
   if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, dy, db, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  TRY(dev_alloc(dg, (size_t)nx * DV * D * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[3], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {y, (size_t)ny * D}, {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({(size_t)nx * DV * D}, out));
   size_t ws = dfm_workspace_size(DFM_OP_VARLIN, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_varlin(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
-                 (double *)db.p, ny, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * DV * D));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_varlin(hipStreamPerThread, D, DV, sigma, prec, in[0], nx, in[1], in[2], ny, out[0], dw,
+                 ws));
+  return L.download({gamma}, {(size_t)nx * DV * D});
 }
 
 int dfm_gauss_hessianconv(int D, int DV, double sigma, const double *x, const double *y,
@@ -451,17 +584,17 @@ int dfm_gauss_hessianconv(int D, int DV, double sigma, const double *x, const do
   TRY(check_common(D, DV, prec, sigma, nx, ny));
   if (nx == 0) return DFM_OK;
   if (!x || !gamma || (ny > 0 && (!y || !beta))) return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, dy, db, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  TRY(dev_alloc(dg, (size_t)nx * DV * D * D * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[3], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {y, (size_t)ny * D}, {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({(size_t)nx * DV * D * D}, out));
   size_t ws = dfm_workspace_size(DFM_OP_HESSIAN, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_hessian(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, nx, (double *)dy.p,
-                  (double *)db.p, ny, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * DV * D * D));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_hessian(hipStreamPerThread, D, DV, sigma, prec, in[0], nx, in[1], in[2], ny, out[0], dw,
+                  ws));
+  return L.download({gamma}, {(size_t)nx * DV * D * D});
 }
 
 int dfm_gauss_graddiffconv(int D, int DV, double sigma, const double *x, const double *beta,
@@ -471,17 +604,17 @@ int dfm_gauss_graddiffconv(int D, int DV, double sigma, const double *x, const d
   if (DV != D) return set_error(DFM_ERR_DIM, "Invalid number of columns of beta");
   if (nx == 0) return DFM_OK;
   if (!x || !beta || !eta || !gamma) return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, db, de, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  TRY(upload(db, beta, (size_t)nx * DV));
-  TRY(upload(de, eta, (size_t)nx * D));
-  TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[3], *out[1];
+  void *dw;
+  TRY(L.upload({{x, (size_t)nx * D}, {beta, (size_t)nx * DV}, {eta, (size_t)nx * D}}, in));
+  TRY(L.outputs({(size_t)nx * D}, out));
   size_t ws = dfm_workspace_size(DFM_OP_GRADDIFF, D, DV, nx, nx, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_graddiff(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)db.p,
-                   (double *)de.p, nx, (double *)dg.p, dw.p, ws));
-  TRY(download(gamma, dg, (size_t)nx * D));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_graddiff(hipStreamPerThread, D, DV, sigma, prec, in[0], in[1], in[2], nx, out[0], dw,
+                   ws));
+  return L.download({gamma}, {(size_t)nx * D});
 }
 
 int dfm_gauss_conv_grad1(int D, int DV, double sigma, const double *x, const double *alpha,
@@ -492,22 +625,20 @@ int dfm_gauss_conv_grad1(int D, int DV, double sigma, const double *x, const dou
   if (nx == 0 || (!conv && !grad)) return DFM_OK;
   if (!x || (grad && !alpha) || (ny > 0 && (!y || !beta)))
     return set_error(DFM_ERR_ARG, "NULL argument");
-  DevBuf dx, da, dy, db, dc, dg, dw;
-  TRY(upload(dx, x, (size_t)nx * D));
-  if (grad) TRY(upload(da, alpha, (size_t)nx * DV));
-  TRY(upload(dy, y, (size_t)ny * D));
-  TRY(upload(db, beta, (size_t)ny * DV));
-  if (conv) TRY(dev_alloc(dc, (size_t)nx * DV * sizeof(double)));
-  if (grad) TRY(dev_alloc(dg, (size_t)nx * D * sizeof(double)));
+  Lease L;
+  TRY(L.init());
+  double *in[4], *out[2];
+  void *dw;
+  const size_t nc = conv ? (size_t)nx * DV : 0, ng = grad ? (size_t)nx * D : 0;
+  TRY(L.upload({{x, (size_t)nx * D}, {alpha, grad ? (size_t)nx * DV : 0}, {y, (size_t)ny * D},
+                {beta, (size_t)ny * DV}}, in));
+  TRY(L.outputs({nc, ng}, out));
   const int op = conv && grad ? DFM_OP_CONV_GRAD1 : conv ? DFM_OP_CONV : DFM_OP_GRAD1;
   size_t ws = dfm_workspace_size(op, D, DV, nx, ny, prec);
-  TRY(dev_alloc(dw, ws));
-  TRY(dfm_conv_grad1(hipStreamPerThread, D, DV, sigma, prec, (double *)dx.p, (double *)da.p, nx,
-                     (double *)dy.p, (double *)db.p, ny, (double *)dc.p, (double *)dg.p, dw.p,
-                     ws));
-  if (conv) TRY(download(conv, dc, (size_t)nx * DV));
-  if (grad) TRY(download(grad, dg, (size_t)nx * D));
-  return finish();
+  TRY(L.buf(ws, &dw));
+  TRY(dfm_conv_grad1(hipStreamPerThread, D, DV, sigma, prec, in[0], in[1], nx, in[2], in[3], ny,
+                     out[0], out[1], dw, ws));
+  return L.download({conv, grad}, {nc, ng});
 }
 
 

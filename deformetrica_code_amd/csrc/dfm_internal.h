@@ -40,12 +40,13 @@ constexpr int kFoldF32 = 64;
 constexpr long kSortMinTargets = 8192;
 
 // DFM_BF16MFMA: targets per workgroup of the matrix-core kernel (4 wavefronts x 4 tiles of
-// 16), which also decides, per group of this many sorted targets, which kernel runs them;
+// 16), which also decides, per group of this many sorted targets, which path sums them;
 // and sources per LDS tile (one staging thread each)
 constexpr int kBfGroup = 256;
 constexpr int kBfTile = 256;
 // DFM_BF16MFMA: a group of kBfGroup targets spanning more than this many scaled kernel units
-// runs on k_pairsum instead (its coordinates, and so the cross term's rounding, would grow)
+// takes the FP32C wavefront path in the same workgroup instead (its coordinates, and so the
+// cross term's rounding, would grow)
 constexpr double kBfExtent = 6.0;
 
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
@@ -87,8 +88,7 @@ struct KParams {
   double ch2;  // GradDiff: -4 / (sigma^4 s^2)
   double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
   double exact_extent_bf;  // DFM_BF16MFMA: group extent above which k_pairsum runs the group
-  int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores);
-                        // 2 only those of wide groups of kBfGroup targets (DFM_BF16MFMA)
+  int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores)
 };
 
 // Work decomposition and workspace layout of one call.

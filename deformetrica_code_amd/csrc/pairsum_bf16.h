@@ -23,10 +23,10 @@
 // A wavefront holds 4 tiles of 16 targets; four wavefronts share the workgroup's 256
 // targets and one LDS tile of 256 staged sources (A rows 64 B, FP32 y and w).
 //
-// A group whose targets span more than kBfExtent scaled units is left to k_pairsum
-// with wave_filter = 2, which runs exactly the wavefronts of such groups (the same FP64
-// min / max over the same targets decides both).  Same accumulator layout, split partials
-// and finalize as k_pairsum: k_reduce_splits is shared.
+// A group whose targets span more than kBfExtent scaled units (where the Morton order
+// jumps; 1.4% of the targets of 1M uniform points) is summed in the same workgroup by the
+// DFM_FP32C wavefront code (sum_wavefront), one wavefront per 64 targets.  Same accumulator
+// layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
 //
 // The reference arithmetic this replaces: the per-pair squared distance of
 // ScalarRadialKernel.h:61-73, tiled over threads in GpuConv1D.cu:55-78.
@@ -169,8 +169,25 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     extent = fmax(extent, hi[k] - lo[k]);
   }
   const double s = p.s;
-  // a wide group is left to k_pairsum (wave_filter 2), which applies the complementary test
-  if (!(extent * s <= p.exact_extent_bf)) return;
+  if (!(extent * s <= p.exact_extent_bf)) {
+    // a wide group (its sorted targets straddle a jump of the Morton order): each wavefront
+    // sums its 64 targets as DFM_FP32C does, on its own centre (exact differences where its
+    // own targets are wide), with a slice of smA as its LDS tile.  All four wavefronts pass
+    // the same barriers: the same source range, every lane taking part.
+    constexpr int NREC = Shape<OP, D, DV>::NSPX;
+    static_assert(4 * kTileF32 * NREC * sizeof(float) <= sizeof(smA), "tiles fit smA");
+    long wpos[1], worig[1];
+    bool wvalid[1];
+    double wxd[1][D], wc[D], wext;
+    load_targets<float, D, 1>(p, lane, gbase + 64 * wave, p.nx, wpos, worig, wvalid, wxd, wc,
+                              wext);
+    const long wjs = (long)split * p.split_len;
+    const long wje = std::min(p.ny, wjs + p.split_len);
+    float *wsm = reinterpret_cast<float *>(smA) + wave * (kTileF32 * NREC);
+    sum_wavefront<float, OP, D, DV, 0, 1, MODE_ADAPTIVE>(p, wsm, lane, split, wjs, wje, wpos, worig,
+                                                        wvalid, wxd, wc, wext);
+    return;
+  }
 
   // ---- this wavefront's targets: tile t, column n -> position gbase + 64 wave + 16 t + n
   const int h = lane >> 4, n = lane & 15;

@@ -1,6 +1,6 @@
 // pairsum_bf16.hip — launches of the DFM_BF16MFMA pair sums: the matrix-core kernel
-// (k_pairsum_bf) over the compact groups of 256 sorted targets, k_pairsum (FP32C, one
-// target per lane) over the wavefronts of the wide groups, then the split reduction.
+// (k_pairsum_bf; its wide groups of 256 sorted targets take the FP32C wavefront path in the
+// same launch), then the split reduction.
 #include "pairsum_bf16.h"
 
 namespace dfm {
@@ -10,10 +10,6 @@ template <int OP, int D, int DV>
 void launch_bf(hipStream_t st, const KParams &p, const Plan &pl) {
   const dim3 gbf((unsigned)((p.nx + kBfGroup - 1) / kBfGroup), (unsigned)pl.nsplit, 1);
   hipLaunchKernelGGL((k_pairsum_bf<OP, D, DV, kBfGroup / 64>), gbf, dim3(256), 0, st, p);
-  KParams q = p;
-  q.wave_filter = 2;
-  const dim3 gw((unsigned)((p.nx + 63) / 64), (unsigned)pl.nsplit, 1);
-  hipLaunchKernelGGL((k_pairsum<float, OP, D, DV, 0, 1, MODE_ADAPTIVE>), gw, dim3(64), 0, st, q);
   if (pl.nsplit > 1)
     hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
                        dim3(256), 0, st, p);

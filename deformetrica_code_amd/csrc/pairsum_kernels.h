@@ -472,39 +472,14 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
   }
 }
 
-template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
-__global__ __launch_bounds__(64) void k_pairsum(KParams p) {
-  using S = Shape<OP, D, DV>;
-  constexpr int NACC = S::NACC;
-  constexpr int NA = S::NACCMAX;
-  constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
-  constexpr int TS = Prec<T>::kTile;
-  constexpr int FOLD = Prec<T>::kFold;
-  static_assert(TS % FOLD == 0 && FOLD % 4 == 0, "fold interval divides the tile");
-  static_assert(S::NSP % 4 == 0 && S::NSPX % 4 == 0, "records are whole 16-byte groups");
-
-  __shared__ __attribute__((aligned(16))) T sm[TS * (Prec<T>::kCenter ? S::NSPX : S::NSP)];
-
-  const int lane = threadIdx.x;
-  const int split = blockIdx.y;
-  // segment of a ragged batch (one segment: the whole target and source sets)
-  long xb = 0, xe = p.nx, yb = 0, ye = p.ny;
-  if (p.xoff) {
-    xb = p.xoff[blockIdx.z];
-    xe = p.xoff[blockIdx.z + 1];
-  }
-  if (p.yoff) {
-    yb = p.yoff[blockIdx.z];
-    ye = p.yoff[blockIdx.z + 1];
-  }
-  const long base = xb + (long)blockIdx.x * (64 * R);
-  if (base >= xe) return;  // the whole wavefront lies past its segment
-
-  // ---- targets -> registers
-  long pos[R];
-  long orig[R];
-  bool valid[R];
-  double xd[R][D];
+// Targets of one wavefront (R per lane, positions base + 64 r + lane of [.., xe)) into
+// registers, with the centre of their bounding box and its extent (FP32 paths).  Lanes past
+// the end are marked invalid and read nothing.
+template <typename T, int D, int R>
+__device__ __forceinline__ void load_targets(const KParams &p, int lane, long base, long xe,
+                                             long (&pos)[R], long (&orig)[R], bool (&valid)[R],
+                                             double (&xd)[R][D], double (&c)[D],
+                                             double &extent) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     pos[r] = base + r * 64 + lane;
@@ -513,10 +488,7 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
 #pragma unroll
     for (int k = 0; k < D; k++) xd[r][k] = valid[r] ? p.x[orig[r] * D + k] : 0.0;
   }
-
-  // centre of the wavefront's targets (FP32 paths) and their extent in kernel units
-  double c[D];
-  double extent = 0.0;
+  extent = 0.0;
 #pragma unroll
   for (int k = 0; k < D; k++) c[k] = 0.0;
   if constexpr (Prec<T>::kCenter) {
@@ -535,39 +507,24 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
       extent = (lo <= hi) ? fmax(extent, hi - lo) : extent;
     }
   }
+}
 
+// The sums of one wavefront's targets over the sources [js, je) streamed through its LDS
+// tile `sm`, stored through finalize (one split) or as split partials.  Every lane of the
+// wavefront takes part (invalid lanes sit at the centre and store nothing), and the number
+// of __syncthreads() depends on js, je only, so the wavefronts of a larger workgroup that
+// call this with the same source range pass the same barriers.
+template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
+__device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane, int split,
+                                              long js, long je, const long (&pos)[R],
+                                              const long (&orig)[R], const bool (&valid)[R],
+                                              const double (&xd)[R][D], const double (&c)[D],
+                                              double extent) {
+  using S = Shape<OP, D, DV>;
+  constexpr int NACC = S::NACC;
+  constexpr int NA = S::NACCMAX;
+  constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
   const double s = p.s;
-  if constexpr (Prec<T>::kCenter) {
-    // the compact wavefronts of this call run on the matrix cores (k_pairsum_mx)
-    if (p.wave_filter == 1 && !(extent * s > p.exact_extent)) return;
-    if (p.wave_filter == 2) {
-      // DFM_BF16MFMA: k_pairsum_bf ran this wavefront's group of kBfGroup sorted targets
-      // unless the group is wide; the same exact FP64 min / max over the same targets
-      // decides in both kernels (single segment, one target per lane)
-      const long gb = base & ~(long)(kBfGroup - 1);
-      double glo[D], ghi[D];
-#pragma unroll
-      for (int k = 0; k < D; k++) {
-        glo[k] = INFINITY;
-        ghi[k] = -INFINITY;
-      }
-      for (int q = 0; q < kBfGroup / 64; q++) {
-        const long gp = gb + 64 * q + lane;
-        if (gp < xe) {
-          const long o = p.perm ? (long)p.perm[gp] : gp;
-#pragma unroll
-          for (int k = 0; k < D; k++) {
-            glo[k] = fmin(glo[k], p.x[o * D + k]);
-            ghi[k] = fmax(ghi[k], p.x[o * D + k]);
-          }
-        }
-      }
-      double gext = 0.0;
-#pragma unroll
-      for (int k = 0; k < D; k++) gext = fmax(gext, wave_max(ghi[k]) - wave_min(glo[k]));
-      if (gext * s <= p.exact_extent_bf) return;
-    }
-  }
   T xt[R][D];
   T xl[R][D];
   T te[R][NTE];
@@ -603,8 +560,6 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
 #pragma unroll
     for (int a = 0; a < NA; a++) accd[r][a] = 0.0;
 
-  const long js = yb + (long)split * p.split_len;
-  const long je = std::min(ye, js + p.split_len);
   if constexpr (Prec<T>::kCenter) {
     // wavefront-uniform choice: coordinates re-centred on a box wider than a few kernel
     // widths lose too much to FP32 rounding, so such wavefronts form exact differences
@@ -638,6 +593,45 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
       for (int a = 0; a < NACC; a++) dst[a] = accd[r][a];
     }
   }
+}
+
+template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
+__global__ __launch_bounds__(64) void k_pairsum(KParams p) {
+  using S = Shape<OP, D, DV>;
+  constexpr int TS = Prec<T>::kTile;
+  constexpr int FOLD = Prec<T>::kFold;
+  static_assert(TS % FOLD == 0 && FOLD % 4 == 0, "fold interval divides the tile");
+  static_assert(S::NSP % 4 == 0 && S::NSPX % 4 == 0, "records are whole 16-byte groups");
+
+  __shared__ __attribute__((aligned(16))) T sm[TS * (Prec<T>::kCenter ? S::NSPX : S::NSP)];
+
+  const int lane = threadIdx.x;
+  const int split = blockIdx.y;
+  // segment of a ragged batch (one segment: the whole target and source sets)
+  long xb = 0, xe = p.nx, yb = 0, ye = p.ny;
+  if (p.xoff) {
+    xb = p.xoff[blockIdx.z];
+    xe = p.xoff[blockIdx.z + 1];
+  }
+  if (p.yoff) {
+    yb = p.yoff[blockIdx.z];
+    ye = p.yoff[blockIdx.z + 1];
+  }
+  const long base = xb + (long)blockIdx.x * (64 * R);
+  if (base >= xe) return;  // the whole wavefront lies past its segment
+
+  long pos[R], orig[R];
+  bool valid[R];
+  double xd[R][D], c[D], extent;
+  load_targets<T, D, R>(p, lane, base, xe, pos, orig, valid, xd, c, extent);
+  if constexpr (Prec<T>::kCenter) {
+    // the compact wavefronts of this call run on the matrix cores (k_pairsum_mx)
+    if (p.wave_filter == 1 && !(extent * p.s > p.exact_extent)) return;
+  }
+  const long js = yb + (long)split * p.split_len;
+  const long je = std::min(ye, js + p.split_len);
+  sum_wavefront<T, OP, D, DV, AUX, R, MODE>(p, sm, lane, split, js, je, pos, orig, valid, xd, c,
+                                            extent);
 }
 
 template <int OP, int D, int DV>

@@ -116,7 +116,12 @@ int dfm_autotune_reset(void);
 int dfm_autotune_report(char *buf, size_t len);
 
 /* ------------------------------------------------------------------------------------ */
-/* Host-pointer wrappers (synchronous; allocate, copy, launch, copy back, free).        */
+/* Host-pointer wrappers (synchronous: copy in, launch, copy back on the calling thread's
+ * per-thread stream, one synchronisation).  Their device buffers come from a per-device
+ * pool that keeps them resident for the life of the process: concurrent calls from
+ * several threads lease separate buffer sets, and a buffer is reallocated only when a call
+ * needs more than it holds (DFM_ERR_NOMEM if that fails), so steady call sequences neither
+ * allocate nor free.                                                                    */
 
 /* gamma (nx x dv) = sum_j K(x_i, y_j) beta_j.
  * Replaces GaussGpuEvalConv1D<TYPE, DIMPOINT, DIMVECT> (GpuConv1D.h:45-47,

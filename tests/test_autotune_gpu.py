@@ -82,13 +82,15 @@ def _ops(n, m, seed):
     }
 
 
-@pytest.mark.parametrize("n,m", [(1000, 1000), (5000, 20000)])
+@pytest.mark.parametrize("n,m", [(1000, 1000), (5000, 20000), (9000, 12000)])
 def test_every_explored_plan_is_correct(tuner, oracle_mod, n, m):
+    """From 8,192 targets on, DFM_BF16MFMA's single Convolve / Grad1 / fused calls explore
+    the matrix-core kernel's plans; its other ops and smaller calls explore DFM_FP32C's."""
     data, ops = _ops(n, m, n)
     refs = _oracle_refs(oracle_mod, data)
     assert set(refs) == set(ops)
     tuner.autotune(1)
-    for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6)):
+    for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6), ("bf16mfma", 1e-6)):
         for k, f in ops.items():
             for _ in range(50):           # > twice the largest candidate count
                 got = f(prec)
@@ -100,6 +102,8 @@ def test_every_explored_plan_is_correct(tuner, oracle_mod, n, m):
     rep = tuner.autotune_report()
     assert "exploring" not in rep and "tuned" in rep
     assert "op=conv_grad1" in rep and "batch=1" in rep
+    if n >= 8192:
+        assert "prec=3" in rep
 
 
 def test_tuned_mode_is_deterministic(tuner):
