@@ -154,6 +154,7 @@ def main():
         from HIP events recorded on its stream."""
         stream.synchronize()
         L.dfm_profile_read(None, None)
+        L.dfm_profile_work(None, None)
         tot, cnt = ctypes.c_double(), ctypes.c_long()
         L.dfm_profile_enable(1)
         t0 = time.perf_counter()
@@ -163,6 +164,9 @@ def main():
         t1 = time.perf_counter()
         L.dfm_profile_enable(0)
         _lib.check(L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt)))
+        wp, wf = ctypes.c_double(), ctypes.c_double()
+        _lib.check(L.dfm_profile_work(ctypes.byref(wp), ctypes.byref(wf)))
+        _WORK.update(pairs=wp.value / steps, flops=wf.value / steps)
         return (t1 - t0) / steps, tot.value / 1e3 / max(cnt.value, 1)
 
     # ---- the headline precision, then the other FP32 forms and plain FP64
@@ -171,6 +175,7 @@ def main():
     for _ in range(args.warmup):
         step(head)
     s_per_step, kern_s = timed(head, args.steps)
+    head_work = dict(_WORK)
     res = {head: (s_per_step, kern_s, conv.numpy(stream), grad.numpy(stream))}
     print("%s: %.4f s/step, kernel %.4f s" % (head, s_per_step, kern_s), file=sys.stderr,
           flush=True)
@@ -281,6 +286,8 @@ def main():
             "kernel": KERNELS[head],
             "kernel_ms": kern_s * 1e3,
             "flops_per_pair": FLOPS_PER_PAIR,
+            "library_count": {"pairs_per_step": head_work["pairs"],
+                              "flops_per_step": head_work["flops"]},
         },
         "cpu_baseline": cpu,
         "fp64_plain_gpu": {"s_per_step": s64, "kernel_s": kern64, "pair_evals_per_s": pairs / s64,
@@ -332,11 +339,17 @@ def _nrel(ref, got):
     return float(np.linalg.norm(ref - got) / max(np.linalg.norm(ref), 1e-300))
 
 
+_WORK = {}  # algorithmic work per call of the last _profiled run: pairs, flops
+
+
 def _profiled(L, fn, reps):
-    """Wall seconds per call and pair-sum kernel seconds per call (HIP events)."""
+    """Wall seconds per call, pair-sum kernel seconds per call (HIP events) and launches per
+    call; the library's count of algorithmic pairs and flops per call goes to _WORK."""
     import ctypes
     L.dfm_profile_read(None, None)
+    L.dfm_profile_work(None, None)
     tot, cnt = ctypes.c_double(), ctypes.c_long()
+    pairs, flops = ctypes.c_double(), ctypes.c_double()
     L.dfm_profile_enable(1)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -344,7 +357,21 @@ def _profiled(L, fn, reps):
     t1 = time.perf_counter()
     L.dfm_profile_enable(0)
     L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt))
+    L.dfm_profile_work(ctypes.byref(pairs), ctypes.byref(flops))
+    _WORK.update(pairs=pairs.value / reps, flops=flops.value / reps)
     return (t1 - t0) / reps, tot.value / 1e3 / reps, cnt.value // reps
+
+
+def _roof(kernel_s, work=None):
+    """Roofline of a workload's pair sums: the library's algorithmic pairs and flops per call
+    (dfm_profile_work: sum over launches of pairs x dfm_flops_per_pair(op)) over the summed
+    pair-sum kernel time, against the FP32 vector peak."""
+    w = dict(work or _WORK)
+    return {"bound": "valu", "pairs_per_call": w["pairs"], "flops_per_call": w["flops"],
+            "flops_per_pair_mean": w["flops"] / max(w["pairs"], 1.0),
+            "kernel_ms": kernel_s * 1e3, "pair_evals_per_s": w["pairs"] / kernel_s,
+            "achieved": w["flops"] / kernel_s / 1e12, "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": w["flops"] / kernel_s / FP32_PEAK}
 
 
 def _tuned(L, fn, reps, explore=64):
@@ -388,9 +415,11 @@ def workload_shooting(args):
             df.SetStartMomentas(a0)
             df.Update()
             wall, kern, launches = _profiled(L, df.Update, 5)
+            roof = _roof(kern)
             twall, tkern, _ = _tuned(L, df.Update, 5)
             row[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3,
-                         "pairsum_launches": launches, "autotuned_ms": twall * 1e3,
+                         "pairsum_launches": launches, "roofline": roof,
+                         "autotuned_ms": twall * 1e3,
                          "autotuned_pairsum_kernel_ms": tkern * 1e3,
                          "rel_err_final_positions_vs_oracle":
                              _nrel(P[-1] - p0, df.GetTrajectoryPositions()[-1] - p0),
@@ -432,8 +461,10 @@ def workload_registration(args):
         run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
         run()
         wall, kern, launches = _profiled(L, run, 3 if prec == "fp32c" else 1)
+        roof = _roof(kern)
         tuned = _tuned(L, run, 3)[:2] if prec == "fp32c" else (None, None)
         out[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "pairsum_launches": launches,
+                     "roofline": roof,
                      "autotuned_ms": tuned[0] and tuned[0] * 1e3,
                      "grads": [A.grad_momenta.numpy(A.stream), A.grad_cp.numpy(A.stream),
                                A.grad_template.numpy(A.stream)],
@@ -467,7 +498,8 @@ def workload_registration(args):
                               "[-1.5,1.5]^3, deformation sigma 0.3, varifold sigma 0.1, 21 time "
                               "points, checkpoints every 5"},
                  dtype="f32", pairsum_kernel_ms=r["pairsum_kernel_ms"],
-                 pairsum_launches=r["pairsum_launches"], autotuned_ms=r["autotuned_ms"],
+                 pairsum_launches=r["pairsum_launches"], roofline=r["roofline"],
+                 fp64_roofline=out["fp64"]["roofline"], autotuned_ms=r["autotuned_ms"],
                  fp64_ms=out["fp64"]["ms"],
                  fp32c_gradient_rel_err_vs_fp64=errs,
                  fp64_gradient_vs_central_differences=fd_err)
@@ -496,6 +528,7 @@ def workload_atlas(args):
     run = lambda: (A.gradient_device(dcp, dmom, dV), A.stream.synchronize())
     run()
     wall, kern, launches = _profiled(L, run, 3)
+    roof = _roof(kern)
     # explore plans, then capture the gradient with the tuned plans
     L.dfm_autotune_reset()
     L.dfm_autotune(1)
@@ -503,6 +536,7 @@ def workload_atlas(args):
         run()
     L.dfm_autotune(2)
     twall, tkern, _ = _profiled(L, run, 3)
+    troof = _roof(tkern)
     g = A.capture(dcp, dmom, dV)
     g.launch()
     A.stream.synchronize()
@@ -538,8 +572,8 @@ def workload_atlas(args):
                               "deformation sigma 0.25, varifold sigma 0.1, 11 time points, "
                               "checkpoints every 5" % (S, min(faces), max(faces), int(np.mean(faces)))},
                  dtype="f32", eager_ms=wall * 1e3, pairsum_kernel_ms=kern * 1e3,
-                 pairsum_launches=launches, autotuned_eager_ms=twall * 1e3,
-                 autotuned_pairsum_kernel_ms=tkern * 1e3,
+                 pairsum_launches=launches, roofline=roof, autotuned_eager_ms=twall * 1e3,
+                 autotuned_pairsum_kernel_ms=tkern * 1e3, autotuned_roofline=troof,
                  graph="replay of the autotuned gradient", autotune_table=report.splitlines(),
                  device_bytes=A.device_bytes(),
                  loglik=[float(v) for v in A.loglik.numpy(A.stream)],
@@ -569,21 +603,17 @@ def workload_varifold(args):
         out = run()[0]
         wall, kern, launches = _profiled(L, run, reps)
         res[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "launches": launches,
+                     "roofline": _roof(kern),
                      "match": float(out[0].numpy(st)[0]), "grad_c": out[1].numpy(st),
                      "grad_n": out[2].numpy(st)}
-    pairs = 3.0 * n * n
-    flops = (2 * 41 + 22) * float(n) * n        # two fused Dv=6 sums + one Dv=6 Convolve
     r = res["fp32c"]
+    pairs = r["roofline"]["pairs_per_call"]
     return _line("varifold data term pair-evals/s (1M x 1M samples, match + gradient)",
                  pairs / (r["ms"] / 1e3), "pair-evals/s", True, r["ms"],
                  {"workload": "two sets of %d oriented samples of the unit sphere (normals radial "
                               "x log-normal(-9, 0.3) areas), varifold kernel, sigma 0.02: "
                               "|S|^2 + |T|^2 - 2<S,T> and d/d centres, d/d normals" % n},
-                 dtype="f32",
-                 roofline={"bound": "valu", "achieved": flops / (r["pairsum_kernel_ms"] / 1e3) / 1e12,
-                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                           "frac": flops / (r["pairsum_kernel_ms"] / 1e3) / FP32_PEAK,
-                           "flops_per_pair": "41 (fused Dv=6) x2 + 22 (Dv=6 Convolve)"},
+                 dtype="f32", roofline=r["roofline"], fp64_roofline=res["fp64"]["roofline"],
                  fp64_ms=res["fp64"]["ms"], speedup_fp32c_over_fp64=res["fp64"]["ms"] / r["ms"],
                  match_rel_err_vs_fp64=abs(r["match"] - res["fp64"]["match"]) / abs(res["fp64"]["match"]),
                  grad_centres_rel_err_vs_fp64=_nrel(res["fp64"]["grad_c"], r["grad_c"]),
@@ -622,23 +652,15 @@ def workload_curves(args):
         m, g = run()
         wall, kern, launches = _profiled(L, run, reps)
         res[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "launches": launches,
-                     "match": float(m.numpy(st)[0]), "grad": g.numpy(st)}
-    pairs = 5.0 * n * n                    # three Convolve sums (match), two fused (gradient)
-    # 2D, Dv = 3: shared part 7 (2 differences, 3 for r^2, 1 scale, 1 exp); Convolve 6;
-    # gradient 11 (alpha.w 5, coefficient 2, 2 FMA 4)
-    flops = (3 * 13 + 2 * 24) * float(n) * n
+                     "roofline": _roof(kern), "match": float(m.numpy(st)[0]), "grad": g.numpy(st)}
     r = res["fp32c"]
+    pairs = r["roofline"]["pairs_per_call"]
     return _line("polyline varifold pair-evals/s (2D, 1M x 1M segments, match + gradient)",
                  pairs / (r["ms"] / 1e3), "pair-evals/s", True, r["ms"],
                  {"workload": "two closed 2D polylines of %d segments each (radius 1 + 0.2 "
                               "sin 5t and a perturbed copy), NonOrientedPolyLine varifold, "
                               "sigma 0.02: match and d match / d points" % n},
-                 dtype="f32",
-                 roofline={"bound": "valu",
-                           "achieved": flops / (r["pairsum_kernel_ms"] / 1e3) / 1e12,
-                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                           "frac": flops / (r["pairsum_kernel_ms"] / 1e3) / FP32_PEAK,
-                           "flops_per_pair": "13 (2D Dv=3 Convolve) x3 + 24 (2D fused) x2"},
+                 dtype="f32", roofline=r["roofline"],
                  fp64_ms=res["fp64"]["ms"], speedup_fp32c_over_fp64=res["fp64"]["ms"] / r["ms"],
                  match_rel_err_vs_fp64=abs(r["match"] - res["fp64"]["match"]) / abs(res["fp64"]["match"]),
                  grad_rel_err_vs_fp64=_nrel(res["fp64"]["grad"], r["grad"]),

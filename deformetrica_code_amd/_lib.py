@@ -172,6 +172,10 @@ def load():
     L.dfm_profile_enable.restype = i
     L.dfm_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
     L.dfm_profile_read.restype = i
+    L.dfm_profile_work.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.dfm_profile_work.restype = i
+    L.dfm_flops_per_pair.argtypes = [i, i, i]
+    L.dfm_flops_per_pair.restype = ctypes.c_double
     # device entry points
     L.dfm_conv.argtypes = [_P, i, i, d, i, _P, l, _P, _P, l, _P, _P, sz]
     L.dfm_grad1.argtypes = [_P, i, i, d, i, _P, _P, l, _P, _P, l, _P, _P, sz]

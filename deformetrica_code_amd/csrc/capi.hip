@@ -207,6 +207,15 @@ int dfm_profile_read(double *total_ms, long *launches) {
   return profile_read(total_ms, launches);
 }
 
+int dfm_profile_work(double *pairs, double *flops) {
+  clear_error();
+  return profile_work(pairs, flops);
+}
+
+double dfm_flops_per_pair(int op, int dimpoint, int dimvect) {
+  return flops_per_pair(op, dimpoint, dimvect);
+}
+
 size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision) {
   if (check_common(dimpoint, dimvect, precision, 1.0, nx, ny) != DFM_OK) return 0;
   if (op < DFM_OP_CONV || op > DFM_OP_HESSIAN) return 0;

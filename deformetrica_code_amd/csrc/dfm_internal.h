@@ -49,6 +49,9 @@ constexpr int kBfTile = 256;
 // cross term's rounding, would grow)
 constexpr double kBfExtent = 6.0;
 
+// |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
+constexpr double kZeroMomenta2 = 1e-40;
+
 // FP32 paths: a wavefront whose targets span more than this many scaled kernel units
 // (sqrt(log2 e) / sigma per unit length) forms exact (hi + lo) coordinate differences.
 constexpr double kExactExtent = 4.0;
@@ -89,6 +92,17 @@ struct KParams {
   double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
   double exact_extent_bf;  // DFM_BF16MFMA: group extent above which k_pairsum runs the group
   int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores)
+  // Fused integrator step (OP_CONVGRAD1 with x = positions, ta = w = momenta):
+  //   e_p1 = e_pb [+ e_cx e_px] + e_c conv,   e_a1 = e_ab [- e_cx e_ax] - e_c grad,
+  // gated per segment by e_gate (squared norm of the segment's initial momenta below 1e-40:
+  // e_p1 = e_pb, e_a1 = 0).  k_pairsum applies it to each target once its sums are complete:
+  // with source splits the last split of each target tile to finish (counters e_done, zero
+  // between launches) adds up the partials; other kernels leave it to k_step_update.
+  int e_on;
+  double *e_p1, *e_a1;
+  const double *e_pb, *e_ab, *e_px, *e_ax, *e_gate;
+  double e_c, e_cx;
+  int *e_done;
 };
 
 // Work decomposition and workspace layout of one call.
@@ -116,8 +130,6 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny);
 // max_ny sources in one segment.  No Morton sort (batched calls are capture-safe).
 Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
                      long max_ny);
-int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p, const Plan &pl,
-                   char *work);
 int fill(hipStream_t st, double *dst, long n, double v);
 
 // One pair-sum call on device pointers: plans the work, checks the workspace, clamps the
@@ -206,5 +218,8 @@ void tune_reset();
 int tune_report(char *buf, size_t len);
 void profile_enable(bool on);
 int profile_read(double *total_ms, long *launches);
+int profile_work(double *pairs, double *flops);
+// algorithmic flops of one (target, source) pair of an op (pairsum.hip)
+double flops_per_pair(int op, int D, int DV);
 
 }  // namespace dfm

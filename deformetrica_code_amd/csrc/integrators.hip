@@ -48,8 +48,6 @@
 namespace dfm {
 namespace {
 
-// |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
-constexpr double kZeroMomenta2 = 1e-40;
 
 // out = ca a + cb b + cc c (b, c may be null) over segment blockIdx.y of off (in points of
 // D coordinates; the flat range [0, n) without offsets).  With a gate (squared norm of the
@@ -235,29 +233,62 @@ size_t pair_ws(int prec, int D, const Seg &cs, const Seg &ls, bool batch) {
 
 // ---- forward steps ------------------------------------------------------------------
 
+// Upper bound of the target tiles (workgroups along the targets times segments) of a pair
+// sum over the control points: the fused step keeps one completion counter per tile.
+long step_tiles(const Seg &cs) {
+  return cs.n > 1 ? (cs.max / 64 + 1) * cs.n : cs.total / 64 + 1;
+}
+
 struct ShootTmp {
   double *dpos, *dmom, *ps, *as, *dpos2, *dmom2;
-  void take(Arena &ar, long nd) {
+  int *done;    // fused-step tile counters, zero between launches
+  long ndone;
+  void take(Arena &ar, long nd, long tiles) {
     dpos = ar.take(nd); dmom = ar.take(nd); ps = ar.take(nd); as = ar.take(nd);
     dpos2 = ar.take(nd); dmom2 = ar.take(nd);
+    ndone = tiles;
+    done = (int *)ar.take((size_t)(tiles + 1) / 2);
   }
 };
 
-// (p, a) -> (p1, a1) by one Euler or Heun step.
+// zero the fused-step counters (a fill kernel: capture-safe)
+int clear_done(const Ctx &cx, const ShootTmp &t) {
+  fill(cx.st, (double *)t.done, (t.ndone + 1) / 2, 0.0);
+  return check_launch("clear counters");
+}
+
+// One pair sum over the control points (p, a) with the step applied in the same launch
+// (KParams e_*): p1 = pb [+ cx px] + c conv, a1 = ab [- cx ax] - c grad, gated by `gate`;
+// conv and grad themselves go to dpos and dmom.
+int fused_step_sum(const Ctx &cx, const double *p, const double *a, const Seg &cs, double *dpos,
+                   double *dmom, double *p1, double *a1, const double *pb, const double *ab,
+                   double c, const double *px, const double *ax, double cxx, const double *gate,
+                   int *done) {
+  KParams q = blank();
+  q.x = p; q.ta = a; q.y = p; q.w = a;
+  q.out0 = dpos; q.out1 = dmom;
+  q.e_on = 1;
+  q.e_p1 = p1; q.e_a1 = a1; q.e_pb = pb; q.e_ab = ab;
+  q.e_px = px; q.e_ax = ax; q.e_c = c; q.e_cx = cxx;
+  q.e_gate = gate;
+  q.e_done = done;
+  const long nd = cs.total * cx.D;
+  return ps(cx, OP_CONVGRAD1, q, cs, cs, nd, nd);
+}
+
+// (p, a) -> (p1, a1) by one Euler or Heun step: one launch per pair sum, the split
+// reduction and the update applied by its last split.
 int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, const Seg &cs,
                double dt, double *p1, double *a1, const ShootTmp &t, const double *gate) {
-  const long nd = cs.total * cx.D;
-  int rc = conv_grad1(cx, p, a, cs, p, a, cs, t.dpos, t.dmom);
+  if (scheme == DFM_EULER)
+    return fused_step_sum(cx, p, a, cs, t.dpos, t.dmom, p1, a1, p, a, dt, nullptr, nullptr, 0.0,
+                          gate, t.done);
+  // Heun: predictor (ungated), then the corrector from the original state
+  int rc = fused_step_sum(cx, p, a, cs, t.dpos, t.dmom, t.ps, t.as, p, a, dt, nullptr, nullptr,
+                          0.0, nullptr, t.done);
   if (rc) return rc;
-  if (scheme == DFM_EULER) {
-    if ((rc = glin(cx, cs, p1, p, 1.0, t.dpos, dt, nullptr, 0.0, gate, 0))) return rc;
-    return glin(cx, cs, a1, a, 1.0, t.dmom, -dt, nullptr, 0.0, gate, 1);
-  }
-  if ((rc = lin(cx, t.ps, p, 1.0, t.dpos, dt, nullptr, 0.0, nd))) return rc;
-  if ((rc = lin(cx, t.as, a, 1.0, t.dmom, -dt, nullptr, 0.0, nd))) return rc;
-  if ((rc = conv_grad1(cx, t.ps, t.as, cs, t.ps, t.as, cs, t.dpos2, t.dmom2))) return rc;
-  if ((rc = glin(cx, cs, p1, p, 1.0, t.dpos, 0.5 * dt, t.dpos2, 0.5 * dt, gate, 0))) return rc;
-  return glin(cx, cs, a1, a, 1.0, t.dmom, -0.5 * dt, t.dmom2, -0.5 * dt, gate, 1);
+  return fused_step_sum(cx, t.ps, t.as, cs, t.dpos2, t.dmom2, p1, a1, p, a, 0.5 * dt, t.dpos,
+                        t.dmom, 0.5 * dt, gate, t.done);
 }
 
 struct FlowTmp {
@@ -311,7 +342,7 @@ struct ShootLayout {
   ShootLayout(void *base, int prec, int D, const Seg &cs, bool batch) {
     Arena ar(base);
     sa.take(ar, cs.n);
-    t.take(ar, cs.total * D);
+    t.take(ar, cs.total * D, step_tiles(cs));
     pw_bytes = pair_ws(prec, D, cs, Seg::single(0), batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
@@ -356,7 +387,7 @@ struct AdjointLayout {
     sa.take(ar, cs.n);
     ckpt = ar.take((size_t)(segs + 1) * rec);
     seg = ar.take((size_t)(std::min(k, std::max(T - 1, 0)) + 1) * rec);
-    st.take(ar, n * D);
+    st.take(ar, n * D, step_tiles(cs));
     ft.take(ar, m * D);
     double **vn[] = {&lp, &la, &lpn, &lan, &lpf, &laf, &jp, &ja, &jsp, &jsa, &nup, &nua};
     for (double **v : vn) *v = ar.take(n * D);
@@ -424,7 +455,7 @@ int shoot_impl(void *stream, int D, double sigma, int prec, int scheme, int nseg
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   ShootLayout L(work, prec, D, cs, batch);
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
-  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
+  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.t))) return rc;
   const long nd = cs.total * D;
   if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, cs, L.sa.gate)))
     return rc;
@@ -589,7 +620,7 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   AdjointLayout L(work, prec, D, cs, ls, T, checkpoint_every, batch);
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
-  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
+  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.st))) return rc;
   const long nd = n * D, md = m * D, rec = 2 * nd + md;
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   double *gate = L.sa.gate;

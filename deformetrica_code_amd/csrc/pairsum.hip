@@ -21,6 +21,7 @@
 
 #include "dfm_hip.h"
 #include "dfm_internal.h"
+#include "pairsum_kernels.h"
 
 namespace dfm {
 
@@ -48,10 +49,44 @@ void clear_error() { g_last_error.clear(); }
 struct ProfileState {
   bool on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double pairs = 0.0, flops = 0.0;  // algorithmic work of the timed launches
 };
 static thread_local ProfileState g_profile;
 
 void profile_enable(bool on) { g_profile.on = on; }
+
+int profile_work(double *pairs, double *flops) {
+  if (pairs) *pairs = g_profile.pairs;
+  if (flops) *flops = g_profile.flops;
+  g_profile.pairs = g_profile.flops = 0.0;
+  return DFM_OK;
+}
+
+// Algorithmic flops of one (target, source) pair of an op, counted on the reference's
+// per-pair arithmetic (ExactKernel.cxx, AbstractKernel.cxx; exp counted as one): the
+// kernel value 3D + 1 (D differences, D squares and D - 1 adds, the scale, the exp), then
+//   Convolve        2 Dv (multiply-add per weight)
+//   Grad1           2 Dv - 1 (alpha . w) + 2 (coefficient) + 2 D
+//   varlin          1 + D (coefficient times difference) + 2 Dv D
+//   gradient(dim)   1 + 2 Dv
+//   Hessian         D(D+1)/2 (d d^T) + Dv (2 + D(D+1))
+//   SpecialHessian  2 Dv - 1 (beta . beta) + D + 2D - 1 (eta difference and dot) + 2 + 4 D
+//   shooting VJP    4D - 1 + 2D - 1 + D + 2D - 1 + 4 + 10 D  (see pair_update)
+double flops_per_pair(int op, int D, int DV) {
+  const double k = 3.0 * D + 1.0;
+  const double conv = 2.0 * DV, grad = 2.0 * DV - 1.0 + 2.0 + 2.0 * D;
+  switch (op) {
+    case OP_CONV: return k + conv;
+    case OP_GRAD1: return k + grad;
+    case OP_CONVGRAD1: return k + conv + grad;
+    case OP_VARLIN: return k + 1.0 + D + 2.0 * DV * D;
+    case OP_GRADDIM: return k + 1.0 + 2.0 * DV;
+    case OP_HESS: return k + D * (D + 1) / 2.0 + DV * (2.0 + D * (D + 1));
+    case OP_GRADDIFF: return k + 2.0 * DV - 1.0 + 3.0 * D - 1.0 + 2.0 + 4.0 * D;
+    case OP_SHOOTVJP: return k + 19.0 * D + 1.0;
+  }
+  return k;
+}
 
 int profile_read(double *total_ms, long *launches) {
   double tot = 0.0;
@@ -164,6 +199,27 @@ __global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, long
   const long stride = (long)gridDim.x * blockDim.x;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+// The fused integrator step (KParams e_*) applied from the stored sums out0 (conv) and
+// out1 (grad), for the kernels that do not apply it themselves; segment blockIdx.y.
+template <int D>
+__global__ __launch_bounds__(256) void k_step_update(KParams p) {
+  const int seg = blockIdx.y;
+  long lo = 0, hi = p.nx;
+  if (p.xoff) {
+    lo = p.xoff[seg];
+    hi = p.xoff[seg + 1];
+  }
+  const long o = lo + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= hi) return;
+  double conv[D], grad[D];
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    conv[k] = p.out0[o * D + k];
+    grad[k] = p.out1[o * D + k];
+  }
+  step_update<D>(p, o, seg, conv, grad);
 }
 
 struct LongChunk {
@@ -614,16 +670,10 @@ int seg_from_host(Seg &s, int n, const long *h, const long *d) {
   return DFM_OK;
 }
 
+// `pairs`: the call's (target, source) pairs, added to the profile's work with the timed
+// launch.
 static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
-                        const Plan &pl, char *work, TimedLaunch *timed);
-
-int launch_pairsum(hipStream_t st, int op, int prec, int D, int DV, KParams p,
-                   const Plan &pl, char *work) {
-  return launch_timed(st, op, prec, D, DV, p, pl, work, nullptr);
-}
-
-static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams p,
-                        const Plan &pl, char *work, TimedLaunch *timed) {
+                        const Plan &pl, char *work, TimedLaunch *timed, double pairs) {
   const double sigma = p.sigma;
   // FP64: K = exp(-|x' - y'|^2) with x' = x / sigma; FP32: K = exp2(-|x' - y'|^2) with
   // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
@@ -655,15 +705,34 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
   }
+  // a fused integrator step runs inside k_pairsum; the matrix-core kernels leave it to
+  // k_step_update after them
+  const bool step_after = p.e_on && (prec == DFM_BF16MFMA ||
+                                     (prec == DFM_FP32C && pl.mx && mx_supported(op, D, DV) &&
+                                      pl.R == 1));
+  if (step_after) p.e_on = 0;
   if (ev0) (void)hipEventRecord(ev0, st);
   int rc = prec == DFM_FP64       ? launch_op_f64(st, op, D, DV, p, pl)
            : prec == DFM_FP32X    ? launch_op_f32x(st, op, D, DV, p, pl)
            : prec == DFM_BF16MFMA ? launch_op_bf16(st, op, D, DV, p, pl)
                                   : launch_op_f32c(st, op, D, DV, p, pl);
+  if (rc == DFM_OK && step_after) {
+    const long rows = p.xoff ? pl.chunks * 64L * pl.R : p.nx;  // the largest segment, or all
+    const dim3 g((unsigned)((rows + 255) / 256), (unsigned)pl.nseg);
+    if (D == 2)
+      hipLaunchKernelGGL(k_step_update<2>, g, dim3(256), 0, st, p);
+    else
+      hipLaunchKernelGGL(k_step_update<3>, g, dim3(256), 0, st, p);
+  }
   hipError_t e = rc == DFM_OK ? hipGetLastError() : hipSuccess;
   if (ev1) (void)hipEventRecord(ev1, st);
-  if (timed) timed->state = (rc == DFM_OK && e == hipSuccess) ? 1 : 2;
-  else if (ev0) g_profile.events.push_back({ev0, ev1});
+  if (timed) {
+    timed->state = (rc == DFM_OK && e == hipSuccess) ? 1 : 2;
+  } else if (ev0) {
+    g_profile.events.push_back({ev0, ev1});
+    g_profile.pairs += pairs;
+    g_profile.flops += pairs * flops_per_pair(op, D, DV);
+  }
   if (rc != DFM_OK) return rc;
   if (e != hipSuccess) return set_error(DFM_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
   return DFM_OK;
@@ -708,7 +777,9 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
                      work ? work_bytes : (size_t)0);
   }
   p.sigma = clamp_width(sigma);
-  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get());
+  double pairs = 0.0;
+  for (int i = 0; i < xs.n; i++) pairs += (double)xs.count(i) * (double)ys.count(i);
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get(), pairs);
 }
 
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
@@ -730,7 +801,8 @@ int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KPara
                      pl.work_bytes, work ? work_bytes : (size_t)0);
   }
   p.sigma = clamp_width(sigma);
-  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get());
+  return launch_timed(st, op, prec, D, DV, p, pl, (char *)work, timed.get(),
+                      (double)p.nx * (double)p.ny);
 }
 
 }  // namespace dfm

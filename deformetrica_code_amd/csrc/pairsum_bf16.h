@@ -184,8 +184,9 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     const long wjs = (long)split * p.split_len;
     const long wje = std::min(p.ny, wjs + p.split_len);
     float *wsm = reinterpret_cast<float *>(smA) + wave * (kTileF32 * NREC);
+    // (a fused integrator step never runs here: it is applied by k_step_update)
     sum_wavefront<float, OP, D, DV, 0, 1, MODE_ADAPTIVE>(p, wsm, lane, split, wjs, wje, wpos, worig,
-                                                        wvalid, wxd, wc, wext);
+                                                        wvalid, wxd, wc, wext, -1, 0);
     return;
   }
 

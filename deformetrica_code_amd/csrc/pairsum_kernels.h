@@ -299,6 +299,34 @@ __device__ __forceinline__ void expand_finish(double (&a)[NA], const float *xt, 
   }
 }
 
+// The fused integrator step for target `o` of segment `seg` from its finished sums (KParams
+// e_*): the same arithmetic, in the same order, as the separate update (k_lin) it replaces.
+template <int D>
+__device__ __forceinline__ void step_update(const KParams &p, long o, int seg, const double *conv,
+                                            const double *grad) {
+  const bool frozen = p.e_gate && p.e_gate[seg] < kZeroMomenta2;
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    const long i = o * D + k;
+    double vp, va;
+    if (frozen) {
+      vp = p.e_pb[i];
+      va = 0.0;
+    } else {
+      vp = 1.0 * p.e_pb[i];
+      va = 1.0 * p.e_ab[i];
+      if (p.e_px) {
+        vp += p.e_cx * p.e_px[i];
+        va += -p.e_cx * p.e_ax[i];
+      }
+      vp += p.e_c * conv[k];
+      va += -p.e_c * grad[k];
+    }
+    p.e_p1[i] = vp;
+    p.e_a1[i] = va;
+  }
+}
+
 // Scale the FP64 accumulators of one target and store them at original index `o`.
 template <int OP, int D, int DV>
 __device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
@@ -514,12 +542,16 @@ __device__ __forceinline__ void load_targets(const KParams &p, int lane, long ba
 // wavefront takes part (invalid lanes sit at the centre and store nothing), and the number
 // of __syncthreads() depends on js, je only, so the wavefronts of a larger workgroup that
 // call this with the same source range pass the same barriers.
+//
+// With a fused integrator step (p.e_on, OP_CONVGRAD1) each target's sums are finished and
+// the step applied here: directly with one split, else by the wavefront of target tile
+// `tile` (segment `seg`) that is the last of the p.nsplit splits to store its partials.
 template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
 __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane, int split,
                                               long js, long je, const long (&pos)[R],
                                               const long (&orig)[R], const bool (&valid)[R],
                                               const double (&xd)[R][D], const double (&c)[D],
-                                              double extent) {
+                                              double extent, long tile, int seg) {
   using S = Shape<OP, D, DV>;
   constexpr int NACC = S::NACC;
   constexpr int NA = S::NACCMAX;
@@ -593,6 +625,41 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
       for (int a = 0; a < NACC; a++) dst[a] = accd[r][a];
     }
   }
+  if constexpr (OP == OP_CONVGRAD1 && DV == D) {
+    if (p.e_on) {
+      if (p.nsplit > 1) {
+        // publish this split's partials, count it, and let the tile's last split finish
+        __threadfence();
+        int last = 0;
+        if (lane == 0) last = atomicAdd(&p.e_done[tile], 1) == p.nsplit - 1;
+        last = __shfl(last, 0);
+        if (!last) return;
+        __threadfence();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (!valid[r]) continue;
+          // the split order of k_reduce_splits
+#pragma unroll
+          for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
+          for (int sp = 0; sp < p.nsplit; sp++) {
+            const double *src = p.partial + ((long)sp * p.nx + pos[r]) * NACC;
+#pragma unroll
+            for (int a = 0; a < NACC; a++) accd[r][a] += src[a];
+          }
+          finalize<OP, D, DV>(accd[r], p, orig[r]);
+        }
+        if (lane == 0) p.e_done[tile] = 0;  // ready for the next launch
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (!valid[r]) continue;
+        double grad[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) grad[l] = p.cg * accd[r][DV + l];
+        step_update<D>(p, orig[r], seg, accd[r], grad);
+      }
+    }
+  }
 }
 
 template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
@@ -631,7 +698,8 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
   const long js = yb + (long)split * p.split_len;
   const long je = std::min(ye, js + p.split_len);
   sum_wavefront<T, OP, D, DV, AUX, R, MODE>(p, sm, lane, split, js, je, pos, orig, valid, xd, c,
-                                            extent);
+                                            extent, (long)blockIdx.z * gridDim.x + blockIdx.x,
+                                            (int)blockIdx.z);
 }
 
 template <int OP, int D, int DV>
@@ -662,7 +730,9 @@ static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
     case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE>), grid, dim3(64), 0, st, p); break;
     default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4, MODE>), grid, dim3(64), 0, st, p); break;
   }
-  if (pl.nsplit > 1)
+  // a fused integrator step finishes the splits in k_pairsum itself
+  constexpr bool kStep = OP == OP_CONVGRAD1 && DV == D;
+  if (pl.nsplit > 1 && !(kStep && p.e_on))
     hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
                        dim3(256), 0, st, p);
 }

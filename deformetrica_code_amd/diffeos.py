@@ -38,6 +38,15 @@ class HIPDiffeos:
         self._P = self._A = self._X = None
         self._stream = Stream()
         self._ws = Workspace()
+        self._bufs = {}
+
+    def _buf(self, name, shape):
+        """A device array kept between calls (reallocated only when the shape changes), so
+        repeated Update calls allocate nothing."""
+        b = self._bufs.get(name)
+        if b is None or b.shape != tuple(shape):
+            b = self._bufs[name] = DeviceArray(shape)
+        return b
 
     # --- parameters (Diffeos.h setters)
     def SetKernelWidth(self, h):
@@ -78,18 +87,21 @@ class HIPDiffeos:
         if self.a0.shape != self.p0.shape:
             raise RuntimeError("start positions and momenta differ in shape")
         st = self._stream
-        p0 = DeviceArray.from_numpy(self.p0, st)
-        a0 = DeviceArray.from_numpy(self.a0, st)
-        self._P = DeviceArray((T, n, d))
-        self._A = DeviceArray((T, n, d))
+        p0 = self._buf("p0", self.p0.shape)
+        a0 = self._buf("a0", self.a0.shape)
+        p0.upload(self.p0, st)
+        a0.upload(self.a0, st)
+        self._P = self._buf("P", (T, n, d))
+        self._A = self._buf("A", (T, n, d))
         wp, wb = self._ws.get(L.dfm_shoot_workspace(d, n, self.precision))
         _lib.check(L.dfm_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme, p0.ptr,
                                a0.ptr, n, T, self.t0, self.tN, self._P.ptr, self._A.ptr, wp, wb))
         self._X = None
         if self.x0 is not None:
             m = self.x0.shape[0]
-            x0 = DeviceArray.from_numpy(self.x0, st)
-            self._X = DeviceArray((T, m, d))
+            x0 = self._buf("x0", self.x0.shape)
+            x0.upload(self.x0, st)
+            self._X = self._buf("X", (T, m, d))
             wp, wb = self._ws.get(L.dfm_flow_workspace(d, n, m, self.precision))
             _lib.check(L.dfm_flow(st.handle, d, self.kernel_width, self.precision,
                                   int(self.improved_euler), self._P.ptr, self._A.ptr, n, x0.ptr, m,

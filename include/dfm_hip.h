@@ -28,8 +28,9 @@
  *              matrix cores (v_mfma_f32_16x16x32_bf16 over three-part BF16 splits of the
  *              FP32 centred coordinates, FP32 accumulation), the target norm applied
  *              exactly per target, exp2 and accumulation as DFM_FP32C.  Groups of 256
- *              sorted targets spanning more than 4 scaled kernel widths, every other
- *              operation, ragged batches and smaller calls run as DFM_FP32C.
+ *              sorted targets spanning more than 6 scaled kernel units take the DFM_FP32C
+ *              wavefront code in the same launch; every other operation, ragged batches
+ *              and smaller calls run as DFM_FP32C.
  *
  * Supported shapes: dimpoint in {2, 3}; dimvect in {1, ..., 6} (the reference supports
  * dimvect in {D, 2D, D(D+1)/2}, CUDAExactKernel.cxx:56-82).  Counts are non-negative and
@@ -92,6 +93,15 @@ int dfm_stream_synchronize(void *stream);
  * duration and launch count, and forgets them. */
 int dfm_profile_enable(int on);
 int dfm_profile_read(double *total_ms, long *launches);
+/* The algorithmic work of the launches timed since the last call (while profiling is
+ * enabled on this thread): (target, source) pairs summed over every launch and segment,
+ * and those pairs times dfm_flops_per_pair of each launch's op; then resets both. */
+int dfm_profile_work(double *pairs, double *flops);
+/* Algorithmic flops of one (target, source) pair of an op (DFM_OP_*), counted on the
+ * reference's per-pair arithmetic with exp as one: 3 dimpoint + 1 for the kernel value,
+ * then 2 dimvect (Convolve), 2 dimvect + 2 dimpoint + 1 (ConvolveGradient(X, alpha)),
+ * both for the fused sum (29 at dimpoint = dimvect = 3). */
+double dfm_flops_per_pair(int op, int dimpoint, int dimvect);
 
 /* Autotuner of the pair-sum work split (no reference counterpart: the reference launches
  * one thread per target in blocks of 192, GpuConv1D.cu:98-132).  Per (op, precision,
