@@ -219,6 +219,8 @@ int tune_report(char *buf, size_t len);
 void profile_enable(bool on);
 int profile_read(double *total_ms, long *launches);
 int profile_work(double *pairs, double *flops);
+hipEvent_t profile_begin(hipStream_t st);
+void profile_end(hipStream_t st, hipEvent_t e0, double pairs, double flops);
 // algorithmic flops of one (target, source) pair of an op (pairsum.hip)
 double flops_per_pair(int op, int D, int DV);
 

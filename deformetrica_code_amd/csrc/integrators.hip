@@ -36,6 +36,13 @@
 // zero-momenta exit is decided per subject.  Batched calls launch kernels only (no
 // allocation, memset, host copy or synchronisation), so a caller may capture them in a
 // HIP graph and replay it.
+//
+// Small single-subject shoots (at most kPersistMaxPoints control points, a stream that is
+// not being captured) run as one cooperative launch for the whole trajectory
+// (k_shoot_persistent): each workgroup owns a few control points, each of its lanes sums
+// a strided slice of all the sources, a wavefront reduction finishes each point's field,
+// the owning lane applies the step, and a grid barrier separates the steps.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -233,10 +240,12 @@ size_t pair_ws(int prec, int D, const Seg &cs, const Seg &ls, bool batch) {
 
 // ---- forward steps ------------------------------------------------------------------
 
-// Upper bound of the target tiles (workgroups along the targets times segments) of a pair
-// sum over the control points: the fused step keeps one completion counter per tile.
+// Counters of the shoot's workspace: one completion counter per target tile (workgroups
+// along the targets times segments) of a fused step's pair sum, and at least the grid
+// barrier's counters of the one-launch shoot.
 long step_tiles(const Seg &cs) {
-  return cs.n > 1 ? (cs.max / 64 + 1) * cs.n : cs.total / 64 + 1;
+  const long tiles = cs.n > 1 ? (cs.max / 64 + 1) * cs.n : cs.total / 64 + 1;
+  return std::max<long>(tiles, 32 * 17);
 }
 
 struct ShootTmp {
@@ -436,6 +445,261 @@ int upload_segs(const Ctx &cx, Seg &cs, Seg &ls, const SegArrays &sa) {
   return DFM_OK;
 }
 
+// ---- one launch for a whole small shoot ------------------------------------------------
+
+// Config-1 shoot (1,000 points, 10 Euler steps, FP32C): 0.165 ms in one launch against
+// 0.214 ms in ten fused-step launches; 2,048 points: 0.264 against 0.467 ms; 4,096 points:
+// 0.487 against 0.579 ms (kernel times, same box).  Of the 16.5 us per step at 1,000
+// points, the grid barrier takes about 7.
+constexpr long kPersistMaxPoints = 4096;
+// lanes per point up to 1,024 points, then 16 (about 256 workgroups either way)
+constexpr int kPersistSpt = 32;
+// 0: cooperative-groups grid sync; 1: the counter barrier below
+constexpr int kPersistBarrier = 1;
+
+struct PersistShoot {
+  unsigned *count;              // barrier counter, zero at launch
+  const double *p0, *a0;
+  double *P, *A;                // trajectories, T x n x D
+  double *ps, *as, *dp, *da;    // Heun: predicted state and the first field, n x D
+  long n;
+  int T, scheme;
+  int spt;                      // lanes (source slices) per control point: 16, 32 or 64
+  double dt;
+  double inv_s2;                // 1 / sigma^2
+  double cg;                    // -2 / sigma^2
+};
+
+// Sources staged through LDS per chunk of this many points (positions and momenta, FP64)
+constexpr int kPersistChunk = 1024;
+
+// Field F = (sum_j K_ij a_j, -2/sigma^2 sum_j K_ij (a_i.a_j)(x_i - x_j)) of point i over the
+// sources j = slice, slice + spt, ... of state (p, m), staged through LDS (`sp`, `sm`) one
+// chunk at a time by the whole workgroup; then the sum over the spt lanes that share point i
+// (lane bits below spt, xor butterfly: the same order in every lane).
+// F32: the kernel value exp2 in FP32 of the FP64 squared distance (the DFM_FP32C kernel
+// form; FP64 differences and accumulation cost nothing at this size).
+template <bool F32, int D>
+__device__ __forceinline__ void persist_field(const PersistShoot &a, const double *p,
+                                              const double *m, long i, int slice,
+                                              double (&conv)[D], double (&grad)[D],
+                                              double *sp, double *sm) {
+  double xi[D], ai[D];
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    conv[k] = 0.0;
+    grad[k] = 0.0;
+    xi[k] = i < a.n ? p[i * D + k] : 0.0;
+    ai[k] = i < a.n ? m[i * D + k] : 0.0;
+  }
+  for (long c0 = 0; c0 < a.n; c0 += kPersistChunk) {
+    const int cn = (int)std::min<long>(kPersistChunk, a.n - c0);
+    __syncthreads();  // the previous chunk is consumed
+    for (int q = threadIdx.x; q < cn * D; q += 256) {
+      sp[q] = p[c0 * D + q];
+      sm[q] = m[c0 * D + q];
+    }
+    __syncthreads();
+    if (i >= a.n) continue;
+#pragma unroll 4
+    for (int j = slice; j < cn; j += a.spt) {
+      double d[D], aj[D], r2 = 0.0, aa = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        d[k] = xi[k] - sp[j * D + k];
+        aj[k] = sm[j * D + k];
+        r2 += d[k] * d[k];
+        aa += ai[k] * aj[k];
+      }
+      double K;
+      if constexpr (F32)
+        K = (double)__builtin_amdgcn_exp2f((float)(-r2 * a.inv_s2 * 1.4426950408889634074));
+      else
+        K = exp(-r2 * a.inv_s2);
+      const double t = K * aa;
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        conv[k] += K * aj[k];
+        grad[k] += t * d[k];
+      }
+    }
+  }
+  for (int off = a.spt / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      conv[k] += __shfl_xor(conv[k], off);
+      grad[k] += __shfl_xor(grad[k], off);
+    }
+#pragma unroll
+  for (int k = 0; k < D; k++) grad[k] *= a.cg;
+}
+
+// Grid barrier number `k` (1, 2, ...) of a cooperative launch (every workgroup resident),
+// as a two-level tree of counters, each on its own 128-byte line (count[32 (1 + l)] for
+// leaf l, count[0] the root): a workgroup arrives at leaf blockIdx.x % kBarrierLeaves, the
+// last arrival of a leaf arrives at the root, and everyone waits for the root.  Arrivals on
+// one address are serialised by the memory, so the tree splits them over the leaves.  The
+// counters only grow (zero at launch); the wait is bounded (about a second), so the launch
+// always ends.
+constexpr int kBarrierLeaves = 16;
+constexpr int kBarrierInts = 32 * (1 + kBarrierLeaves);
+__device__ __forceinline__ void counter_barrier(unsigned *count, unsigned k) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned g = gridDim.x;
+    const unsigned leaf = blockIdx.x % kBarrierLeaves;
+    const unsigned leaves = g < (unsigned)kBarrierLeaves ? g : (unsigned)kBarrierLeaves;
+    const unsigned leaf_size = (g - leaf + kBarrierLeaves - 1) / kBarrierLeaves;
+    __threadfence();  // this workgroup's writes before its arrival
+    if (atomicAdd(&count[32 * (1 + leaf)], 1u) + 1 == k * leaf_size) atomicAdd(&count[0], 1u);
+    const unsigned target = k * leaves;
+    for (unsigned spins = 0; spins < (1u << 24); spins++) {
+      if (__hip_atomic_load(&count[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
+        break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();  // the other workgroups' writes after everyone's arrival
+  }
+  __syncthreads();
+}
+
+template <bool F32, int D>
+__global__ __launch_bounds__(256) void k_shoot_persistent(PersistShoot a) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  unsigned barriers = 0;
+  auto sync = [&]() {
+    if constexpr (kPersistBarrier == 1)
+      counter_barrier(a.count, ++barriers);
+    else
+      grid.sync();
+  };
+  __shared__ double sred[256];
+  __shared__ double sp[kPersistChunk * D], sm[kPersistChunk * D];
+  const int tid = threadIdx.x;
+  const long i = (long)blockIdx.x * (256 / a.spt) + tid / a.spt;  // this lane's point
+  const int slice = tid % a.spt;
+  const bool owner = slice == 0 && i < a.n;
+  const long nd = a.n * D;
+
+  // |a0|^2 (every workgroup sums it: no barrier needed before the first step)
+  double s = 0.0;
+  for (long k = tid; k < nd; k += 256) s += a.a0[k] * a.a0[k];
+  sred[tid] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) sred[tid] += sred[tid + st];
+    __syncthreads();
+  }
+  const bool frozen = sred[0] < kZeroMomenta2;  // Diffeos.cxx:609-611
+  if (owner)
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      a.P[i * D + k] = a.p0[i * D + k];
+      a.A[i * D + k] = a.a0[i * D + k];
+    }
+
+  double conv[D], grad[D];
+  for (int t = 0; t + 1 < a.T; t++) {
+    const double *p = t == 0 ? a.p0 : a.P + t * nd;
+    const double *m = t == 0 ? a.a0 : a.A + t * nd;
+    double *p1 = a.P + (t + 1) * nd;
+    double *m1 = a.A + (t + 1) * nd;
+    persist_field<F32, D>(a, p, m, i, slice, conv, grad, sp, sm);
+    if (a.scheme == DFM_EULER) {
+      if (owner)
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          const long q = i * D + k;
+          double vp = 1.0 * p[q], va = 1.0 * m[q];
+          vp += a.dt * conv[k];
+          va += -a.dt * grad[k];
+          p1[q] = frozen ? p[q] : vp;
+          m1[q] = frozen ? 0.0 : va;
+        }
+    } else {  // Heun: predictor, barrier, corrector from the original state
+      if (owner)
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          const long q = i * D + k;
+          double vp = 1.0 * p[q], va = 1.0 * m[q];
+          vp += a.dt * conv[k];
+          va += -a.dt * grad[k];
+          a.ps[q] = vp;
+          a.as[q] = va;
+          a.dp[q] = conv[k];
+          a.da[q] = grad[k];
+        }
+      sync();
+      persist_field<F32, D>(a, a.ps, a.as, i, slice, conv, grad, sp, sm);
+      if (owner)
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          const long q = i * D + k;
+          double vp = 1.0 * p[q], va = 1.0 * m[q];
+          vp += 0.5 * a.dt * a.dp[q];
+          va += -0.5 * a.dt * a.da[q];
+          vp += 0.5 * a.dt * conv[k];
+          va += -0.5 * a.dt * grad[k];
+          p1[q] = frozen ? p[q] : vp;
+          m1[q] = frozen ? 0.0 : va;
+        }
+    }
+    if (t + 2 < a.T) sync();  // the next step reads every point of this one
+  }
+}
+
+// The whole shoot in one cooperative launch when it fits (returns 1 if it ran, 0 to fall
+// back to the per-step launches, or an error).
+int shoot_persistent(hipStream_t st, int D, double sigma, int prec, int scheme, const double *p0,
+                     const double *a0, long n, int T, double dt, double *P, double *A,
+                     const ShootTmp &tmp) {
+  if (n > kPersistMaxPoints || T < 2) return 0;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int dev = 0, coop = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      !coop) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  PersistShoot a;
+  a.count = (unsigned *)tmp.done;
+  a.p0 = p0; a.a0 = a0; a.P = P; a.A = A;
+  a.ps = tmp.ps; a.as = tmp.as; a.dp = tmp.dpos; a.da = tmp.dmom;
+  a.n = n; a.T = T; a.scheme = scheme; a.dt = dt;
+  a.inv_s2 = 1.0 / (sigma * sigma);
+  a.cg = -2.0 / (sigma * sigma);
+  a.spt = n <= 1024 ? kPersistSpt : 16;
+  const long grid = (n + 256 / a.spt - 1) / (256 / a.spt);
+  const bool f32 = prec != DFM_FP64;
+  const void *fn = D == 2 ? (f32 ? (const void *)k_shoot_persistent<true, 2>
+                                 : (const void *)k_shoot_persistent<false, 2>)
+                          : (f32 ? (const void *)k_shoot_persistent<true, 3>
+                                 : (const void *)k_shoot_persistent<false, 3>);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+      (long)per_cu * cus < grid) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  static_assert(kBarrierInts <= 32 * 17, "barrier counters fit the counter array");
+  fill(st, (double *)tmp.done, kBarrierInts / 2, 0.0);  // the barrier counters
+  void *args[] = {&a};
+  hipEvent_t ev = profile_begin(st);
+  hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, st);
+  // one fused Convolve + Grad1 over n x n pairs per field evaluation
+  const double pairs = (double)n * n * (T - 1) * (scheme == DFM_HEUN ? 2 : 1);
+  profile_end(st, ev, pairs, pairs * flops_per_pair(OP_CONVGRAD1, D, D));
+  if (e != hipSuccess) return set_error(DFM_ERR_HIP, "cooperative shoot: %s", hipGetErrorString(e));
+  return 1;
+}
+
 // ---- drivers (nseg == 0: one subject through the sorted single-call pair sums) ---------
 
 int shoot_impl(void *stream, int D, double sigma, int prec, int scheme, int nseg,
@@ -455,11 +719,16 @@ int shoot_impl(void *stream, int D, double sigma, int prec, int scheme, int nseg
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   ShootLayout L(work, prec, D, cs, batch);
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
+  const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
+  if (!batch) {
+    rc = shoot_persistent(cx.st, D, sigma, prec, scheme, p0, a0, cs.total, T, dt, P, A, L.t);
+    if (rc == 1) return DFM_OK;
+    if (rc) return rc;
+  }
   if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.t))) return rc;
   const long nd = cs.total * D;
   if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, cs, L.sa.gate)))
     return rc;
-  const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   for (int t = 0; t + 1 < T; t++)
     if ((rc = shoot_step(cx, scheme, P + t * nd, A + t * nd, cs, dt, P + (t + 1) * nd,
                          A + (t + 1) * nd, L.t, L.sa.gate)))

@@ -55,6 +55,29 @@ static thread_local ProfileState g_profile;
 
 void profile_enable(bool on) { g_profile.on = on; }
 
+// Timing of a launch made outside the pair-sum entry points (the persistent shoot): an
+// event before it and, with its algorithmic work, one after it, while profiling is on.
+hipEvent_t profile_begin(hipStream_t st) {
+  if (!g_profile.on) return nullptr;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  (void)hipEventRecord(e, st);
+  return e;
+}
+
+void profile_end(hipStream_t st, hipEvent_t e0, double pairs, double flops) {
+  if (!e0) return;
+  hipEvent_t e1 = nullptr;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return;
+  }
+  (void)hipEventRecord(e1, st);
+  g_profile.events.push_back({e0, e1});
+  g_profile.pairs += pairs;
+  g_profile.flops += flops;
+}
+
 int profile_work(double *pairs, double *flops) {
   if (pairs) *pairs = g_profile.pairs;
   if (flops) *flops = g_profile.flops;

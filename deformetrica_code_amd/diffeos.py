@@ -87,15 +87,15 @@ class HIPDiffeos:
         if self.a0.shape != self.p0.shape:
             raise RuntimeError("start positions and momenta differ in shape")
         st = self._stream
-        p0 = self._buf("p0", self.p0.shape)
-        a0 = self._buf("a0", self.a0.shape)
-        p0.upload(self.p0, st)
-        a0.upload(self.a0, st)
+        # the start positions and momenta in one buffer, one copy
+        pa0 = self._buf("pa0", (2, n, d))
+        pa0.upload(np.stack([self.p0, self.a0]), st)
+        p0_ptr, a0_ptr = pa0.ptr, pa0.ptr + n * d * 8
         self._P = self._buf("P", (T, n, d))
         self._A = self._buf("A", (T, n, d))
         wp, wb = self._ws.get(L.dfm_shoot_workspace(d, n, self.precision))
-        _lib.check(L.dfm_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme, p0.ptr,
-                               a0.ptr, n, T, self.t0, self.tN, self._P.ptr, self._A.ptr, wp, wb))
+        _lib.check(L.dfm_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme, p0_ptr,
+                               a0_ptr, n, T, self.t0, self.tN, self._P.ptr, self._A.ptr, wp, wb))
         self._X = None
         if self.x0 is not None:
             m = self.x0.shape[0]

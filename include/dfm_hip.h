@@ -88,8 +88,8 @@ int dfm_stream_synchronize(void *stream);
 
 /* Kernel timing for measurement (no reference counterpart: the reference times nothing
  * on the device).  While enabled on the calling thread, every pair-sum kernel launch
- * (with its split reduction, when there is one) is bracketed by HIP events on the stream
- * it runs on.  dfm_profile_read waits for the recorded events, returns their summed
+ * (with its split reduction, when there is one, or the single launch of a small
+ * dfm_shoot) is bracketed by HIP events on the stream it runs on.  dfm_profile_read waits for the recorded events, returns their summed
  * duration and launch count, and forgets them. */
 int dfm_profile_enable(int on);
 int dfm_profile_read(double *total_ms, long *launches);

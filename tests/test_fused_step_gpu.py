@@ -1,11 +1,14 @@
-"""One launch per integrator step: the pair sum over the control points finishes its own
+"""Shooting in few launches.  Up to 4,096 control points a single-subject shoot is one
+cooperative launch for all its steps (k_shoot_persistent, a grid barrier between steps);
+otherwise each step is one launch: the pair sum over the control points finishes its own
 source splits (the last split of each target tile adds up the partials) and applies the
 Euler / Heun update of Diffeos::Shoot (Diffeos.cxx:617-632; the Heun variant of
 Diffeos.cxx:634-652) in the same kernel.  Checked against the step taken with separate
-launches (device conv_grad1, then the update on the host), against the oracle, for single
-subjects and ragged batches, with and without source splits, with zero momenta (the
-reference leaves the points in place, Diffeos.cxx:609-611), on the matrix-core paths that
-apply the step in a kernel of its own, and in explore mode of the autotuner."""
+launches (device conv_grad1, then the update on the host), against the oracle, the two
+forms against each other, for single subjects and ragged batches, with and without source
+splits, with zero momenta (the reference leaves the points in place,
+Diffeos.cxx:609-611), on the matrix-core paths that apply the step in a kernel of its own,
+and in explore mode of the autotuner."""
 import re
 
 import numpy as np
@@ -57,10 +60,11 @@ def _stepwise(p0, a0, width, T, prec, scheme):
 
 
 @pytest.mark.parametrize("scheme", ["euler", "heun"])
-@pytest.mark.parametrize("n", [100, 3000, 9000])
+@pytest.mark.parametrize("n", [100, 3000, 4096, 4097, 9000])
 def test_fused_step_equals_separate_launches_fp64(scheme, n):
-    """n = 100: one split; 3000: many splits (the last split finishes each tile); 9000:
-    FP64 grids of many workgroups."""
+    """n = 100, 3000, 4096: one launch for the whole shoot (1 to 4 LDS chunks of sources);
+    4097, 9000: one launch per step with source splits (the last split finishes each
+    tile)."""
     rng = np.random.default_rng(n)
     p0 = rng.uniform(size=(n, 3))
     a0 = rng.normal(scale=0.01, size=(n, 3))
@@ -93,6 +97,29 @@ def test_zero_momenta_gate_in_the_fused_step():
         for t in range(5):
             np.testing.assert_array_equal(P[t], p0)
             np.testing.assert_array_equal(A[t], np.zeros_like(p0))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32c", 1e-6)])
+def test_one_launch_shoot_equals_per_step_launches(prec, tol):
+    """The same 4,096-point shoot as one cooperative launch (dfm_shoot) and as a batch of
+    one subject (dfm_batch_shoot: one launch per step)."""
+    from deformetrica_code_amd.diffeos import HIPBatchDiffeos
+    rng = np.random.default_rng(77)
+    p0 = rng.uniform(size=(4096, 3))
+    a0 = rng.normal(scale=0.01, size=(4096, 3))
+    for scheme in ("euler", "heun"):
+        P, A = _shoot(p0, a0, 0.15, 6, prec, scheme)
+        b = HIPBatchDiffeos(3, prec)
+        b.SetKernelWidth(0.15)
+        b.SetNumberOfTimePoints(6)
+        b.SetShootingScheme(scheme)
+        b.SetStartPositions([p0])
+        b.SetStartMomentas([a0])
+        b.SetLandmarkPoints([p0[:5]])
+        b.Update()
+        Pb, Ab = np.array(b.GetTrajectoryPositions()[0]), np.array(b.GetTrajectoryMomentas()[0])
+        assert rel(P - p0, Pb - p0) < tol, scheme
+        assert rel(A, Ab) < tol, scheme
 
 
 def test_batched_fused_step_equals_single_subjects():
@@ -128,8 +155,8 @@ def test_fused_step_in_autotuner_explore_mode(oracle_mod):
     from deformetrica_code_amd import _lib
     from oracle import deformations as dfo
     rng = np.random.default_rng(21)
-    p0 = rng.uniform(size=(3000, 3))
-    a0 = rng.normal(scale=0.01, size=(3000, 3))
+    p0 = rng.uniform(size=(5000, 3))   # above the one-launch shoot's 4,096 points
+    a0 = rng.normal(scale=0.01, size=(5000, 3))
     Po, Ao = dfo.shoot(p0, a0, 0.1, 3)
     L = _lib.load()
     L.dfm_autotune_reset()
