@@ -92,17 +92,21 @@ struct KParams {
   double exact_extent;  // FP32C: wavefront extent (scaled units) above which differences are exact
   double exact_extent_bf;  // DFM_BF16MFMA: group extent above which k_pairsum runs the group
   int wave_filter;      // 0 every wavefront; 1 only wide ones (the rest run on the matrix cores)
+  // Source splits finished in k_pairsum (null: by k_reduce_splits): the wavefront of each
+  // target tile (workgroup along the targets, times segment) that is the last of the
+  // p.nsplit splits to store its partials adds them up and finalizes.  One counter per tile,
+  // zero at launch and reset by that last wavefront; the caller (the integrators) owns the
+  // array, clears it once, and orders the launches that share it on one stream.
+  int *done;
   // Fused integrator step (OP_CONVGRAD1 with x = positions, ta = w = momenta):
   //   e_p1 = e_pb [+ e_cx e_px] + e_c conv,   e_a1 = e_ab [- e_cx e_ax] - e_c grad,
   // gated per segment by e_gate (squared norm of the segment's initial momenta below 1e-40:
-  // e_p1 = e_pb, e_a1 = 0).  k_pairsum applies it to each target once its sums are complete:
-  // with source splits the last split of each target tile to finish (counters e_done, zero
-  // between launches) adds up the partials; other kernels leave it to k_step_update.
+  // e_p1 = e_pb, e_a1 = 0).  k_pairsum applies it to each target once its sums are complete;
+  // other kernels leave it to k_step_update.
   int e_on;
   double *e_p1, *e_a1;
   const double *e_pb, *e_ab, *e_px, *e_ax, *e_gate;
   double e_c, e_cx;
-  int *e_done;
 };
 
 // Work decomposition and workspace layout of one call.

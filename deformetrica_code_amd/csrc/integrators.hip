@@ -134,6 +134,7 @@ struct Ctx {
   void *pw;      // pair-sum workspace
   size_t pw_bytes;
   bool batch;    // capture-safe batched pair sums (no Morton sort)
+  int *done = nullptr;  // split-completion counters of the fused steps (KParams::done)
 };
 
 int check_launch(const char *what) {
@@ -240,39 +241,42 @@ size_t pair_ws(int prec, int D, const Seg &cs, const Seg &ls, bool batch) {
 
 // ---- forward steps ------------------------------------------------------------------
 
-// Counters of the shoot's workspace: one completion counter per target tile (workgroups
-// along the targets times segments) of a fused step's pair sum, and at least the grid
-// barrier's counters of the one-launch shoot.
-long step_tiles(const Seg &cs) {
-  const long tiles = cs.n > 1 ? (cs.max / 64 + 1) * cs.n : cs.total / 64 + 1;
-  return std::max<long>(tiles, 32 * 17);
+// Target tiles (workgroups along the targets, times segments) of a pair sum whose targets
+// are segmented by s: at most this many.
+long tile_bound(const Seg &s) { return s.n > 1 ? (s.max / 64 + 1) * s.n : s.total / 64 + 1; }
+
+// The driver's counters: one split-completion counter per target tile of its pair sums
+// (KParams::done; targets are the control points or the landmarks), which also hold the
+// grid barrier of the one-launch shoot (kBarrierInts).
+struct Counters {
+  int *done;
+  long n;
+  void take(Arena &ar, const Seg &cs, const Seg &ls) {
+    n = std::max<long>(std::max(tile_bound(cs), ls.total > 0 ? tile_bound(ls) : 0), 32 * 17);
+    done = (int *)ar.take((size_t)(n + 1) / 2);
+  }
+};
+
+// zero them (a fill kernel: capture-safe); each pair sum leaves them zero again
+int clear_done(const Ctx &cx, const Counters &c) {
+  fill(cx.st, (double *)c.done, (c.n + 1) / 2, 0.0);
+  return check_launch("clear counters");
 }
 
 struct ShootTmp {
   double *dpos, *dmom, *ps, *as, *dpos2, *dmom2;
-  int *done;    // fused-step tile counters, zero between launches
-  long ndone;
-  void take(Arena &ar, long nd, long tiles) {
+  void take(Arena &ar, long nd) {
     dpos = ar.take(nd); dmom = ar.take(nd); ps = ar.take(nd); as = ar.take(nd);
     dpos2 = ar.take(nd); dmom2 = ar.take(nd);
-    ndone = tiles;
-    done = (int *)ar.take((size_t)(tiles + 1) / 2);
   }
 };
-
-// zero the fused-step counters (a fill kernel: capture-safe)
-int clear_done(const Ctx &cx, const ShootTmp &t) {
-  fill(cx.st, (double *)t.done, (t.ndone + 1) / 2, 0.0);
-  return check_launch("clear counters");
-}
 
 // One pair sum over the control points (p, a) with the step applied in the same launch
 // (KParams e_*): p1 = pb [+ cx px] + c conv, a1 = ab [- cx ax] - c grad, gated by `gate`;
 // conv and grad themselves go to dpos and dmom.
 int fused_step_sum(const Ctx &cx, const double *p, const double *a, const Seg &cs, double *dpos,
                    double *dmom, double *p1, double *a1, const double *pb, const double *ab,
-                   double c, const double *px, const double *ax, double cxx, const double *gate,
-                   int *done) {
+                   double c, const double *px, const double *ax, double cxx, const double *gate) {
   KParams q = blank();
   q.x = p; q.ta = a; q.y = p; q.w = a;
   q.out0 = dpos; q.out1 = dmom;
@@ -280,7 +284,11 @@ int fused_step_sum(const Ctx &cx, const double *p, const double *a, const Seg &c
   q.e_p1 = p1; q.e_a1 = a1; q.e_pb = pb; q.e_ab = ab;
   q.e_px = px; q.e_ax = ax; q.e_c = c; q.e_cx = cxx;
   q.e_gate = gate;
-  q.e_done = done;
+  // The step's own sums finish their splits in the kernel.  Other sums keep the separate
+  // reduction: the device-scope fence before each wavefront's arrival writes back the L2,
+  // which on the large grids of the flow and the adjoint cost more than the launch it saves
+  // (registration gradient 25.3 against 12.7 ms, same box).
+  q.done = cx.done;
   const long nd = cs.total * cx.D;
   return ps(cx, OP_CONVGRAD1, q, cs, cs, nd, nd);
 }
@@ -291,13 +299,13 @@ int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, cons
                double dt, double *p1, double *a1, const ShootTmp &t, const double *gate) {
   if (scheme == DFM_EULER)
     return fused_step_sum(cx, p, a, cs, t.dpos, t.dmom, p1, a1, p, a, dt, nullptr, nullptr, 0.0,
-                          gate, t.done);
+                          gate);
   // Heun: predictor (ungated), then the corrector from the original state
   int rc = fused_step_sum(cx, p, a, cs, t.dpos, t.dmom, t.ps, t.as, p, a, dt, nullptr, nullptr,
-                          0.0, nullptr, t.done);
+                          0.0, nullptr);
   if (rc) return rc;
   return fused_step_sum(cx, t.ps, t.as, cs, t.dpos2, t.dmom2, p1, a1, p, a, 0.5 * dt, t.dpos,
-                        t.dmom, 0.5 * dt, gate, t.done);
+                        t.dmom, 0.5 * dt, gate);
 }
 
 struct FlowTmp {
@@ -345,13 +353,15 @@ struct SegArrays {
 struct ShootLayout {
   SegArrays sa;
   ShootTmp t;
+  Counters c;
   void *pw;
   size_t pw_bytes;
   size_t total;
   ShootLayout(void *base, int prec, int D, const Seg &cs, bool batch) {
     Arena ar(base);
     sa.take(ar, cs.n);
-    t.take(ar, cs.total * D, step_tiles(cs));
+    t.take(ar, cs.total * D);
+    c.take(ar, cs, Seg::single(0));
     pw_bytes = pair_ws(prec, D, cs, Seg::single(0), batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
@@ -361,6 +371,7 @@ struct ShootLayout {
 struct FlowLayout {
   SegArrays sa;
   FlowTmp t;
+  Counters c;
   void *pw;
   size_t pw_bytes;
   size_t total;
@@ -368,6 +379,7 @@ struct FlowLayout {
     Arena ar(base);
     sa.take(ar, cs.n);
     t.take(ar, ls.total * D);
+    c.take(ar, cs, ls);
     pw_bytes = pair_ws(prec, D, cs, ls, batch);
     pw = ar.take_bytes(pw_bytes);
     total = ar.off;
@@ -381,6 +393,7 @@ struct AdjointLayout {
   double *seg;    // (k + 1) x (2n + m) x D
   ShootTmp st;
   FlowTmp ft;
+  Counters c;
   double *lp, *la, *lpn, *lan, *lpf, *laf, *jp, *ja, *jsp, *jsa, *nup, *nua;  // n x D
   double *th, *thn, *ths, *mu;                                               // m x D
   void *pw;
@@ -396,8 +409,9 @@ struct AdjointLayout {
     sa.take(ar, cs.n);
     ckpt = ar.take((size_t)(segs + 1) * rec);
     seg = ar.take((size_t)(std::min(k, std::max(T - 1, 0)) + 1) * rec);
-    st.take(ar, n * D, step_tiles(cs));
+    st.take(ar, n * D);
     ft.take(ar, m * D);
+    c.take(ar, cs, ls);
     double **vn[] = {&lp, &la, &lpn, &lan, &lpf, &laf, &jp, &ja, &jsp, &jsa, &nup, &nua};
     for (double **v : vn) *v = ar.take(n * D);
     double **vm[] = {&th, &thn, &ths, &mu};
@@ -653,7 +667,7 @@ __global__ __launch_bounds__(256) void k_shoot_persistent(PersistShoot a) {
 // back to the per-step launches, or an error).
 int shoot_persistent(hipStream_t st, int D, double sigma, int prec, int scheme, const double *p0,
                      const double *a0, long n, int T, double dt, double *P, double *A,
-                     const ShootTmp &tmp) {
+                     const ShootTmp &tmp, int *counters) {
   if (n > kPersistMaxPoints || T < 2) return 0;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
@@ -669,7 +683,7 @@ int shoot_persistent(hipStream_t st, int D, double sigma, int prec, int scheme, 
     return 0;
   }
   PersistShoot a;
-  a.count = (unsigned *)tmp.done;
+  a.count = (unsigned *)counters;
   a.p0 = p0; a.a0 = a0; a.P = P; a.A = A;
   a.ps = tmp.ps; a.as = tmp.as; a.dp = tmp.dpos; a.da = tmp.dmom;
   a.n = n; a.T = T; a.scheme = scheme; a.dt = dt;
@@ -689,7 +703,7 @@ int shoot_persistent(hipStream_t st, int D, double sigma, int prec, int scheme, 
     return 0;
   }
   static_assert(kBarrierInts <= 32 * 17, "barrier counters fit the counter array");
-  fill(st, (double *)tmp.done, kBarrierInts / 2, 0.0);  // the barrier counters
+  fill(st, (double *)counters, kBarrierInts / 2, 0.0);  // the barrier counters
   void *args[] = {&a};
   hipEvent_t ev = profile_begin(st);
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, st);
@@ -721,11 +735,13 @@ int shoot_impl(void *stream, int D, double sigma, int prec, int scheme, int nseg
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   if (!batch) {
-    rc = shoot_persistent(cx.st, D, sigma, prec, scheme, p0, a0, cs.total, T, dt, P, A, L.t);
+    rc = shoot_persistent(cx.st, D, sigma, prec, scheme, p0, a0, cs.total, T, dt, P, A, L.t,
+                          L.c.done);
     if (rc == 1) return DFM_OK;
     if (rc) return rc;
   }
-  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.t))) return rc;
+  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.c))) return rc;
+  cx.done = L.c.done;
   const long nd = cs.total * D;
   if ((rc = copy(cx, P, p0, nd)) || (rc = copy(cx, A, a0, nd)) || (rc = norm2(cx, a0, cs, L.sa.gate)))
     return rc;
@@ -754,7 +770,8 @@ int flow_impl(void *stream, int D, double sigma, int prec, int improved_euler, i
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   FlowLayout L(work, prec, D, cs, ls, batch);
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
-  if ((rc = upload_segs(cx, cs, ls, L.sa))) return rc;
+  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.c))) return rc;
+  cx.done = L.c.done;
   const long nd = cs.total * D, md = ls.total * D;
   if ((rc = copy(cx, X, x0, md))) return rc;
   if (cs.total == 0) {
@@ -889,7 +906,8 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   AdjointLayout L(work, prec, D, cs, ls, T, checkpoint_every, batch);
   Ctx cx{(hipStream_t)stream, D, sigma, prec, L.pw, L.pw_bytes, batch};
-  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.st))) return rc;
+  if ((rc = upload_segs(cx, cs, ls, L.sa)) || (rc = clear_done(cx, L.c))) return rc;
+  cx.done = L.c.done;
   const long nd = n * D, md = m * D, rec = 2 * nd + md;
   const double dt = T > 1 ? (tN - t0) / (T - 1) : 0.0;
   double *gate = L.sa.gate;

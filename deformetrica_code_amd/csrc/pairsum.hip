@@ -728,12 +728,13 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
       return set_error(DFM_ERR_HIP, "hipEventCreate failed");
   }
-  // a fused integrator step runs inside k_pairsum; the matrix-core kernels leave it to
-  // k_step_update after them
-  const bool step_after = p.e_on && (prec == DFM_BF16MFMA ||
-                                     (prec == DFM_FP32C && pl.mx && mx_supported(op, D, DV) &&
-                                      pl.R == 1));
+  // k_pairsum finishes its own source splits and a fused integrator step; the matrix-core
+  // kernels leave the splits to k_reduce_splits and the step to k_step_update after them
+  const bool matrix_cores = prec == DFM_BF16MFMA ||
+                            (prec == DFM_FP32C && pl.mx && mx_supported(op, D, DV) && pl.R == 1);
+  const bool step_after = p.e_on && matrix_cores;
   if (step_after) p.e_on = 0;
+  if (matrix_cores) p.done = nullptr;
   if (ev0) (void)hipEventRecord(ev0, st);
   int rc = prec == DFM_FP64       ? launch_op_f64(st, op, D, DV, p, pl)
            : prec == DFM_FP32X    ? launch_op_f32x(st, op, D, DV, p, pl)

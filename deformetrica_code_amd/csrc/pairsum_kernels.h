@@ -625,31 +625,33 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
       for (int a = 0; a < NACC; a++) dst[a] = accd[r][a];
     }
   }
+  if (p.nsplit > 1 && p.done) {
+    // publish this split's partials, count it, and let the tile's last split add them up
+    // (in the split order of k_reduce_splits) and finalize
+    __threadfence();
+    int last = 0;
+    if (lane == 0) last = atomicAdd(&p.done[tile], 1) == p.nsplit - 1;
+    last = __shfl(last, 0);
+    if (!last) return;
+    __threadfence();
+    if (lane == 0) p.done[tile] = 0;  // zero again for the next launch
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (!valid[r]) continue;
+#pragma unroll
+      for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
+      for (int sp = 0; sp < p.nsplit; sp++) {
+        const double *src = p.partial + ((long)sp * p.nx + pos[r]) * NACC;
+#pragma unroll
+        for (int a = 0; a < NACC; a++) accd[r][a] += src[a];
+      }
+      finalize<OP, D, DV>(accd[r], p, orig[r]);
+    }
+  } else if (p.nsplit > 1) {
+    return;  // k_reduce_splits finishes
+  }
   if constexpr (OP == OP_CONVGRAD1 && DV == D) {
     if (p.e_on) {
-      if (p.nsplit > 1) {
-        // publish this split's partials, count it, and let the tile's last split finish
-        __threadfence();
-        int last = 0;
-        if (lane == 0) last = atomicAdd(&p.e_done[tile], 1) == p.nsplit - 1;
-        last = __shfl(last, 0);
-        if (!last) return;
-        __threadfence();
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (!valid[r]) continue;
-          // the split order of k_reduce_splits
-#pragma unroll
-          for (int a = 0; a < NACC; a++) accd[r][a] = 0.0;
-          for (int sp = 0; sp < p.nsplit; sp++) {
-            const double *src = p.partial + ((long)sp * p.nx + pos[r]) * NACC;
-#pragma unroll
-            for (int a = 0; a < NACC; a++) accd[r][a] += src[a];
-          }
-          finalize<OP, D, DV>(accd[r], p, orig[r]);
-        }
-        if (lane == 0) p.e_done[tile] = 0;  // ready for the next launch
-      }
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if (!valid[r]) continue;
@@ -730,9 +732,8 @@ static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
     case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE>), grid, dim3(64), 0, st, p); break;
     default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4, MODE>), grid, dim3(64), 0, st, p); break;
   }
-  // a fused integrator step finishes the splits in k_pairsum itself
-  constexpr bool kStep = OP == OP_CONVGRAD1 && DV == D;
-  if (pl.nsplit > 1 && !(kStep && p.e_on))
+  // with completion counters k_pairsum finishes its own source splits
+  if (pl.nsplit > 1 && !p.done)
     hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
                        dim3(256), 0, st, p);
 }
