@@ -104,6 +104,9 @@ def parse():
                     choices=["kernel", "shooting", "registration", "atlas", "varifold",
                              "curves"])
     ap.add_argument("--subjects", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="workload legs: skip the autotuned repeat (for kernel traces of the "
+                         "default plans)")
     return ap.parse_args()
 
 
@@ -416,11 +419,11 @@ def workload_shooting(args):
             df.Update()
             wall, kern, launches = _profiled(L, df.Update, 5)
             roof = _roof(kern)
-            twall, tkern, _ = _tuned(L, df.Update, 5)
+            twall, tkern, _ = (None, None, None) if args.no_tune else _tuned(L, df.Update, 5)
             row[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3,
                          "pairsum_launches": launches, "roofline": roof,
-                         "autotuned_ms": twall * 1e3,
-                         "autotuned_pairsum_kernel_ms": tkern * 1e3,
+                         "autotuned_ms": twall and twall * 1e3,
+                         "autotuned_pairsum_kernel_ms": tkern and tkern * 1e3,
                          "rel_err_final_positions_vs_oracle":
                              _nrel(P[-1] - p0, df.GetTrajectoryPositions()[-1] - p0),
                          "rel_err_final_momenta_vs_oracle": _nrel(A[-1], df.GetTrajectoryMomentas()[-1])}
@@ -462,7 +465,7 @@ def workload_registration(args):
         run()
         wall, kern, launches = _profiled(L, run, 3 if prec == "fp32c" else 1)
         roof = _roof(kern)
-        tuned = _tuned(L, run, 3)[:2] if prec == "fp32c" else (None, None)
+        tuned = _tuned(L, run, 3)[:2] if prec == "fp32c" and not args.no_tune else (None, None)
         out[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "pairsum_launches": launches,
                      "roofline": roof,
                      "autotuned_ms": tuned[0] and tuned[0] * 1e3,
