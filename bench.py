@@ -600,7 +600,7 @@ def workload_varifold(args):
     ws = Workspace()
     d = [DeviceArray.from_numpy(v, st) for v in (cs, Ns, ct, Nt)]
     res = {}
-    for prec, reps in (("fp32c", 2), ("fp64", 1)):
+    for prec, reps in (("bf16mfma", 2), ("fp32c", 2), ("fp64", 1)):
         run = lambda: (samples_match_gradient("varifold", 0.02, *d, precision=prec, stream=st,
                                               workspace=ws), st.synchronize())
         out = run()[0]
@@ -609,18 +609,25 @@ def workload_varifold(args):
                      "roofline": _roof(kern),
                      "match": float(out[0].numpy(st)[0]), "grad_c": out[1].numpy(st),
                      "grad_n": out[2].numpy(st)}
-    r = res["fp32c"]
+    f64 = res["fp64"]
+
+    def errs(r):
+        return {"match_rel_err_vs_fp64": abs(r["match"] - f64["match"]) / abs(f64["match"]),
+                "grad_centres_rel_err_vs_fp64": _nrel(f64["grad_c"], r["grad_c"]),
+                "grad_normals_rel_err_vs_fp64": _nrel(f64["grad_n"], r["grad_n"])}
+
+    r, c = res["bf16mfma"], res["fp32c"]
     pairs = r["roofline"]["pairs_per_call"]
     return _line("varifold data term pair-evals/s (1M x 1M samples, match + gradient)",
                  pairs / (r["ms"] / 1e3), "pair-evals/s", True, r["ms"],
                  {"workload": "two sets of %d oriented samples of the unit sphere (normals radial "
                               "x log-normal(-9, 0.3) areas), varifold kernel, sigma 0.02: "
-                              "|S|^2 + |T|^2 - 2<S,T> and d/d centres, d/d normals" % n},
-                 dtype="f32", roofline=r["roofline"], fp64_roofline=res["fp64"]["roofline"],
-                 fp64_ms=res["fp64"]["ms"], speedup_fp32c_over_fp64=res["fp64"]["ms"] / r["ms"],
-                 match_rel_err_vs_fp64=abs(r["match"] - res["fp64"]["match"]) / abs(res["fp64"]["match"]),
-                 grad_centres_rel_err_vs_fp64=_nrel(res["fp64"]["grad_c"], r["grad_c"]),
-                 grad_normals_rel_err_vs_fp64=_nrel(res["fp64"]["grad_n"], r["grad_n"]))
+                              "|S|^2 + |T|^2 - 2<S,T> and d/d centres, d/d normals; "
+                              "DFM_BF16MFMA (distances' cross term on the matrix cores)" % n},
+                 dtype="f32", roofline=r["roofline"], fp64_roofline=f64["roofline"],
+                 fp64_ms=f64["ms"], speedup_over_fp64=f64["ms"] / r["ms"], **errs(r),
+                 fp32c={"ms": c["ms"], "pair_evals_per_s": pairs / (c["ms"] / 1e3),
+                        "roofline": c["roofline"], **errs(c)})
 
 
 def workload_curves(args):

@@ -50,7 +50,8 @@ class HIPExactKernel : public ExactKernel<ScalarType, PointDim> {
   virtual ~HIPExactKernel() {}
 
   /// DFM_FP64 (the reference arithmetic) or DFM_FP32C.  The process-wide default is
-  /// DFM_FP64, or the value of the environment variable DFM_PRECISION ("fp64"/"fp32c").
+  /// DFM_FP64, or the value of the environment variable DFM_PRECISION ("fp64", "fp32c",
+  /// "fp32x" or "bf16mfma").
   void SetPrecision(int precision) { m_Precision = precision; }
   int GetPrecision() const { return m_Precision; }
 
@@ -58,6 +59,8 @@ class HIPExactKernel : public ExactKernel<ScalarType, PointDim> {
     CheckAbi();
     const char *env = std::getenv("DFM_PRECISION");
     if (env && std::strcmp(env, "fp32c") == 0) return DFM_FP32C;
+    if (env && std::strcmp(env, "fp32x") == 0) return DFM_FP32X;
+    if (env && std::strcmp(env, "bf16mfma") == 0) return DFM_BF16MFMA;
     return DFM_FP64;
   }
   /// Throws when the loaded libdfm_hip.so was built from another ABI version of dfm_hip.h

@@ -10,8 +10,9 @@ Checks:
   * d/d centres and d/d normals on 256 sampled source rows against the oracle's kernel
     sums with Dv = 6 weights over all 2,000,000 samples
     (oracle/surfaces.py samples_varifold_gradient_rows, pinned by central differences of
-    the definitional double sum): FP64 to 1e-10 norm-wise, FP32C to 1e-6 norm-wise;
-  * FP32C against FP64 over every row: gradients norm-wise <= 1e-6, the match to 1e-8
+    the definitional double sum): FP64 to 1e-10 norm-wise, FP32C and DFM_BF16MFMA to 1e-6;
+  * FP32C and DFM_BF16MFMA (the distances' cross term on the matrix cores, as the workload
+    names it) against FP64 over every row: gradients norm-wise <= 1e-6, the match to 1e-8
     relative (the match is a difference of three sums of 1e12 terms each).
 """
 import numpy as np
@@ -40,7 +41,7 @@ def result():
     ws = Workspace()
     d = [DeviceArray.from_numpy(v, st) for v in (cs, Ns, ct, Nt)]
     out = {}
-    for prec in ("fp64", "fp32c"):
+    for prec in ("fp64", "fp32c", "bf16mfma"):
         print("varifold 1M x 1M samples:", prec, flush=True)
         m, gc, gn = samples_match_gradient("varifold", SIGMA, *d, precision=prec, stream=st,
                                            workspace=ws)
@@ -58,7 +59,7 @@ def oracle_rows(result):
     return rows, gc, gn
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32c", 1e-6), ("bf16mfma", 1e-6)])
 def test_gradient_rows_match_oracle(result, oracle_rows, prec, tol):
     rows, oc, on = oracle_rows
     _, gc, gn = result[4][prec]
@@ -67,9 +68,10 @@ def test_gradient_rows_match_oracle(result, oracle_rows, prec, tol):
     assert errs[0] <= tol and errs[1] <= tol, errs
 
 
-def test_fp32c_matches_fp64(result):
+@pytest.mark.parametrize("prec", ["fp32c", "bf16mfma"])
+def test_fp32_forms_match_fp64(result, prec):
     m64, gc64, gn64 = result[4]["fp64"]
-    m32, gc32, gn32 = result[4]["fp32c"]
+    m32, gc32, gn32 = result[4][prec]
     errs = {"match": abs(m32 - m64) / abs(m64), "centres": nrel(gc32, gc64),
             "normals": nrel(gn32, gn64)}
     print(errs)
