@@ -118,7 +118,8 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   constexpr int NTE = (OP == OP_CONV) ? 1 : DV;
   // the weight dot product a_i . w_j on the matrix cores too (six slots per component)
   constexpr bool DOT = OP != OP_CONV && 6 * DV <= 32;
-  static_assert(NT * 16 * 4 == kBfGroup, "four wavefronts of NT 16-target tiles cover a group");
+  constexpr int GROUP = 64 * NT;  // four wavefronts of NT tiles of 16 targets
+  static_assert(NT >= 1 && NT <= 4, "a lane group of 16 lanes finishes each tile");
   static_assert(S::EXPANDABLE, "Convolve, Grad1 and the fused sum");
 
   // A rows by K-slot group: smA[q * TS + j] holds slots 8q .. 8q + 7 of staged source j, so
@@ -131,14 +132,15 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int split = blockIdx.y;
-  const long gbase = (long)blockIdx.x * kBfGroup;
+  const long gbase = (long)blockIdx.x * GROUP;
+  const long gend = std::min(p.nx, gbase + GROUP);
   if (gbase >= p.nx) return;  // the whole workgroup
 
   // ---- the group's bounding box (FP64, exact min / max) and centre
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   {
     const long pos = gbase + tid;
-    if (pos < p.nx) {
+    if (pos < gend) {
       const long o = p.perm ? (long)p.perm[pos] : pos;
 #pragma unroll
       for (int k = 0; k < D; k++) lo[k] = hi[k] = p.x[o * D + k];
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     long wpos[1], worig[1];
     bool wvalid[1];
     double wxd[1][D], wc[D], wext;
-    load_targets<float, D, 1>(p, lane, gbase + 64 * wave, p.nx, wpos, worig, wvalid, wxd, wc,
+    load_targets<float, D, 1>(p, lane, gbase + 64 * wave, gend, wpos, worig, wvalid, wxd, wc,
                               wext);
     const long wjs = (long)split * p.split_len;
     const long wje = std::min(p.ny, wjs + p.split_len);
@@ -190,15 +192,16 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     return;
   }
 
-  // ---- this wavefront's targets: tile t, column n -> position gbase + 64 wave + 16 t + n
+  // ---- this wavefront's targets: tile t, column n -> position wbase + 16 t + n
   const int h = lane >> 4, n = lane & 15;
+  const long wbase = gbase + 16 * NT * wave;
   float al[DOT ? 1 : NT][NTE];  // a_i in FP32 (VALU dot product only)
   bf16x8 bop[NT];               // distance columns
   bf16x8 bdot[DOT ? NT : 1];    // dot-product columns
 #pragma unroll
   for (int t = 0; t < NT; t++) {
-    const long pos = gbase + 64 * wave + 16 * t + n;
-    const bool v = pos < p.nx;
+    const long pos = wbase + 16 * t + n;
+    const bool v = pos < gend;
     const long o = v ? (p.perm ? (long)p.perm[pos] : pos) : 0;
     __bf16 xp[D][3];
 #pragma unroll
@@ -366,9 +369,10 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     }
   }
 
-  // ---- lane (h, n) finishes and stores target tile h, column n
-  const long pos = gbase + 64 * wave + 16 * h + n;
-  if (pos >= p.nx) return;
+  // ---- lane (h, n) finishes and stores target tile h, column n (lane groups h >= NT hold
+  // no tile)
+  const long pos = wbase + 16 * h + n;
+  if (h >= NT || pos >= gend) return;
   const long o = p.perm ? (long)p.perm[pos] : pos;
   float xm[D], am[NTE];
 #pragma unroll
