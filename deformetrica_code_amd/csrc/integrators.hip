@@ -133,7 +133,7 @@ struct Ctx {
   int prec;
   void *pw;      // pair-sum workspace
   size_t pw_bytes;
-  bool batch;    // capture-safe batched pair sums (no Morton sort)
+  bool batch;    // capture-safe batched pair sums (no Hilbert sort)
   int *done = nullptr;  // split-completion counters of the fused steps (KParams::done)
 };
 

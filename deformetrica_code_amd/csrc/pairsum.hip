@@ -1,5 +1,5 @@
 // pairsum.hip — host side of the pair-sum engine: per-thread error and timing state,
-// the Morton ordering of the targets (bounding box, 30-bit keys, rocPRIM radix sort),
+// the Hilbert ordering of the targets (bounding box, 30-bit keys, rocPRIM radix sort),
 // the work plan and workspace layout of a call, and the launch of the per-precision
 // kernel instantiations (pairsum_f64.hip, pairsum_f32c.hip, pairsum_f32x.hip).
 #include <hip/hip_runtime.h>
@@ -134,7 +134,7 @@ int profile_read(double *total_ms, long *launches) {
 }
 
 // ------------------------------------------------------------------------------------
-// Morton ordering of the targets (FP32C)
+// Hilbert ordering of the targets (FP32C, BF16MFMA)
 
 __global__ __launch_bounds__(256) void k_bbox_partial(const double *x, long n, int D,
                                                       double *part) {
@@ -186,7 +186,38 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) {  // 15 bits -> every o
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_morton_keys(const double *x, long n, int D,
+// Hilbert index of a cell with N-dimensional coordinates X[] of B bits each: J. Skilling,
+// "Programming the Hilbert curve", AIP Conf. Proc. 707, 381 (2004): the axes are
+// transformed in place into the "transpose" of the index (undo the excess work of each
+// level's inversions and exchanges, then Gray-encode), whose bits interleaved, most
+// significant first and X[0] first within a bit, give the index.  Consecutive indices are
+// face-adjacent cells, so runs of sorted targets stay compact where a Morton (Z) order
+// jumps across octant boundaries.
+template <int N, int B>
+__device__ __forceinline__ void hilbert_transpose(uint32_t (&X)[N]) {
+  for (uint32_t Q = 1u << (B - 1); Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;  // invert the low bits of X[0]
+      } else {      // exchange the low bits of X[0] and X[i]
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 1; i < N; i++) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = 1u << (B - 1); Q > 1; Q >>= 1)
+    if (X[N - 1] & Q) t ^= Q - 1;
+#pragma unroll
+  for (int i = 0; i < N; i++) X[i] ^= t;
+}
+
+__global__ __launch_bounds__(256) void k_curve_keys(const double *x, long n, int D,
                                                      const double *part, int nparts,
                                                      uint32_t *keys, uint32_t *vals) {
   double lo[3] = {INFINITY, INFINITY, INFINITY};
@@ -209,8 +240,15 @@ __global__ __launch_bounds__(256) void k_morton_keys(const double *x, long n, in
     f = fmin(fmax(f, 0.0), cells - 1.0);
     q[k] = (uint32_t)f;
   }
-  keys[i] = D == 3 ? (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2))
-                   : (spread2(q[0]) | (spread2(q[1]) << 1));
+  if (D == 3) {
+    uint32_t X[3] = {q[0], q[1], q[2]};
+    hilbert_transpose<3, 10>(X);
+    keys[i] = (spread3(X[0]) << 2) | (spread3(X[1]) << 1) | spread3(X[2]);
+  } else {
+    uint32_t X[2] = {q[0], q[1]};
+    hilbert_transpose<2, 15>(X);
+    keys[i] = (spread2(X[0]) << 1) | spread2(X[1]);
+  }
   vals[i] = (uint32_t)i;
 }
 
@@ -277,7 +315,7 @@ static int tile_of(int prec) {
 }
 
 // DFM_BF16MFMA runs the matrix-core kernel only for single Convolve / Grad1 / fused calls
-// large enough to be Morton-sorted; everything else runs as DFM_FP32C.
+// large enough to be Hilbert-sorted; everything else runs as DFM_FP32C.
 static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx) {
   if (prec != DFM_BF16MFMA) return prec;
   return (nseg == 1 && nx >= kSortMinTargets && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
@@ -458,7 +496,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
 
 Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
                      long max_ny) {
-  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Morton-sorted
+  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Hilbert-sorted
   Plan best = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny,
                          default_choice(op, prec, D, DV, nseg, max_nx));
   for (const Choice &c : candidates(op, prec, D, DV, nseg, max_nx)) {
@@ -621,7 +659,7 @@ int tune_report(char *buf, size_t len) {
   return (int)out.size();
 }
 
-// Sort targets along a Morton curve; writes perm into the workspace.
+// Sort targets along a Hilbert curve; writes perm into the workspace.
 int sort_targets(hipStream_t st, const double *x, long nx, int D, char *work, const Plan &pl) {
   double *part = (double *)(work + pl.off_bbox);
   uint32_t *kin = (uint32_t *)(work + pl.off_keys_in);
@@ -630,7 +668,7 @@ int sort_targets(hipStream_t st, const double *x, long nx, int D, char *work, co
   uint32_t *perm = (uint32_t *)(work + pl.off_perm);
   const int nparts = (int)std::min<long>(kBBoxParts, std::max(1L, (nx + 255) / 256));
   hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(256), 0, st, x, nx, D, part);
-  hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, x, nx,
+  hipLaunchKernelGGL(k_curve_keys, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, x, nx,
                      D, part, nparts, kin, vin);
   size_t tmp = pl.sort_tmp_bytes;
   hipError_t e = rocprim::radix_sort_pairs((void *)(work + pl.off_sort_tmp), tmp, kin, kout, vin,
@@ -779,7 +817,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
                   const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
                   size_t work_bytes) {
   if (xs.n != ys.n) return set_error(DFM_ERR_ARG, "target and source segment counts differ");
-  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Morton-sorted
+  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Hilbert-sorted
   p.nx = xs.total;
   p.ny = ys.total;
   p.xoff = xs.n > 1 ? xs.d : nullptr;

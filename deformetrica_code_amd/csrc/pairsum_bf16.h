@@ -19,13 +19,14 @@
 // Layout: sources on the MFMA rows (A, lane l holds source row l%16, K slots 8(l/16)..+7),
 // targets on the columns (B, fixed per wavefront), so lane l receives r' of its target
 // column l%16 against sources 4(l/16)..4(l/16)+3 of each 16-source group and accumulates
-// per target; the four lane groups' partial sums are added with two shuffles at the end.
+// per target; the four lane groups' partial sums are added by a reduce-scatter (fold4).
 // A wavefront holds 4 tiles of 16 targets; four wavefronts share the workgroup's 256
 // targets and one LDS tile of 256 staged sources (A rows 64 B, FP32 y and w).
 //
-// A group whose targets span more than kBfExtent scaled units (where the Morton order
-// jumps; 1.4% of the targets of 1M uniform points) is summed in the same workgroup by the
-// DFM_FP32C wavefront code (sum_wavefront), one wavefront per 64 targets.  Same accumulator
+// A group whose targets span more than kBfExtent scaled units (none of the Hilbert-ordered
+// groups of 1M uniform points at sigma 0.05; 12% of those of 1M samples of a sphere at
+// sigma 0.02) is summed in the same workgroup by the DFM_FP32C wavefront code
+// (sum_wavefront), one wavefront per 64 targets.  Same accumulator
 // layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
 //
 // The reference arithmetic this replaces: the per-pair squared distance of
@@ -59,6 +60,7 @@ __host__ __device__ constexpr BfSlot bf_slot(int s) {
   return BfSlot{0, 0, 0, 0};
 }
 static_assert(3 + 6 * 3 <= 32, "the split terms fit the K dimension");
+static_assert(6 * 6 <= 32 + 8, "the dot product's slots fit two MFMAs, the second in slots 32..39");
 
 __device__ __forceinline__ void bf_split3(float v, __bf16 (&o)[3]) {
   // v = o0 + o1 + o2 exactly for an FP32 v: each residual of a round-to-nearest is exact
@@ -78,8 +80,24 @@ __device__ __forceinline__ float sum_lane_groups(float v) {
   return __uint_as_float(b[0]) + __uint_as_float(b[1]);
 }
 
-// K slots of the weight dot product a_i . w_j (GRAD1, CONVGRAD1 with DV <= 5): the products
-// w_wp a_ap of part levels wp + ap <= 2 for each weight component k, by level.
+// Four tiles' partial sums t0..t3, each summed over the four lane groups, with lane group h
+// keeping tile h: a reduce-scatter of two permlane32 swaps (lanes 32..63 of the first operand
+// trade places with lanes 0..31 of the second) and one permlane16 swap (odd rows of 16 lanes
+// of the first operand with even rows of the second).  Every value is the sum that
+// sum_lane_groups forms, (v_h + v_h^2) + (v_h^1 + v_h^3), at 3 swaps and 3 adds for four
+// tiles instead of 8 and 8.
+__device__ __forceinline__ float fold4(float t0, float t1, float t2, float t3) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(t0), __float_as_uint(t2), false, false);
+  const float u0 = __uint_as_float(a[0]) + __uint_as_float(a[1]);  // lanes 0-31 tile 0, 32-63 tile 2
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(t1), __float_as_uint(t3), false, false);
+  const float u1 = __uint_as_float(b[0]) + __uint_as_float(b[1]);  // lanes 0-31 tile 1, 32-63 tile 3
+  const auto c = __builtin_amdgcn_permlane16_swap(__float_as_uint(u0), __float_as_uint(u1), false, false);
+  return __uint_as_float(c[0]) + __uint_as_float(c[1]);  // row h of 16 lanes: tile h
+}
+
+// K slots of the weight dot product a_i . w_j (GRAD1, CONVGRAD1): the products w_wp a_ap of
+// part levels wp + ap <= 2 for each weight component k, by level: 6 DV slots, one MFMA for
+// DV <= 5, two chained MFMAs for DV = 6 (slots 32..35 in the second).
 template <int DV>
 __host__ __device__ constexpr BfSlot bf_dot_slot(int s) {
   int i = 0;
@@ -95,6 +113,11 @@ __host__ __device__ constexpr BfSlot bf_dot_slot(int s) {
 __device__ __forceinline__ floatx4 bf_mfma(const bf16x8 &a, const bf16x8 &b) {
   const floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, zero4, 0, 0, 0);
+}
+
+// The same, added to c.
+__device__ __forceinline__ floatx4 bf_mfma_acc(const bf16x8 &a, const bf16x8 &b, const floatx4 &c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // Lane (h, n)'s eight B values of a column, slots 8h .. 8h + 7.
@@ -116,8 +139,10 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   constexpr int NV = D + DV;    // FP32 values per staged source (y, w)
   constexpr int TS = kBfTile;
   constexpr int NTE = (OP == OP_CONV) ? 1 : DV;
-  // the weight dot product a_i . w_j on the matrix cores too (six slots per component)
-  constexpr bool DOT = OP != OP_CONV && 6 * DV <= 32;
+  // the weight dot product a_i . w_j on the matrix cores too (six slots per component, one
+  // MFMA up to DV = 5, two from DV = 6)
+  constexpr bool DOT = OP != OP_CONV;
+  constexpr int NDOT = 6 * DV <= 32 ? 1 : 2;
   constexpr int GROUP = 64 * NT;  // four wavefronts of NT tiles of 16 targets
   static_assert(NT >= 1 && NT <= 4, "a lane group of 16 lanes finishes each tile");
   static_assert(S::EXPANDABLE, "Convolve, Grad1 and the fused sum");
@@ -126,6 +151,8 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   // the 16 lanes of one slot group read 16 consecutive 16-byte rows (no bank conflicts)
   __shared__ __attribute__((aligned(16))) bf16x8 smA[TS * 4];  // distance rows: 4 x 8 BF16
   __shared__ __attribute__((aligned(16))) bf16x8 smW[DOT ? TS * 4 : 4];  // dot-product rows
+  // slots 32..39 of the dot-product rows (DV = 6)
+  __shared__ __attribute__((aligned(16))) bf16x8 smW2[NDOT == 2 ? TS : 1];
   __shared__ __attribute__((aligned(16))) float smV[TS * NV];
   __shared__ double sbox[4][2][3];
 
@@ -172,7 +199,7 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   }
   const double s = p.s;
   if (!(extent * s <= p.exact_extent_bf)) {
-    // a wide group (its sorted targets straddle a jump of the Morton order): each wavefront
+    // a wide group (its sorted targets straddle a fold of the Hilbert curve): each wavefront
     // sums its 64 targets as DFM_FP32C does, on its own centre (exact differences where its
     // own targets are wide), with a slice of smA as its LDS tile.  All four wavefronts pass
     // the same barriers: the same source range, every lane taking part.
@@ -195,9 +222,8 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   // ---- this wavefront's targets: tile t, column n -> position wbase + 16 t + n
   const int h = lane >> 4, n = lane & 15;
   const long wbase = gbase + 16 * NT * wave;
-  float al[DOT ? 1 : NT][NTE];  // a_i in FP32 (VALU dot product only)
   bf16x8 bop[NT];               // distance columns
-  bf16x8 bdot[DOT ? NT : 1];    // dot-product columns
+  bf16x8 bdot[DOT ? NT : 1][NDOT];  // dot-product columns (slots 0..31, 32..63)
 #pragma unroll
   for (int t = 0; t < NT; t++) {
     const long pos = wbase + 16 * t + n;
@@ -221,14 +247,14 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
 #pragma unroll
       for (int k = 0; k < DV; k++) bf_split3(v ? (float)p.ta[o * DV + k] : 0.f, ap[k]);
 #pragma unroll
-      for (int q = 0; q < 32; q++) {
-        const BfSlot sl = bf_dot_slot<DV>(q);
-        col[q] = sl.kind == 2 ? ap[sl.k][sl.xp] : (__bf16)0.0f;
-      }
-      bdot[t] = bf_column(col, h);
-    } else if constexpr (OP != OP_CONV) {
+      for (int m = 0; m < NDOT; m++) {
 #pragma unroll
-      for (int k = 0; k < NTE; k++) al[t][k] = v ? (float)p.ta[o * DV + k] : 0.f;
+        for (int q = 0; q < 32; q++) {
+          const BfSlot sl = bf_dot_slot<DV>(32 * m + q);
+          col[q] = sl.kind == 2 ? ap[sl.k][sl.xp] : (__bf16)0.0f;
+        }
+        bdot[t][m] = bf_column(col, h);
+      }
     }
   }
 
@@ -289,6 +315,15 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
           }
           smW[q4 * TS + tid] = row;
         }
+        if constexpr (NDOT == 2) {
+          bf16x8 row;
+#pragma unroll
+          for (int v8 = 0; v8 < 8; v8++) {
+            const BfSlot sl = bf_dot_slot<DV>(32 + v8);
+            row[v8] = sl.kind == 2 ? wp[sl.k][sl.yp] : (__bf16)0.0f;
+          }
+          smW2[tid] = row;
+        }
       }
       float *dv = smV + tid * NV;
 #pragma unroll
@@ -301,11 +336,10 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     const long left = je - j0;
     const int ngrp = (int)(left < TS ? (left + 15) / 16 : TS / 16);
     float acc[NT][NU];
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int a = 0; a < NU; a++) acc[t][a] = 0.f;
-    for (int g = 0; g < ngrp; g++) {
+    // 16 sources of the tile; the first group (FIRST) sets the partial sums instead of
+    // adding to zero (the same values: an FMA into 0 rounds as the product does)
+    auto group = [&](int g, auto first) {
+      constexpr bool FIRST = decltype(first)::value;
       // r' of sources 16 g + 4 h + v against each tile's column n (and a . w)
       const bf16x8 a = smA[h * TS + 16 * g + n];
       floatx4 r[NT];
@@ -315,7 +349,13 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
       if constexpr (DOT) {
         const bf16x8 aw = smW[h * TS + 16 * g + n];
 #pragma unroll
-        for (int t = 0; t < NT; t++) dt[t] = bf_mfma(aw, bdot[t]);
+        for (int t = 0; t < NT; t++) dt[t] = bf_mfma(aw, bdot[t][0]);
+        if constexpr (NDOT == 2) {
+          // slots 32..39 (every lane reads them; the B columns of lane groups h > 0 are zero)
+          const bf16x8 aw2 = smW2[16 * g + n];
+#pragma unroll
+          for (int t = 0; t < NT; t++) dt[t] = bf_mfma_acc(aw2, bdot[t][1], dt[t]);
+        }
       }
       // FP32 y, w of sources 16 g + 4 h + v, v = 0..3 (4 NV consecutive floats)
       float sv[4][NV];
@@ -331,39 +371,38 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
       for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int v4 = 0; v4 < 4; v4++) {
+          const bool set = FIRST && v4 == 0;
           const float K = Prec<float>::kexp(r[t][v4]);
           const float *y = sv[v4];
           const float *w = sv[v4] + D;
           if constexpr (OP == OP_CONV || OP == OP_CONVGRAD1) {
 #pragma unroll
-            for (int k = 0; k < DV; k++) acc[t][k] += K * w[k];
+            for (int k = 0; k < DV; k++) acc[t][k] = set ? K * w[k] : acc[t][k] + K * w[k];
           }
           if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
             constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
-            float sa;
-            if constexpr (DOT) {
-              sa = dt[t][v4];
-            } else {
-              sa = al[t][0] * w[0];
+            const float tt = K * dt[t][v4];
+            if constexpr (OP == OP_GRAD1) acc[t][D] = set ? tt : acc[t][D] + tt;
 #pragma unroll
-              for (int k = 1; k < DV; k++) sa += al[t][k] * w[k];
-            }
-            const float tt = K * sa;
-            if constexpr (OP == OP_GRAD1) acc[t][D] += tt;
-#pragma unroll
-            for (int l = 0; l < D; l++) acc[t][G + l] += tt * y[l];
+            for (int l = 0; l < D; l++) acc[t][G + l] = set ? tt * y[l] : acc[t][G + l] + tt * y[l];
           }
         }
-    }
+    };
+    group(0, std::true_type{});
+    for (int g = 1; g < ngrp; g++) group(g, std::false_type{});
     // fold: sum each tile's partials over the lane groups (sources 4h..4h+3 of every group);
     // lane (h, n) keeps tile h
 #pragma unroll
     for (int a = 0; a < NU; a++) {
       float mine = 0.f;
+      if constexpr (NT == 4) {
+        mine = fold4(acc[0][a], acc[1][a], acc[2][a], acc[3][a]);
+      } else {
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const float v = sum_lane_groups(acc[t][a]);
-        mine = (t == h) ? v : mine;
+        for (int t = 0; t < NT; t++) {
+          const float v = sum_lane_groups(acc[t][a]);
+          mine = (t == h) ? v : mine;
+        }
       }
       accd[a] += (double)mine;
     }

@@ -16,7 +16,7 @@
 //
 // FP64 path: coordinates scaled by 1/sigma, K = exp(-r^2) in FP64 — the arithmetic of the
 // reference's ExactKernel.
-// FP32C path: targets are ordered along a Morton curve (radix sort of 30-bit keys) so
+// FP32C path: targets are ordered along a Hilbert curve (radix sort of 30-bit keys) so
 // each wavefront's targets are spatially compact; each wavefront re-centres its targets
 // and every staged source on the centre of its targets' bounding box in FP64, scales by
 // sqrt(log2 e)/sigma and only then rounds to FP32, so the FP32 differences carry the

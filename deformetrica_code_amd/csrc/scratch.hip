@@ -9,7 +9,7 @@
 // address with hipGetSymbolAddress and keeps it per device; every launch then uses that
 // address.  The call runs the same plan as dfm_conv_grad1 at the same sizes (heuristic or
 // tuned), so its results are bitwise those of dfm_conv_grad1.  Nothing in the scratch must
-// start at zero: the Morton keys, permutation and split partials a call reads are all
+// start at zero: the Hilbert keys, permutation and split partials a call reads are all
 // written earlier in the same call.  Calls that overlap on one GPU share the array and race.
 #include <hip/hip_runtime.h>
 
