@@ -66,22 +66,22 @@ FORMS = {
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a 24 MB device copy.
 # Counter runs cannot be made from inside this process, so the figures are the profiled ones.
 PROFILED = {
-    # k_pairsum_bf: FETCH_SIZE 6,270,480 KiB x 1.998 (the copy kernel's 24 MB read counts
-    # 11,731 KiB) = 12.83 GB, WRITE_SIZE 785,452 KiB x 1.0 = 0.80 GB (the FP64 split partials)
-    "bf16mfma": {"traffic": 12.83e9 + 0.80e9,
+    # k_pairsum_bf: FETCH_SIZE 7,689,250 KiB x 1.997 (the copy kernel's 24 MB read counts
+    # 11,734 KiB) = 15.73 GB, WRITE_SIZE 796,877 KiB x 1.0 = 0.82 GB (the FP64 split partials)
+    "bf16mfma": {"traffic": 15.73e9 + 0.82e9,
                  "source": "profiles/k_pairsum_bf_convgrad1_N1M_sigma0.05.md",
                  "note": "VALU-issue bound with matrix-core co-issue: two "
                          "v_mfma_f32_16x16x32_bf16 per 256 pairs form the distances' cross "
                          "term and alpha.w, the VALU keeps exp2 and the FP32 accumulation "
-                         "(9.7 VALU instructions per pair against 15.0 for fp32c); 13.6 GB per "
-                         "launch is 54 GB/s, under 1% of HBM bandwidth; the peak is the FP32 "
+                         "(8.7 VALU instructions per pair against 14.7 for fp32c); 16.5 GB per "
+                         "launch is 68 GB/s, under 1% of HBM bandwidth; the peak is the FP32 "
                          "vector peak of the 29 algorithmic flops per pair"},
-    # k_pairsum<float, ..., R=2>: FETCH_SIZE 13,446,624 KiB x 1.998 = 27.51 GB, WRITE_SIZE
+    # k_pairsum<float, ..., R=2>: FETCH_SIZE 2,920,758 KiB x 1.997 = 5.97 GB, WRITE_SIZE
     # 421,875 KiB = 0.43 GB
-    "fp32c": {"traffic": 27.51e9 + 0.43e9,
+    "fp32c": {"traffic": 5.97e9 + 0.43e9,
               "source": "profiles/k_pairsum_convgrad1_N1M_sigma0.05.md",
-              "note": "VALU-issue bound, no matrix-core work (15.0 VALU instructions per pair); "
-                      "28 GB per launch is 89 GB/s, about 1% of HBM bandwidth"},
+              "note": "VALU-issue bound, no matrix-core work (14.7 VALU instructions per pair); "
+                      "6.4 GB per launch is 21 GB/s, under 1% of HBM bandwidth"},
 }
 METRIC = "kernel pair-evals/s (Convolve+Gradient), FP32-comp vs FP64, at ≤1e-6 rel err"
 
