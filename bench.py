@@ -346,18 +346,22 @@ _WORK = {}  # algorithmic work per call of the last _profiled run: pairs, flops
 
 
 def _profiled(L, fn, reps):
-    """Wall seconds per call, pair-sum kernel seconds per call (HIP events) and launches per
-    call; the library's count of algorithmic pairs and flops per call goes to _WORK."""
+    """Wall seconds per call (timed with profiling off: the per-launch HIP events are not
+    part of the work), pair-sum kernel seconds per call (HIP events, a second pass of `reps`
+    calls) and launches per call; the library's count of algorithmic pairs and flops per
+    call goes to _WORK."""
     import ctypes
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    t1 = time.perf_counter()
     L.dfm_profile_read(None, None)
     L.dfm_profile_work(None, None)
     tot, cnt = ctypes.c_double(), ctypes.c_long()
     pairs, flops = ctypes.c_double(), ctypes.c_double()
     L.dfm_profile_enable(1)
-    t0 = time.perf_counter()
     for _ in range(reps):
         fn()
-    t1 = time.perf_counter()
     L.dfm_profile_enable(0)
     L.dfm_profile_read(ctypes.byref(tot), ctypes.byref(cnt))
     L.dfm_profile_work(ctypes.byref(pairs), ctypes.byref(flops))
@@ -417,9 +421,9 @@ def workload_shooting(args):
             df.SetStartPositions(p0)
             df.SetStartMomentas(a0)
             df.Update()
-            wall, kern, launches = _profiled(L, df.Update, 5)
+            wall, kern, launches = _profiled(L, df.Update, 50)
             roof = _roof(kern)
-            twall, tkern, _ = (None, None, None) if args.no_tune else _tuned(L, df.Update, 5)
+            twall, tkern, _ = (None, None, None) if args.no_tune else _tuned(L, df.Update, 50)
             row[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3,
                          "pairsum_launches": launches, "roofline": roof,
                          "autotuned_ms": twall and twall * 1e3,

@@ -59,10 +59,12 @@ def load():
     L.dfm_malloc.argtypes = [ctypes.POINTER(vp), sz]
     L.dfm_free.argtypes = [vp]
     L.dfm_memcpy.argtypes = [vp, vp, sz, i, vp]
+    L.dfm_memcpy_async.argtypes = [vp, vp, sz, i, vp]
     L.dfm_stream_create.argtypes = [ctypes.POINTER(vp)]
     L.dfm_stream_destroy.argtypes = [vp]
     L.dfm_stream_synchronize.argtypes = [vp]
-    for name in ("dfm_device_count", "dfm_malloc", "dfm_free", "dfm_memcpy", "dfm_stream_create",
+    for name in ("dfm_device_count", "dfm_malloc", "dfm_free", "dfm_memcpy", "dfm_memcpy_async",
+                 "dfm_stream_create",
                  "dfm_stream_destroy", "dfm_stream_synchronize"):
         getattr(L, name).restype = i
     # integrators

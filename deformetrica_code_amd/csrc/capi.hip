@@ -101,26 +101,31 @@ int dfm_free(void *ptr) {
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "hipFree: %s", hipGetErrorString(e));
 }
 
-int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+static int memcpy_on(void *dst, const void *src, size_t bytes, int kind, void *stream, bool wait) {
   clear_error();
   if (bytes == 0) return DFM_OK;
   if (!dst || !src) return set_error(DFM_ERR_ARG, "NULL pointer");
-  hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
-                    : kind == 1 ? hipMemcpyDeviceToHost
-                    : kind == 2 ? hipMemcpyDeviceToDevice
-                                : hipMemcpyDefault;
   if (kind < 0 || kind > 2) return set_error(DFM_ERR_ARG, "unknown copy kind %d", kind);
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (kind == 2) {
     int rc = copy_d2d(st, dst, src, bytes);
     if (rc != DFM_OK) return rc;
-    e = hipStreamSynchronize(st);
+    e = wait ? hipStreamSynchronize(st) : hipSuccess;
   } else {
     e = hipMemcpyAsync(dst, src, bytes, k, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && wait) e = hipStreamSynchronize(st);
   }
   return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "memcpy: %s", hipGetErrorString(e));
+}
+
+int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+  return memcpy_on(dst, src, bytes, kind, stream, true);
+}
+
+int dfm_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+  return memcpy_on(dst, src, bytes, kind, stream, false);
 }
 
 int dfm_stream_create(void **stream) {

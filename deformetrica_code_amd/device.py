@@ -43,6 +43,17 @@ class DeviceArray:
         _lib.check(_lib.load().dfm_memcpy(self._ptr, a.ctypes.data_as(ctypes.c_void_p),
                                           self.nbytes, 0, _sp(stream)))
 
+    def upload_async(self, a, stream=None):
+        """Queue the copy of host array `a` on `stream` without waiting (dfm_memcpy_async):
+        `a` must be C-contiguous of this array's dtype and shape, and the caller keeps it
+        alive and unchanged until the stream has passed the copy."""
+        if not (isinstance(a, np.ndarray) and a.dtype == self.dtype and a.shape == self.shape
+                and a.flags.c_contiguous):
+            raise ValueError("upload_async needs a C-contiguous %s array of shape %s"
+                             % (self.dtype, self.shape))
+        _lib.check(_lib.load().dfm_memcpy_async(self._ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                                self.nbytes, 0, _sp(stream)))
+
     def numpy(self, stream=None) -> np.ndarray:
         out = np.empty(self.shape, dtype=self.dtype)
         _lib.check(_lib.load().dfm_memcpy(out.ctypes.data_as(ctypes.c_void_p), self._ptr,

@@ -87,26 +87,31 @@ class HIPDiffeos:
         if self.a0.shape != self.p0.shape:
             raise RuntimeError("start positions and momenta differ in shape")
         st = self._stream
-        # the start positions and momenta in one buffer, one copy
+        # the start positions and momenta in one buffer, one copy queued ahead of the shoot
+        # (the host array lives until the synchronisation below)
         pa0 = self._buf("pa0", (2, n, d))
-        pa0.upload(np.stack([self.p0, self.a0]), st)
-        p0_ptr, a0_ptr = pa0.ptr, pa0.ptr + n * d * 8
-        self._P = self._buf("P", (T, n, d))
-        self._A = self._buf("A", (T, n, d))
-        wp, wb = self._ws.get(L.dfm_shoot_workspace(d, n, self.precision))
-        _lib.check(L.dfm_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme, p0_ptr,
-                               a0_ptr, n, T, self.t0, self.tN, self._P.ptr, self._A.ptr, wp, wb))
-        self._X = None
-        if self.x0 is not None:
-            m = self.x0.shape[0]
-            x0 = self._buf("x0", self.x0.shape)
-            x0.upload(self.x0, st)
-            self._X = self._buf("X", (T, m, d))
-            wp, wb = self._ws.get(L.dfm_flow_workspace(d, n, m, self.precision))
-            _lib.check(L.dfm_flow(st.handle, d, self.kernel_width, self.precision,
-                                  int(self.improved_euler), self._P.ptr, self._A.ptr, n, x0.ptr, m,
-                                  T, self.t0, self.tN, self._X.ptr, wp, wb))
-        st.synchronize()
+        host_pa0 = np.ascontiguousarray(np.stack([self.p0, self.a0]), dtype=np.float64)
+        pa0.upload_async(host_pa0, st)
+        try:
+            p0_ptr, a0_ptr = pa0.ptr, pa0.ptr + n * d * 8
+            self._P = self._buf("P", (T, n, d))
+            self._A = self._buf("A", (T, n, d))
+            wp, wb = self._ws.get(L.dfm_shoot_workspace(d, n, self.precision))
+            _lib.check(L.dfm_shoot(st.handle, d, self.kernel_width, self.precision, self.scheme,
+                                   p0_ptr, a0_ptr, n, T, self.t0, self.tN, self._P.ptr,
+                                   self._A.ptr, wp, wb))
+            self._X = None
+            if self.x0 is not None:
+                m = self.x0.shape[0]
+                x0 = self._buf("x0", self.x0.shape)
+                x0.upload(self.x0, st)
+                self._X = self._buf("X", (T, m, d))
+                wp, wb = self._ws.get(L.dfm_flow_workspace(d, n, m, self.precision))
+                _lib.check(L.dfm_flow(st.handle, d, self.kernel_width, self.precision,
+                                      int(self.improved_euler), self._P.ptr, self._A.ptr, n,
+                                      x0.ptr, m, T, self.t0, self.tN, self._X.ptr, wp, wb))
+        finally:
+            st.synchronize()  # also on an error: the queued copy still reads host_pa0
 
     def GetTrajectoryPositions(self):
         return list(self._P.numpy(self._stream))

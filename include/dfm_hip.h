@@ -77,11 +77,14 @@ const char *dfm_last_error(void);
  * has no HIP runtime of its own (Python through ctypes, the C++ plugin) needs nothing
  * else.  dfm_memcpy is ordered on `stream` and returns after the copy completes.
  * kind: 0 host->device, 1 device->host, 2 device->device (a plain copy kernel, k_copy,
- * for 16-byte-aligned buffers). */
+ * for 16-byte-aligned buffers).  dfm_memcpy_async is the same copy queued on `stream`
+ * without waiting: a host buffer must stay valid and unchanged until the stream has passed
+ * the copy (for example until the caller's next dfm_stream_synchronize). */
 int dfm_device_count(int *count);
 int dfm_malloc(void **ptr, size_t bytes);
 int dfm_free(void *ptr);
 int dfm_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int dfm_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
 int dfm_stream_create(void **stream);
 int dfm_stream_destroy(void *stream);
 int dfm_stream_synchronize(void *stream);

@@ -65,6 +65,13 @@ def test_argument_errors_without_gpu(lib):
     assert need > 0
     assert lib.dfm_conv(None, 3, 3, 1.0, 0, p, 10, p, p, 100000, p, None, 0) == _lib.DFM_ERR_ARG
     assert b"workspace" in lib.dfm_last_error()
+    # copies: unknown kind and NULL pointers are refused before the runtime is touched, in
+    # both the waiting and the queued form; an empty copy is a no-op
+    for copy in (lib.dfm_memcpy, lib.dfm_memcpy_async):
+        assert copy(p, p, 8, 3, None) == _lib.DFM_ERR_ARG
+        assert copy(None, p, 8, 0, None) == _lib.DFM_ERR_ARG
+        assert copy(p, None, 8, 1, None) == _lib.DFM_ERR_ARG
+        assert copy(p, p, 0, 0, None) == _lib.DFM_OK
 
 
 def test_python_mirror_errors():
