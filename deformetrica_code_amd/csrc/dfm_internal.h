@@ -50,11 +50,13 @@ constexpr int kBfTile = 256;
 // 2^24 + 1 - 2^24 gives 1, 2^25 + 1 - 2^25 gives 0) and the groups in FP32, so
 // r' = |y|^2 - 2 x.y rounds at ~2^-24 of |x|^2 for a target at distance |x| from the group
 // centre: a target norm added in the K dimension would not cancel exactly (and
-// exp2(-r') reaches 2^|x|^2: 2^48 at this bound, far from FP32 overflow).  On 1M samples of
-// the unit sphere at sigma 0.02, where a third of the Hilbert-ordered groups span more than 6
-// units, 8 measured 1089 against 1194 ms for the varifold sum at 3.1e-7 against 2.4e-7
-// norm-wise error; 1M uniform points at sigma 0.05 have no group above 6.
-constexpr double kBfExtent = 8.0;
+// exp2(-r') reaches 2^|x|^2: 2^27 at this bound, far from FP32 overflow).  The gradient's
+// x sum(t) - sum(t y) also loses more to cancellation the farther a target sits from the
+// centre.  A bound of 8 ran the varifold sum of 1M samples of the unit sphere at sigma 0.02
+// in 1089 against 1194 ms (a third of its Hilbert-ordered groups span more than 6 units), but
+// the 64-subject atlas gradient's template part then differed from FP64 by 1.28e-6 (7:
+// 1.14e-6, 6: 9.6e-7; FP32C 6.0e-7).  1M uniform points at sigma 0.05 have no group above 6.
+constexpr double kBfExtent = 6.0;
 
 // |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
 constexpr double kZeroMomenta2 = 1e-40;

@@ -314,11 +314,13 @@ static int tile_of(int prec) {
   return prec == DFM_FP64 ? kTileF64 : prec == DFM_BF16MFMA ? kBfTile : kTileF32;
 }
 
-// DFM_BF16MFMA runs the matrix-core kernel only for single Convolve / Grad1 / fused calls
-// large enough to be Hilbert-sorted; everything else runs as DFM_FP32C.
+// DFM_BF16MFMA runs the matrix-core kernel for Convolve / Grad1 / fused sums: single calls
+// large enough to be Hilbert-sorted, and ragged batches (nseg > 1, the caller's target
+// order: groups of 256 targets that span too far take the FP32C code inside the kernel);
+// everything else runs as DFM_FP32C.
 static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx) {
   if (prec != DFM_BF16MFMA) return prec;
-  return (nseg == 1 && nx >= kSortMinTargets && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
+  return ((nseg > 1 || nx >= kSortMinTargets) && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
 }
 
 static int nacc_of(int op, int D, int DV) {
@@ -496,7 +498,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
 
 Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
                      long max_ny) {
-  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Hilbert-sorted
+  prec = effective_prec(op, prec, D, DV, nseg, max_nx);
   Plan best = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny,
                          default_choice(op, prec, D, DV, nseg, max_nx));
   for (const Choice &c : candidates(op, prec, D, DV, nseg, max_nx)) {
@@ -817,7 +819,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
                   const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
                   size_t work_bytes) {
   if (xs.n != ys.n) return set_error(DFM_ERR_ARG, "target and source segment counts differ");
-  if (prec == DFM_BF16MFMA) prec = DFM_FP32C;  // batches are never Hilbert-sorted
+  prec = effective_prec(op, prec, D, DV, xs.n, xs.max);
   p.nx = xs.total;
   p.ny = ys.total;
   p.xoff = xs.n > 1 ? xs.d : nullptr;

@@ -24,9 +24,11 @@
 // targets and one LDS tile of 256 staged sources (A rows 64 B, FP32 y and w).
 //
 // A group whose targets span more than kBfExtent scaled units (none of the Hilbert-ordered
-// groups of 1M uniform points at sigma 0.05; 12% of those of 1M samples of a sphere at
+// groups of 1M uniform points at sigma 0.05; 34% of those of 1M samples of a sphere at
 // sigma 0.02) is summed in the same workgroup by the DFM_FP32C wavefront code
-// (sum_wavefront), one wavefront per 64 targets.  Same accumulator
+// (sum_wavefront), one wavefront per 64 targets.  Ragged batches (grid.z = segment) keep
+// the caller's target order: the groups of a triangle mesh's faces in mesh order are as
+// compact as Hilbert-ordered ones.  Same accumulator
 // layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
 //
 // The reference arithmetic this replaces: the per-pair squared distance of
@@ -159,9 +161,19 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int split = blockIdx.y;
-  const long gbase = (long)blockIdx.x * GROUP;
-  const long gend = std::min(p.nx, gbase + GROUP);
-  if (gbase >= p.nx) return;  // the whole workgroup
+  // segment of a ragged batch (blockIdx.z; one segment: the whole target and source sets)
+  long xb = 0, xe = p.nx, yb = 0, ye = p.ny;
+  if (p.xoff) {
+    xb = p.xoff[blockIdx.z];
+    xe = p.xoff[blockIdx.z + 1];
+  }
+  if (p.yoff) {
+    yb = p.yoff[blockIdx.z];
+    ye = p.yoff[blockIdx.z + 1];
+  }
+  const long gbase = xb + (long)blockIdx.x * GROUP;
+  const long gend = std::min(xe, gbase + GROUP);
+  if (gbase >= xe) return;  // the whole workgroup lies past its segment
 
   // ---- the group's bounding box (FP64, exact min / max) and centre
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -210,8 +222,8 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     double wxd[1][D], wc[D], wext;
     load_targets<float, D, 1>(p, lane, gbase + 64 * wave, gend, wpos, worig, wvalid, wxd, wc,
                               wext);
-    const long wjs = (long)split * p.split_len;
-    const long wje = std::min(p.ny, wjs + p.split_len);
+    const long wjs = yb + (long)split * p.split_len;
+    const long wje = std::min(ye, wjs + p.split_len);
     float *wsm = reinterpret_cast<float *>(smA) + wave * (kTileF32 * NREC);
     // (a fused integrator step never runs here: it is applied by k_step_update)
     sum_wavefront<float, OP, D, DV, 0, 1, MODE_ADAPTIVE>(p, wsm, lane, split, wjs, wje, wpos, worig,
@@ -264,8 +276,8 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
 #pragma unroll
   for (int a = 0; a < NA; a++) accd[a] = 0.0;
 
-  const long js = (long)split * p.split_len;
-  const long je = std::min(p.ny, js + p.split_len);
+  const long js = yb + (long)split * p.split_len;
+  const long je = std::min(ye, js + p.split_len);
   for (long j0 = js; j0 < je; j0 += TS) {
     __syncthreads();
     // ---- stage source j0 + tid: A rows (BF16 splits in slot order) and FP32 y, w

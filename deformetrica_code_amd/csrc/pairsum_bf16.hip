@@ -1,6 +1,7 @@
 // pairsum_bf16.hip — launches of the DFM_BF16MFMA pair sums: the matrix-core kernel
-// (k_pairsum_bf; its wide groups of 256 sorted targets take the FP32C wavefront path in the
-// same launch), then the split reduction.
+// (k_pairsum_bf; its wide groups of 256 targets take the FP32C wavefront path in the same
+// launch), then the split reduction.  Single calls (Hilbert-sorted targets) and ragged
+// batches (grid.z = segment, the caller's target order) alike.
 #include "pairsum_bf16.h"
 
 namespace dfm {
@@ -8,7 +9,7 @@ namespace {
 
 template <int OP, int D, int DV>
 void launch_bf(hipStream_t st, const KParams &p, const Plan &pl) {
-  const dim3 gbf((unsigned)((p.nx + kBfGroup - 1) / kBfGroup), (unsigned)pl.nsplit, 1);
+  const dim3 gbf((unsigned)pl.chunks, (unsigned)pl.nsplit, (unsigned)pl.nseg);
   hipLaunchKernelGGL((k_pairsum_bf<OP, D, DV, kBfGroup / 64>), gbf, dim3(256), 0, st, p);
   if (pl.nsplit > 1)
     hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
@@ -43,8 +44,9 @@ bool bf_supported(int op, int D, int DV) {
 }
 
 int launch_op_bf16(hipStream_t st, int op, int D, int DV, const KParams &p, const Plan &pl) {
-  if (p.xoff || p.yoff || !bf_supported(op, D, DV))
-    return set_error(DFM_ERR_ARG, "the BF16 cross-term kernel takes single Convolve / Grad1 calls");
+  if (!bf_supported(op, D, DV) || pl.R != kBfGroup / 64)
+    return set_error(DFM_ERR_ARG, "the BF16 cross-term kernel takes Convolve / Grad1 / fused "
+                                  "sums on plans of %d-target groups", kBfGroup);
   switch (op) {
     case OP_CONV: return launch_bf_d<OP_CONV>(st, D, DV, p, pl);
     case OP_GRAD1: return launch_bf_d<OP_GRAD1>(st, D, DV, p, pl);

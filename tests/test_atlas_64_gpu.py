@@ -13,7 +13,9 @@ the library shoots and back-propagates every subject in one batch.  Checks:
     one-subject control-point gradients against the batch's to 1e-12;
   * HIP-graph replay of the batched gradient bit-equal to the eager call (FP32C);
   * the FP32C batch against the FP64 batch: norm-wise <= 1e-6 for the momenta, the
-    control points and the template gradient.
+    control points and the template gradient;
+  * the same for DFM_BF16MFMA (the batch's Convolve / Grad1 / fused sums with the
+    distances' cross term on the matrix cores, groups of 256 targets in mesh order).
 """
 import numpy as np
 import pytest
@@ -55,7 +57,7 @@ def _atlas(V, F, cp, targets, prec):
 def batched(problem):
     V, F, targets, cp, mom = problem
     out = {}
-    for prec in ("fp64", "fp32c"):
+    for prec in ("fp64", "fp32c", "bf16mfma"):
         print("batched atlas, 64 subjects:", prec, flush=True)
         A = _atlas(V, F, cp, targets, prec)
         ll, gcp, gmom, gtpl = A.ComputeCompleteLogLikelihoodGradient(cp, mom, V)
@@ -98,9 +100,10 @@ def test_graph_replay_equals_eager(problem, batched):
             np.testing.assert_array_equal(e, r, err_msg="replay %d" % rep)
 
 
-def test_fp32c_batch_matches_fp64(batched):
+@pytest.mark.parametrize("prec", ["fp32c", "bf16mfma"])
+def test_fp32_batch_matches_fp64(batched, prec):
     _, ll64, gcp64, gmom64, gtpl64, _ = batched["fp64"]
-    _, ll32, gcp32, gmom32, gtpl32, _ = batched["fp32c"]
+    _, ll32, gcp32, gmom32, gtpl32, _ = batched[prec]
     errs = {"momenta": nrel(gmom32, gmom64), "control_points": nrel(gcp32, gcp64),
             "template": nrel(gtpl32, gtpl64),
             "loglik": float(abs(np.sum(ll32) - np.sum(ll64)) / abs(np.sum(ll64)))}

@@ -19,7 +19,7 @@ def _offsets(counts):
     return off
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32c", "fp32x"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32c", "fp32x", "bf16mfma"])
 def test_batch_conv_grad1_matches_per_segment_calls(prec):
     from deformetrica_code_amd import device
     rng = np.random.default_rng(1)
@@ -49,6 +49,53 @@ def test_batch_conv_grad1_matches_per_segment_calls(prec):
         tol = 1e-12 if prec == "fp64" else 1e-6
         assert rel(conv[xs], c1.numpy()) < tol, s
         assert rel(grad[xs], g1.numpy()) < tol, s
+
+
+def _zorder(x, bits=10):
+    """Z-order keys of points in [0, 1]^3 (bit-interleaved cell indices)."""
+    q = np.clip((x * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    key = np.zeros(len(x), dtype=np.int64)
+    for b in range(bits):
+        for k in range(3):
+            key |= ((q[:, k] >> b) & 1) << (3 * b + k)
+    return key
+
+
+@pytest.mark.parametrize("op", ["conv", "grad1", "conv_grad1"])
+def test_bf16_batch_on_compact_segments_matches_fp64(op):
+    """DFM_BF16MFMA batches in the caller's order: segments whose targets are in Z (Morton)
+    order (most groups of 256 compact: the matrix-core path) and one in random order (every
+    group wide: the FP32C code inside the same kernel), Dv = 3 and 6, against the FP64 batch
+    to 1e-6 norm-wise per segment."""
+    from deformetrica_code_amd import device
+    rng = np.random.default_rng(5)
+    nx = [9000, 300, 0, 2500, 257]
+    ny = [7000, 4000, 100, 0, 3000]
+    xo, yo = _offsets(nx), _offsets(ny)
+    X = rng.uniform(size=(xo[-1], 3))
+    for s in (0, 3, 4):
+        seg = X[xo[s]:xo[s + 1]]
+        X[xo[s]:xo[s + 1]] = seg[np.argsort(_zorder(seg), kind="stable")]
+    Y = rng.uniform(size=(yo[-1], 3))
+    sigma = 0.1
+    for dv in (3, 6):
+        W = rng.normal(size=(yo[-1], dv))
+        A = rng.normal(size=(xo[-1], dv))
+        dX, dA, dY, dW = (device.DeviceArray.from_numpy(v) for v in (X, A, Y, W))
+        out = {}
+        for prec in ("fp64", "bf16mfma"):
+            c, g = device.batch_conv_grad1(dX, dA, dY, dW, xo, yo, sigma, prec,
+                                           conv=op != "grad1", grad=op != "conv")
+            out[prec] = tuple(v.numpy() for v in (c, g) if v is not None)
+        for s in range(len(nx)):
+            xs = slice(xo[s], xo[s + 1])
+            if nx[s] == 0:
+                continue
+            for a, b in zip(out["bf16mfma"], out["fp64"]):
+                if ny[s] == 0:
+                    np.testing.assert_array_equal(a[xs], 0.0)
+                else:
+                    assert rel(a[xs], b[xs]) < 1e-6, (op, dv, s, rel(a[xs], b[xs]))
 
 
 def _subjects(seed, counts_cp, counts_lm, d=3, scale=0.2):

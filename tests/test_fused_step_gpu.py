@@ -77,7 +77,7 @@ def test_fused_step_equals_separate_launches_fp64(scheme, n):
 @pytest.mark.parametrize("prec,n,tol", [("fp32c", 3000, 1e-6), ("fp32c", 12000, 1e-6),
                                         ("bf16mfma", 12000, 1e-6), ("fp32x", 3000, 1e-6)])
 def test_fused_step_fp32_forms_against_oracle(oracle_mod, prec, n, tol):
-    """FP32 shoots (12,000 points: Morton-sorted targets, and for DFM_BF16MFMA the
+    """FP32 shoots (12,000 points: Hilbert-sorted targets, and for DFM_BF16MFMA the
     matrix-core kernel with the update in a kernel of its own) against the FP64 oracle."""
     from oracle import deformations as dfo
     rng = np.random.default_rng(n + 1)

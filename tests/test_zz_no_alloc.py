@@ -37,7 +37,7 @@ def test_noalloc_equals_workspace_entry_point(prec_name):
     L = _lib.load()
     prec = _lib.precision_code(prec_name)
     rng = np.random.default_rng(7)
-    # first call: 20,000 x 30,000 (Morton-sorted FP32 targets, split partials in the
+    # first call: 20,000 x 30,000 (Hilbert-sorted FP32 targets, split partials in the
     # scratch); second call smaller: 3,000 x 2,500
     for n, m in ((20000, 30000), (3000, 2500)):
         print("noalloc case", prec_name, n, m, flush=True)
