@@ -22,7 +22,7 @@ ADJOINTS = {"discrete": ADJOINT_DISCRETE, "continuous": ADJOINT_CONTINUOUS}
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X, "bf16mfma": BF16MFMA}
 
 # DFM_ABI_VERSION of include/dfm_hip.h, the header these bindings follow
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class DfmError(RuntimeError):

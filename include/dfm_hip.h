@@ -65,8 +65,9 @@ enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2, DFM_BF16MFMA = 
  *   1  first release
  *   2  dfm_atlas_desc gained dimension and cell_corners (appended); new entry points for
  *      polylines, volume meshes, point clouds, landmarks, ConvolveHessian and the
- *      adjoint's jump conditions; DFM_OP_HESSIAN */
-#define DFM_ABI_VERSION 2
+ *      adjoint's jump conditions; DFM_OP_HESSIAN
+ *   3  dfm_memcpy_async; DFM_BF16MFMA also serves ragged batches */
+#define DFM_ABI_VERSION 3
 int dfm_abi_version(void);
 /* Last error message of the calling thread ("" when none). */
 const char *dfm_last_error(void);
