@@ -53,6 +53,8 @@ def _kernel(d, sigma, prec, Y, W):
     (30000, 20000, 0.08, 2, 3),
     (10000, 10000, 0.3, 3, 1),
     (8192, 40000, 0.02, 3, 3),
+    (12000, 15000, 0.1, 3, 5),   # five weights: the dot product fills 30 of one MFMA's 32 slots
+    (9000, 9000, 0.15, 2, 6),    # 2D, six weights: the dot product's second, chained MFMA
 ])
 def test_bf16mfma_vs_oracle(oracle_mod, n, m, sigma, d, dv):
     """Convolve, ConvolveGradient(X, alpha) and the fused sum on 400 sampled target rows
