@@ -469,7 +469,7 @@ def workload_registration(args):
         run()
         wall, kern, launches = _profiled(L, run, 3 if prec == "fp32c" else 1)
         roof = _roof(kern)
-        tuned, graph_ms = (None, None), None
+        tuned, graph_ms, table = (None, None), None, None
         if prec == "fp32c" and not args.no_tune:
             # explore plans, time the tuned eager gradient, then capture it and replay
             L.dfm_autotune_reset()
@@ -478,6 +478,7 @@ def workload_registration(args):
                 run()
             L.dfm_autotune(2)
             tuned = _profiled(L, run, 3)[:2]
+            table = _lib.autotune_report().splitlines()
             g = A.capture(dcp, dmom, dV)
             g.launch()
             A.stream.synchronize()
@@ -489,7 +490,7 @@ def workload_registration(args):
             L.dfm_autotune(0)
             L.dfm_autotune_reset()
         out[prec] = {"ms": wall * 1e3, "pairsum_kernel_ms": kern * 1e3, "pairsum_launches": launches,
-                     "roofline": roof, "graph_ms": graph_ms,
+                     "roofline": roof, "graph_ms": graph_ms, "autotune_table": table,
                      "autotuned_ms": tuned[0] and tuned[0] * 1e3,
                      "grads": [A.grad_momenta.numpy(A.stream), A.grad_cp.numpy(A.stream),
                                A.grad_template.numpy(A.stream)],
@@ -525,7 +526,7 @@ def workload_registration(args):
                  dtype="f32", pairsum_kernel_ms=r["pairsum_kernel_ms"],
                  pairsum_launches=r["pairsum_launches"], roofline=r["roofline"],
                  fp64_roofline=out["fp64"]["roofline"], autotuned_ms=r["autotuned_ms"],
-                 autotuned_graph_ms=r["graph_ms"],
+                 autotuned_graph_ms=r["graph_ms"], autotune_table=r["autotune_table"],
                  fp64_ms=out["fp64"]["ms"],
                  fp32c_gradient_rel_err_vs_fp64=errs,
                  fp64_gradient_vs_central_differences=fd_err)

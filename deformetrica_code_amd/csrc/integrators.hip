@@ -284,7 +284,8 @@ int fused_step_sum(const Ctx &cx, const double *p, const double *a, const Seg &c
   q.e_p1 = p1; q.e_a1 = a1; q.e_pb = pb; q.e_ab = ab;
   q.e_px = px; q.e_ax = ax; q.e_c = c; q.e_cx = cxx;
   q.e_gate = gate;
-  // The step's own sums finish their splits in the kernel.  Other sums keep the separate
+  // The step's own sums may finish their splits in the kernel (on grids of few target
+  // tiles, where it saves launches; launch_timed decides).  Other sums keep the separate
   // reduction: the device-scope fence before each wavefront's arrival writes back the L2,
   // which on the large grids of the flow and the adjoint cost more than the launch it saves
   // (registration gradient 25.3 against 12.7 ms, same box).
@@ -293,8 +294,8 @@ int fused_step_sum(const Ctx &cx, const double *p, const double *a, const Seg &c
   return ps(cx, OP_CONVGRAD1, q, cs, cs, nd, nd);
 }
 
-// (p, a) -> (p1, a1) by one Euler or Heun step: one launch per pair sum, the split
-// reduction and the update applied by its last split.
+// (p, a) -> (p1, a1) by one Euler or Heun step: one pair sum per field evaluation, the
+// update applied in its kernel on small grids, else after its split reduction.
 int shoot_step(const Ctx &cx, int scheme, const double *p, const double *a, const Seg &cs,
                double dt, double *p1, double *a1, const ShootTmp &t, const double *gate) {
   if (scheme == DFM_EULER)

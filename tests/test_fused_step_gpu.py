@@ -1,9 +1,10 @@
 """Shooting in few launches.  Up to 4,096 control points a single-subject shoot is one
 cooperative launch for all its steps (k_shoot_persistent, a grid barrier between steps);
-otherwise each step is one launch: the pair sum over the control points finishes its own
-source splits (the last split of each target tile adds up the partials) and applies the
-Euler / Heun update of Diffeos::Shoot (Diffeos.cxx:617-632; the Heun variant of
-Diffeos.cxx:634-652) in the same kernel.  Checked against the step taken with separate
+otherwise each step's pair sum over the control points applies the Euler / Heun update of
+Diffeos::Shoot (Diffeos.cxx:617-632; the Heun variant of Diffeos.cxx:634-652): on grids of
+fewer than 64 target tiles in the same kernel, which also finishes its own source splits
+(the last split of each target tile adds up the partials), on larger grids in a launch of
+its own after the split reduction.  Checked against the step taken with separate
 launches (device conv_grad1, then the update on the host), against the oracle, the two
 forms against each other, for single subjects and ragged batches, with and without source
 splits, with zero momenta (the reference leaves the points in place,
@@ -63,8 +64,8 @@ def _stepwise(p0, a0, width, T, prec, scheme):
 @pytest.mark.parametrize("n", [100, 3000, 4096, 4097, 9000])
 def test_fused_step_equals_separate_launches_fp64(scheme, n):
     """n = 100, 3000, 4096: one launch for the whole shoot (1 to 4 LDS chunks of sources);
-    4097, 9000: one launch per step with source splits (the last split finishes each
-    tile)."""
+    4097, 9000: per-step launches with source splits on grids of 65 and more target tiles
+    (split reduction and update after the pair sum)."""
     rng = np.random.default_rng(n)
     p0 = rng.uniform(size=(n, 3))
     a0 = rng.normal(scale=0.01, size=(n, 3))
@@ -122,12 +123,15 @@ def test_one_launch_shoot_equals_per_step_launches(prec, tol):
         assert rel(A, Ab) < tol, scheme
 
 
-def test_batched_fused_step_equals_single_subjects():
+@pytest.mark.parametrize("sizes", [[700, 1, 2600, 64, 65], [500, 1, 300, 64, 65]])
+def test_batched_fused_step_equals_single_subjects(sizes):
     """Ragged batch of subjects (the atlas's shooting): each subject's trajectory equals its
-    own single-subject shoot, one of them with zero momenta (gated per segment)."""
+    own single-subject shoot, one of them with zero momenta (gated per segment).  The first
+    batch's grid (41 x 5 target tiles) takes the split reduction and the update after the
+    pair sum; the second's (8 x 5, split sources) finishes its splits and the step in the
+    pair-sum kernel."""
     from deformetrica_code_amd.diffeos import HIPBatchDiffeos
     rng = np.random.default_rng(8)
-    sizes = [700, 1, 2600, 64, 65]
     p0s = [rng.uniform(size=(s, 3)) for s in sizes]
     a0s = [rng.normal(scale=0.02, size=(s, 3)) for s in sizes]
     a0s[3] = np.zeros((64, 3))
