@@ -369,7 +369,18 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
   // targets per lane and 65536 wavefronts against 354 ms with the one-target plan below
   // (same box, every candidate timed by the autotuner).
   const bool expandable = op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1;
-  if (prec != DFM_FP64 && expandable && nseg == 1 && blocks >= 4096) return Choice{2, 65536, 4};
+  // Grad1 and the fused sum with more than 3 weights keep one target per lane (the second
+  // target's accumulators cost occupancy): the 1M-sample varifold's fused Dv = 6 sum runs
+  // 446 ms with R = 1 and 262,144 wavefronts against 481 ms with R = 2; the 64-subject
+  // atlas's batched one 11.8 ms with R = 1 and 65,536 wavefronts against 15.9 with R = 2.
+  const bool two = expandable && (op == OP_CONV || DV <= 3);
+  if (prec != DFM_FP64 && expandable && nseg == 1 && blocks >= 4096)
+    return two ? Choice{2, 65536, 4} : Choice{1, 262144, 4};
+  // ragged batches on grids of >= 1,024 target workgroups (the atlas's flow, adjoint and
+  // data-term sums): the 64-subject atlas's batched Convolve over the template vertices
+  // runs 0.431 ms with R = 2 and 16,384 wavefronts against 0.525 ms with the plan below
+  if (prec != DFM_FP64 && expandable && nseg > 1 && blocks >= 1024)
+    return two ? Choice{2, 16384, 4} : Choice{1, 65536, 4};
   // at 1,000 x 1,000 (16 target workgroups) the fused FP32C sum takes 0.017 ms with
   // one-tile splits against 0.030 ms with four-tile ones (FP64: 0.021 against 0.053 ms);
   // from 64 workgroups up (a 4,096-point registration's sums) one-tile FP32 splits cost
