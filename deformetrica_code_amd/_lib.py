@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(PKG, "libdfm_hip.so")
 
 DFM_OK, DFM_ERR_ARG, DFM_ERR_MISMATCH, DFM_ERR_DIM, DFM_ERR_INDEX, DFM_ERR_HIP, DFM_ERR_NOMEM = range(7)
 FP64, FP32C, FP32X, BF16MFMA = 0, 1, 2, 3
-OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP, OP_HESSIAN = range(8)
+(OP_CONV, OP_GRAD1, OP_CONV_GRAD1, OP_VARLIN, OP_GRADDIM, OP_GRADDIFF, OP_SHOOT_VJP, OP_HESSIAN,
+ OP_VARIFOLD, OP_VARIFOLD_CONV) = range(10)
 EULER, HEUN = 0, 1
 CURRENT, VARIFOLD = 0, 1
 ADJOINT_DISCRETE, ADJOINT_CONTINUOUS = 0, 1
@@ -22,7 +23,7 @@ ADJOINTS = {"discrete": ADJOINT_DISCRETE, "continuous": ADJOINT_CONTINUOUS}
 PRECISIONS = {"fp64": FP64, "fp32c": FP32C, "fp32x": FP32X, "bf16mfma": BF16MFMA}
 
 # DFM_ABI_VERSION of include/dfm_hip.h, the header these bindings follow
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class DfmError(RuntimeError):

@@ -166,8 +166,10 @@ struct dfm_atlas {
     L.adj = base ? (void *)(base + off) : nullptr;
     off += a256(L.adj_bytes);
     size_t pw = 0;
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, dv, face_seg, face_seg));
-    pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, dv, face_seg, tgt_seg));
+    pw = std::max(pw, pairsum_batch_workspace(data_grad_op(c.kind), c.precision, D, dv, face_seg,
+                                              face_seg));
+    pw = std::max(pw, pairsum_batch_workspace(data_grad_op(c.kind), c.precision, D, dv, face_seg,
+                                              tgt_seg));
     pw = std::max(pw, pairsum_batch_workspace(OP_CONVGRAD1, c.precision, D, D, cp_seg, cp_seg));
     pw = std::max(pw, pairsum_batch_workspace(OP_CONV, c.precision, D, D, Seg::single(m),
                                               Seg::single(m)));
@@ -232,7 +234,7 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   const int D = A->D, K = A->corners;
   // weights per cell: the normal or tangent (current), the upper triangle of its square
   // (varifold)
-  A->dv = desc->kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  A->dv = D;  // normals (current) or unit-root normals u (varifold, OP_VARIFOLD)
   A->S = desc->nsubjects;
   A->n = desc->n_control_points;
   A->m = desc->n_template_vertices;
@@ -294,7 +296,8 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
   int *tfd = nullptr;
   void *pw = nullptr;
   const size_t pwb =
-      pairsum_batch_workspace(OP_CONV, desc->precision, D, dv, A->tgt_seg, A->tgt_seg);
+      pairsum_batch_workspace(data_conv_op(desc->kind), desc->precision, D, dv, A->tgt_seg,
+                              A->tgt_seg);
   struct Tmp {
     std::vector<void *> v;
     ~Tmp() {
@@ -320,9 +323,9 @@ int dfm_atlas_create(const dfm_atlas_desc *desc, dfm_atlas **out) {
                            bad)))
     return rc;
   KParams p = blank();
-  p.x = A->cT; p.y = A->cT; p.w = A->wT; p.out0 = kT;
-  if ((rc = pairsum_batch(st, OP_CONV, desc->precision, D, dv, desc->attachment_width, p,
-                          A->tgt_seg, A->tgt_seg, ntf * dv, 0, pw, pwb)))
+  p.x = A->cT; p.ta = A->aT; p.y = A->cT; p.w = A->wT; p.out0 = kT;
+  if ((rc = pairsum_batch(st, data_conv_op(desc->kind), desc->precision, D, dv,
+                          desc->attachment_width, p, A->tgt_seg, A->tgt_seg, ntf * dv, 0, pw, pwb)))
     return rc;
   if ((rc = seg_dot(st, kT, A->aT, dv, S, 0, A->d_tgt_off, A->T2))) return rc;
   hipError_t e = hipDeviceSynchronize();
@@ -376,12 +379,12 @@ int dfm_atlas_gradient(dfm_atlas *A, void *stream, const double *cp, const doubl
     return rc;
   KParams p = blank();
   p.x = L.cS; p.ta = L.aS; p.y = L.cS; p.w = L.wS; p.out0 = L.k1; p.out1 = L.g1;
-  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, D, dv, c.attachment_width, p, A->face_seg,
-                          A->face_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
+  if ((rc = pairsum_batch(st, data_grad_op(c.kind), prec, D, dv, c.attachment_width, p,
+                          A->face_seg, A->face_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
     return rc;
   p.y = A->cT; p.w = A->wT; p.out0 = L.k2; p.out1 = L.g2;
-  if ((rc = pairsum_batch(st, OP_CONVGRAD1, prec, D, dv, c.attachment_width, p, A->face_seg,
-                          A->tgt_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
+  if ((rc = pairsum_batch(st, data_grad_op(c.kind), prec, D, dv, c.attachment_width, p,
+                          A->face_seg, A->tgt_seg, SF * dv, SF * D, L.pw, L.pw_bytes)))
     return rc;
   if ((rc = seg_dot(st, L.k1, L.aS, dv, S, F, nullptr, L.S2))) return rc;
   if ((rc = seg_dot(st, L.k2, L.aS, dv, S, F, nullptr, L.ST))) return rc;

@@ -223,7 +223,8 @@ double dfm_flops_per_pair(int op, int dimpoint, int dimvect) {
 
 size_t dfm_workspace_size(int op, int dimpoint, int dimvect, long nx, long ny, int precision) {
   if (check_common(dimpoint, dimvect, precision, 1.0, nx, ny) != DFM_OK) return 0;
-  if (op < DFM_OP_CONV || op > DFM_OP_HESSIAN) return 0;
+  if (op < DFM_OP_CONV || op > DFM_OP_VARIFOLD_CONV) return 0;
+  if ((op == DFM_OP_VARIFOLD || op == DFM_OP_VARIFOLD_CONV) && dimvect != dimpoint) return 0;
   if (op == DFM_OP_GRADDIFF || op == DFM_OP_SHOOT_VJP) ny = nx;
   if (nx == 0 || ny == 0) return 0;
   return make_plan(op, precision, dimpoint, dimvect, nx, ny).work_bytes;

@@ -18,8 +18,20 @@ enum : int {
   OP_GRADDIM = 4,
   OP_GRADDIFF = 5,
   OP_SHOOTVJP = 6,
-  OP_HESS = 7
+  OP_HESS = 7,
+  OP_VARIFOLD = 8,
+  OP_VARIFOLD_CONV = 9  // the Convolve part of OP_VARIFOLD alone
 };
+
+// The pair-sum ops of a current or varifold data term (dimvect = dimpoint): the current's
+// Convolve and fused Convolve + Grad1 over the normals N; the varifold's sums over the
+// unit-root normals u = N / sqrt|N| (OP_VARIFOLD_CONV and OP_VARIFOLD).  The reference's varifold
+// convolves the upper triangle W_j of u_j u_j^T (Dv = D (D + 1) / 2) and contracts its
+// varlin list with u_i u_i^T (NonOrientedSurfaceMesh.cxx:169-260, NonOrientedPolyLine.cxx:
+// 203-251); every use of the result is S((K W)(c_i)) u_i = sum_j K_ij (u_i . u_j) u_j, the
+// Convolve part of OP_VARIFOLD, and the gradient is Grad1 with weights (u_i . u_j)^2.
+inline int data_conv_op(int kind) { return kind == DFM_CURRENT ? OP_CONV : OP_VARIFOLD_CONV; }
+inline int data_grad_op(int kind) { return kind == DFM_CURRENT ? OP_CONVGRAD1 : OP_VARIFOLD; }
 
 constexpr int kBBoxParts = 256;
 
@@ -57,6 +69,11 @@ constexpr int kBfTile = 256;
 // the 64-subject atlas gradient's template part then differed from FP64 by 1.28e-6 (7:
 // 1.14e-6, 6: 9.6e-7; FP32C 6.0e-7).  1M uniform points at sigma 0.05 have no group above 6.
 constexpr double kBfExtent = 6.0;
+// Ragged batches: the 64-subject atlas's varifold gradient sums (OP_VARIFOLD, weights
+// K (u_i . u_j)^2) put its template gradient 1.08e-6 from FP64 at 6 and 7.7e-7 at 5 (FP32C
+// 7.5e-7; eager gradient 82.8 against 79.2 ms), while the single 1M-sample varifold call
+// runs 953 ms at 6 against 1076 ms at 5 with errors of 2.5e-7 and 2.0e-7.
+constexpr double kBfExtentBatch = 5.0;
 
 // |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
 constexpr double kZeroMomenta2 = 1e-40;

@@ -107,6 +107,8 @@ double flops_per_pair(int op, int D, int DV) {
     case OP_HESS: return k + D * (D + 1) / 2.0 + DV * (2.0 + D * (D + 1));
     case OP_GRADDIFF: return k + 2.0 * DV - 1.0 + 3.0 * D - 1.0 + 2.0 + 4.0 * D;
     case OP_SHOOTVJP: return k + 19.0 * D + 1.0;
+    case OP_VARIFOLD: return k + conv + grad + 1.0;  // the fused sum, Convolve weight K (u.u)
+    case OP_VARIFOLD_CONV: return k + conv + 2.0 * DV;  // its Convolve part: u.u, K (u.u)
   }
   return k;
 }
@@ -328,6 +330,8 @@ static int nacc_of(int op, int D, int DV) {
     case OP_CONV: return DV;
     case OP_GRAD1: return D;
     case OP_CONVGRAD1: return DV + D;
+    case OP_VARIFOLD: return DV + D;
+    case OP_VARIFOLD_CONV: return DV;
     case OP_VARLIN: return DV * D;
     case OP_GRADDIM: return DV;
     case OP_GRADDIFF: return 2 * D;
@@ -368,12 +372,14 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
   // the one-target plan.  At N = M = 1M the fused FP32C sum runs in 318 ms with two
   // targets per lane and 65536 wavefronts against 354 ms with the one-target plan below
   // (same box, every candidate timed by the autotuner).
-  const bool expandable = op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1;
+  const bool expandable =
+      op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1 || op == OP_VARIFOLD ||
+      op == OP_VARIFOLD_CONV;
   // Grad1 and the fused sum with more than 3 weights keep one target per lane (the second
   // target's accumulators cost occupancy): the 1M-sample varifold's fused Dv = 6 sum runs
   // 446 ms with R = 1 and 262,144 wavefronts against 481 ms with R = 2; the 64-subject
   // atlas's batched one 11.8 ms with R = 1 and 65,536 wavefronts against 15.9 with R = 2.
-  const bool two = expandable && (op == OP_CONV || DV <= 3);
+  const bool two = expandable && (op == OP_CONV || op == OP_VARIFOLD_CONV || DV <= 3);
   if (prec != DFM_FP64 && expandable && nseg == 1 && blocks >= 4096)
     return two ? Choice{2, 65536, 4} : Choice{1, 262144, 4};
   // ragged batches on grids of >= 1,024 target workgroups (the atlas's flow, adjoint and
@@ -761,7 +767,7 @@ static int launch_timed(hipStream_t st, int op, int prec, int D, int DV, KParams
   // x' = x sqrt(log2 e) / sigma, so one v_exp_f32 evaluates the kernel.
   p.s = prec == DFM_FP64 ? 1.0 / sigma : std::sqrt(1.4426950408889634074) / sigma;
   p.exact_extent = kExactExtent;
-  p.exact_extent_bf = kBfExtent;
+  p.exact_extent_bf = pl.nseg > 1 ? kBfExtentBatch : kBfExtent;
   const double s2 = sigma * sigma;
   p.cg = -2.0 / (s2 * p.s);
   p.ch1 = 2.0 / s2;

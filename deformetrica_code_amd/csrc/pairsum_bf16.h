@@ -1,5 +1,6 @@
-// pairsum_bf16.h — Convolve / ConvolveGradient(X, alpha) / the fused sum with the
-// coordinate cross term of the distances on the matrix cores (precision DFM_BF16MFMA).
+// pairsum_bf16.h — Convolve / ConvolveGradient(X, alpha) / the fused sum / the varifold's
+// sums with the coordinate cross term of the distances on the matrix cores (precision
+// DFM_BF16MFMA).
 //
 // For a group of 256 spatially sorted targets the workgroup re-centres targets and sources
 // on the group's bounding-box centre (FP64), scales by sqrt(log2 e) / sigma and rounds to
@@ -23,13 +24,13 @@
 // A wavefront holds 4 tiles of 16 targets; four wavefronts share the workgroup's 256
 // targets and one LDS tile of 256 staged sources (A rows 64 B, FP32 y and w).
 //
-// A group whose targets span more than kBfExtent scaled units (none of the Hilbert-ordered
-// groups of 1M uniform points at sigma 0.05; 34% of those of 1M samples of a sphere at
-// sigma 0.02) is summed in the same workgroup by the DFM_FP32C wavefront code
-// (sum_wavefront), one wavefront per 64 targets.  Ragged batches (grid.z = segment) keep
-// the caller's target order: the groups of a triangle mesh's faces in mesh order are as
-// compact as Hilbert-ordered ones.  Same accumulator
-// layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
+// A group whose targets span more than kBfExtent scaled units (kBfExtentBatch in ragged
+// batches of several segments; none of the Hilbert-ordered groups of 1M uniform points at
+// sigma 0.05; 34% of those of 1M samples of a sphere at sigma 0.02) is summed in the same
+// workgroup by the DFM_FP32C wavefront code (sum_wavefront), one wavefront per 64 targets.
+// Ragged batches (grid.z = segment) keep the caller's target order: the groups of a
+// triangle mesh's faces in mesh order are as compact as Hilbert-ordered ones.  Same
+// accumulator layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
 //
 // The reference arithmetic this replaces: the per-pair squared distance of
 // ScalarRadialKernel.h:61-73, tiled over threads in GpuConv1D.cu:55-78.
@@ -387,13 +388,15 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
           const float K = Prec<float>::kexp(r[t][v4]);
           const float *y = sv[v4];
           const float *w = sv[v4] + D;
-          if constexpr (OP == OP_CONV || OP == OP_CONVGRAD1) {
+          float kc = K;  // the Convolve weight: K, or K (u_i . u_j) for the varifold
+          if constexpr (OP == OP_VARIFOLD || OP == OP_VARIFOLD_CONV) kc = K * dt[t][v4];
+          if constexpr (OP != OP_GRAD1) {
 #pragma unroll
-            for (int k = 0; k < DV; k++) acc[t][k] = set ? K * w[k] : acc[t][k] + K * w[k];
+            for (int k = 0; k < DV; k++) acc[t][k] = set ? kc * w[k] : acc[t][k] + kc * w[k];
           }
-          if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
-            constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
-            const float tt = K * dt[t][v4];
+          if constexpr (OP != OP_CONV && OP != OP_VARIFOLD_CONV) {
+            constexpr int G = (OP == OP_GRAD1) ? 0 : DV;
+            const float tt = kc * dt[t][v4];
             if constexpr (OP == OP_GRAD1) acc[t][D] = set ? tt : acc[t][D] + tt;
 #pragma unroll
             for (int l = 0; l < D; l++) acc[t][G + l] = set ? tt * y[l] : acc[t][G + l] + tt * y[l];

@@ -81,7 +81,9 @@ template <int OP, int D, int DV>
 struct Shape {
   // per-target extra inputs: alpha (GRAD1, CONVGRAD1), beta + eta (GRADDIFF) or
   // alpha + mu_p + mu_a (SHOOTVJP)
-  static constexpr int NTE = (OP == OP_GRAD1 || OP == OP_CONVGRAD1) ? DV
+  static constexpr int NTE = (OP == OP_GRAD1 || OP == OP_CONVGRAD1 || OP == OP_VARIFOLD ||
+                              OP == OP_VARIFOLD_CONV)
+                                 ? DV
                              : (OP == OP_GRADDIFF)                  ? DV + D
                              : (OP == OP_SHOOTVJP)                  ? DV + 2 * D
                                                                     : 0;
@@ -94,6 +96,8 @@ struct Shape {
   static constexpr int NACC = OP == OP_CONV        ? DV
                               : OP == OP_GRAD1     ? D
                               : OP == OP_CONVGRAD1 ? DV + D
+                              : OP == OP_VARIFOLD  ? DV + D
+                              : OP == OP_VARIFOLD_CONV ? DV
                               : OP == OP_VARLIN    ? DV * D
                               : OP == OP_GRADDIM   ? DV
                               : OP == OP_GRADDIFF  ? 2 * D
@@ -102,7 +106,9 @@ struct Shape {
   // FP32 compact wavefronts of CONV, GRAD1 and CONVGRAD1 use the expanded distance (see
   // pair_update_expand): record [y (D) | w (DV) | |y|^2], and GRAD1 accumulates sum_j t_ij
   // beside sum_j t_ij y_j
-  static constexpr bool EXPANDABLE = OP == OP_CONV || OP == OP_GRAD1 || OP == OP_CONVGRAD1;
+  static constexpr bool EXPANDABLE =
+      OP == OP_CONV || OP == OP_GRAD1 || OP == OP_CONVGRAD1 || OP == OP_VARIFOLD ||
+      OP == OP_VARIFOLD_CONV;
   static constexpr int NSPE = (D + DV + 1 + 3) / 4 * 4;
   static constexpr int NACCE = OP == OP_GRAD1 ? D + 1 : NACC;
   static constexpr int NACCMAX = NACCE > NACC ? NACCE : NACC;
@@ -137,18 +143,24 @@ __device__ __forceinline__ void pair_update(const T *xi, const T *xl, const T *t
   if constexpr (OP == OP_CONV) {
 #pragma unroll
     for (int k = 0; k < DV; k++) acc[k] += K * w[k];
-  } else if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
-    constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
-    if constexpr (OP == OP_CONVGRAD1) {
-#pragma unroll
-      for (int k = 0; k < DV; k++) acc[k] += K * w[k];
-    }
+  } else if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1 || OP == OP_VARIFOLD ||
+                       OP == OP_VARIFOLD_CONV) {
+    // VARIFOLD: the fused sum with K (a_i . w_j) as the Convolve weight (a = w = u);
+    // VARIFOLD_CONV its Convolve part alone
+    constexpr int G = (OP == OP_GRAD1) ? 0 : DV;
     T sa = te[0] * w[0];
 #pragma unroll
     for (int k = 1; k < DV; k++) sa += te[k] * w[k];
-    const T t = K * sa;
+    const T kc = (OP == OP_VARIFOLD || OP == OP_VARIFOLD_CONV) ? K * sa : K;
+    if constexpr (OP != OP_GRAD1) {
 #pragma unroll
-    for (int l = 0; l < D; l++) acc[G + l] += t * d[l];
+      for (int k = 0; k < DV; k++) acc[k] += kc * w[k];
+    }
+    if constexpr (OP != OP_VARIFOLD_CONV) {
+      const T t = kc * sa;
+#pragma unroll
+      for (int l = 0; l < D; l++) acc[G + l] += t * d[l];
+    }
   } else if constexpr (OP == OP_VARLIN) {
 #pragma unroll
     for (int k = 0; k < DV; k++) {
@@ -253,19 +265,25 @@ __device__ __forceinline__ void pair_update_expand(const float *xm2, const float
   for (int k = 0; k < D; k++) e = __builtin_fmaf(xm2[k], rec[k], e);
   const float K = Prec<float>::kexp(e);
   const float *w = rec + D;
-  if constexpr (OP == OP_CONV || OP == OP_CONVGRAD1) {
+  if constexpr (OP == OP_CONV) {
 #pragma unroll
     for (int k = 0; k < DV; k++) acc[k] += K * w[k];
-  }
-  if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
-    constexpr int G = (OP == OP_CONVGRAD1) ? DV : 0;
+  } else {  // GRAD1, CONVGRAD1, VARIFOLD (Convolve weight K (a_i . w_j)), VARIFOLD_CONV
+    constexpr int G = (OP == OP_GRAD1) ? 0 : DV;
     float sa = te[0] * w[0];
 #pragma unroll
     for (int k = 1; k < DV; k++) sa += te[k] * w[k];
-    const float t = K * sa;
-    if constexpr (OP == OP_GRAD1) acc[D] += t;
+    const float kc = (OP == OP_VARIFOLD || OP == OP_VARIFOLD_CONV) ? K * sa : K;
+    if constexpr (OP != OP_GRAD1) {
 #pragma unroll
-    for (int l = 0; l < D; l++) acc[G + l] += t * rec[l];
+      for (int k = 0; k < DV; k++) acc[k] += kc * w[k];
+    }
+    if constexpr (OP != OP_VARIFOLD_CONV) {
+      const float t = kc * sa;
+      if constexpr (OP == OP_GRAD1) acc[D] += t;
+#pragma unroll
+      for (int l = 0; l < D; l++) acc[G + l] += t * rec[l];
+    }
   }
 }
 
@@ -281,14 +299,14 @@ __device__ __forceinline__ void expand_finish(double (&a)[NA], const float *xt, 
     n2 += x[k] * x[k];
   }
   const double f = exp2(-n2);
-  if constexpr (OP == OP_CONV) {
+  if constexpr (OP == OP_CONV || OP == OP_VARIFOLD_CONV) {
 #pragma unroll
     for (int k = 0; k < DV; k++) a[k] *= f;
   } else if constexpr (OP == OP_GRAD1) {
     const double st = a[D];
 #pragma unroll
     for (int l = 0; l < D; l++) a[l] = f * (x[l] * st - a[l]);
-  } else {  // OP_CONVGRAD1
+  } else {  // OP_CONVGRAD1, OP_VARIFOLD: sum_j t_ij = a_i . conv_i in both
     double st = 0.0;
 #pragma unroll
     for (int k = 0; k < DV; k++) st += (double)te[k] * a[k];
@@ -330,13 +348,13 @@ __device__ __forceinline__ void step_update(const KParams &p, long o, int seg, c
 // Scale the FP64 accumulators of one target and store them at original index `o`.
 template <int OP, int D, int DV>
 __device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
-  if constexpr (OP == OP_CONV) {
+  if constexpr (OP == OP_CONV || OP == OP_VARIFOLD_CONV) {
 #pragma unroll
     for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
   } else if constexpr (OP == OP_GRAD1) {
 #pragma unroll
     for (int l = 0; l < D; l++) p.out0[o * D + l] = p.cg * a[l];
-  } else if constexpr (OP == OP_CONVGRAD1) {
+  } else if constexpr (OP == OP_CONVGRAD1 || OP == OP_VARIFOLD) {
     if (p.out0) {
 #pragma unroll
       for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
@@ -569,7 +587,8 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
       xt[r][k] = (T)v;
       xl[r][k] = (T)(v - (double)xt[r][k]);
     }
-    if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1) {
+    if constexpr (OP == OP_GRAD1 || OP == OP_CONVGRAD1 || OP == OP_VARIFOLD ||
+                  OP == OP_VARIFOLD_CONV) {
 #pragma unroll
       for (int k = 0; k < DV; k++) te[r][k] = valid[r] ? (T)p.ta[orig[r] * DV + k] : T(0);
     } else if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
@@ -756,8 +775,9 @@ static int launch_aux(hipStream_t st, const KParams &p, const Plan &pl) {
 
 template <typename T, int MODE, int OP, int D>
 static int launch_dv(hipStream_t st, int DV, const KParams &p, const Plan &pl) {
-  if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
-    if (DV != D) return set_error(DFM_ERR_DIM, "self-interaction ops need dimvect == dimpoint");
+  if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP || OP == OP_VARIFOLD ||
+                OP == OP_VARIFOLD_CONV) {
+    if (DV != D) return set_error(DFM_ERR_DIM, "this op needs dimvect == dimpoint");
     return launch_aux<T, MODE, OP, D, D>(st, p, pl);
   } else {
     switch (DV) {
@@ -790,6 +810,8 @@ static int launch_op(hipStream_t st, int op, int D, int DV, const KParams &p, co
     case OP_GRADDIFF: return launch_d<T, MODE, OP_GRADDIFF>(st, D, DV, p, pl);
     case OP_SHOOTVJP: return launch_d<T, MODE, OP_SHOOTVJP>(st, D, DV, p, pl);
     case OP_HESS: return launch_d<T, MODE, OP_HESS>(st, D, DV, p, pl);
+    case OP_VARIFOLD: return launch_d<T, MODE, OP_VARIFOLD>(st, D, DV, p, pl);
+    case OP_VARIFOLD_CONV: return launch_d<T, MODE, OP_VARIFOLD_CONV>(st, D, DV, p, pl);
   }
   return set_error(DFM_ERR_ARG, "unknown op %d", op);
 }

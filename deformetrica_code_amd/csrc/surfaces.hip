@@ -9,18 +9,20 @@
 //   k_face_geometry   centers, kernel weights W_f and per-face coefficients A_f
 //                     current:  W = A = N = (p2 - p0) x (p1 - p0) / 2
 //                               (OrientedSurfaceMesh.cxx:298-326)
-//                     varifold: N /= sqrt(|N| + 1e-20); W = upper triangle of N N^T,
-//                               A = the quadratic-form coefficients
-//                               (N0^2, 2N0N1, 2N0N2, N1^2, 2N1N2, N2^2) so that
-//                               N^T S(m) N = A . m (NonOrientedSurfaceMesh.cxx:269-308,
-//                               special_product :364-379)
+//                     varifold: W = A = N = u, the normal divided by sqrt(|N| + 1e-20)
+//                               (NonOrientedSurfaceMesh.cxx:269-308)
 //   match             |S|^2 + |T|^2 - 2 <S, T> with <X, Y> = sum_f A^Y_f . (K W^X)(c^Y_f):
 //                     three Convolve calls and deterministic two-stage FP64 reductions
-//                     (OrientedSurfaceMesh.cxx:125-163, NonOrientedSurfaceMesh.cxx:124-165)
+//                     (OrientedSurfaceMesh.cxx:125-163, NonOrientedSurfaceMesh.cxx:124-165);
+//                     for the varifold the Convolve is the first half of OP_VARIFOLD,
+//                     V_f = sum_j K (u_f . u_j) u_j, and u_f . V_f = sum_j K (u_f . u_j)^2 is
+//                     the reference's special_product of N N^T with (K W)(c_f) for its
+//                     W_j = upper triangle of u_j u_j^T (:364-379)
 //   gradient          (K W)(c_S) and ConvolveGradient(c_S, A) from the template and from
-//                     the target (two fused Convolve + Grad1 calls; for the varifold the
-//                     reference's contraction of the varlin list with N N^T
-//                     (NonOrientedSurfaceMesh.cxx:243-249) is exactly Grad1 with alpha = A),
+//                     the target (two fused calls: Convolve + Grad1 for the current,
+//                     OP_VARIFOLD for the varifold, whose Grad1 part is the reference's
+//                     contraction of the varlin list with N N^T (:243-249) and whose
+//                     Convolve part is S((K W)(c_f)) u_f, all the contraction needs),
 //                     then per-face corner contributions (k_face_contrib) and a
 //                     deterministic per-vertex gather over a vertex->corner incidence list
 //                     (k_vertex_gather), in place of the reference's serial scatter
@@ -59,11 +61,10 @@ __global__ __launch_bounds__(256) void k_face_geometry(int kind, const double *V
   if (f >= nf * nsub) return;
   const long sub = f / nf, fl = f - sub * nf;
   V += sub * nv * 3;
-  const int dv = kind == DFM_CURRENT ? 3 : 6;
   int i0 = F[3 * fl], i1 = F[3 * fl + 1], i2 = F[3 * fl + 2];
   if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) {
     for (int k = 0; k < 3; k++) C[3 * f + k] = 0.0, Nout[3 * f + k] = 0.0;
-    for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
+    for (int k = 0; k < 3; k++) W[3 * f + k] = 0.0, A[3 * f + k] = 0.0;
     *bad = 1.0;
     return;
   }
@@ -83,16 +84,7 @@ __global__ __launch_bounds__(256) void k_face_geometry(int kind, const double *V
     return;
   }
   const double s = sqrt(sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) + 1e-20);
-  for (int k = 0; k < 3; k++) {
-    n[k] /= s;
-    Nout[3 * f + k] = n[k];
-  }
-  int idx = 0;
-  for (int p = 0; p < 3; p++)
-    for (int q = p; q < 3; q++, idx++) {
-      W[6 * f + idx] = n[p] * n[q];
-      A[6 * f + idx] = (p == q ? 1.0 : 2.0) * n[p] * n[q];
-    }
+  for (int k = 0; k < 3; k++) W[3 * f + k] = A[3 * f + k] = Nout[3 * f + k] = n[k] / s;
 }
 
 // partial[b] = sum over the block's faces of dot(X_f, Y_f) (dv columns)
@@ -127,13 +119,6 @@ __global__ __launch_bounds__(256) void k_combine(const double *p0, double c0, co
   if (t == 0) *out = sh[0];
 }
 
-__device__ __forceinline__ void sym_times(const double *m, const double *x, double *y) {
-  // symmetric matrix from (00, 01, 02, 11, 12, 22) times x
-  y[0] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
-  y[1] = m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
-  y[2] = m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
-}
-
 // contrib[f][c] (3 corners x 3) of the match gradient (nsub meshes as in k_face_geometry)
 __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V, long nv,
                                                       const int *F, long nf, int nsub,
@@ -148,7 +133,6 @@ __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V,
   for (int k = 0; k < 9; k++) out[k] = 0.0;
   const int i0 = F[3 * fl], i1 = F[3 * fl + 1], i2 = F[3 * fl + 2];
   if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= nv || i1 >= nv || i2 >= nv) return;
-  const int dv = kind == DFM_CURRENT ? 3 : 6;
   double e[3][3], kt[3], gk[3];
   for (int k = 0; k < 3; k++) {
     e[0][k] = V[3 * (long)i2 + k] - V[3 * (long)i1 + k];
@@ -162,10 +146,10 @@ __global__ __launch_bounds__(256) void k_face_contrib(int kind, const double *V,
     const double *n = N + 3 * f;
     const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
     if (!(n2 > 1e-20)) return;
-    double dm[6], t[3];
-    for (int k = 0; k < 6; k++) dm[k] = ktS[dv * f + k] - ktT[dv * f + k];
-    sym_times(dm, n, t);
-    for (int k = 0; k < 3; k++) t[k] *= 2.0;
+    // t = 2 S((K W_S)(c_f) - (K W_T)(c_f)) N = 2 (V_S - V_T), the Convolve parts of the
+    // two OP_VARIFOLD sums
+    double t[3];
+    for (int k = 0; k < 3; k++) t[k] = 2.0 * (ktS[3 * f + k] - ktT[3 * f + k]);
     const double nt = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];  // S(N N^T) t = N (N . t)
     const double r = 1.0 / sqrt(n2);
     for (int k = 0; k < 3; k++) kt[k] = (t[k] - n[k] * nt / (2 * n2)) * r;
@@ -231,13 +215,7 @@ __global__ __launch_bounds__(256) void k_sample_geometry(int kind, const double 
     return;
   }
   const double s = sqrt(sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 1e-20);
-  for (int k = 0; k < 3; k++) Nt[3 * i + k] = v[k] /= s;
-  int idx = 0;
-  for (int p = 0; p < 3; p++)
-    for (int q = p; q < 3; q++, idx++) {
-      W[6 * i + idx] = v[p] * v[q];
-      A[6 * i + idx] = (p == q ? 1.0 : 2.0) * v[p] * v[q];
-    }
+  for (int k = 0; k < 3; k++) W[3 * i + k] = A[3 * i + k] = Nt[3 * i + k] = v[k] / s;
 }
 
 // d match / d c_i = 2 (g1_i - g2_i);  d match / d N_i = 2 (k1_i - k2_i) for currents and,
@@ -258,9 +236,9 @@ __global__ __launch_bounds__(256) void k_sample_grads(int kind, const double *N,
     for (int k = 0; k < 3; k++) grad_n[3 * i + k] = 2.0 * (k1[3 * i + k] - k2[3 * i + k]);
     return;
   }
-  double dm[6], g[3];
-  for (int k = 0; k < 6; k++) dm[k] = k1[6 * i + k] - k2[6 * i + k];
-  sym_times(dm, Nt + 3 * i, g);
+  // S((K W_S)(c_i) - (K W_T)(c_i)) u_i: the Convolve parts of the two OP_VARIFOLD sums
+  double g[3];
+  for (int k = 0; k < 3; k++) g[k] = k1[3 * i + k] - k2[3 * i + k];
   const double *v = N + 3 * i;
   const double a = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   if (!(a > 0.0)) {
@@ -275,22 +253,10 @@ __global__ __launch_bounds__(256) void k_sample_grads(int kind, const double *N,
 
 // ---- polylines: segments (i0, i1) of points in D = 2 or 3 ----------------------------
 
-// symmetric D x D matrix from its upper triangle (row by row) times x
-template <int D>
-__device__ __forceinline__ void sym_times_d(const double *m, const double *x, double *y) {
-  for (int p = 0; p < D; p++) y[p] = 0.0;
-  int idx = 0;
-  for (int p = 0; p < D; p++)
-    for (int q = p; q < D; q++, idx++) {
-      y[p] += m[idx] * x[q];
-      if (q != p) y[q] += m[idx] * x[p];
-    }
-}
-
 // Geometry of each segment: centre (p0 + p1) / 2; current W = A = T = p1 - p0
-// (OrientedPolyLine.cxx:249-272); varifold T = (p1 - p0) / sqrt(|p1 - p0| + 1e-20),
-// W = upper triangle of T T^T, A = its quadratic-form coefficients
-// (NonOrientedPolyLine.cxx:266-297).  Bad indices flag the segment (weights zero).
+// (OrientedPolyLine.cxx:249-272); varifold W = A = T = (p1 - p0) / sqrt(|p1 - p0| + 1e-20)
+// (NonOrientedPolyLine.cxx:266-297; its upper triangle of T T^T enters through OP_VARIFOLD).
+// Bad indices flag the segment (weights zero).
 // nsub polylines share one segment list E (ne segments over nv points): segment f of the
 // output belongs to polyline f / ne, whose points start at row (f / ne) * nv of V.
 template <int D>
@@ -302,11 +268,10 @@ __global__ __launch_bounds__(256) void k_seg_geometry(int kind, const double *V,
   if (f >= ne * nsub) return;
   const long sub = f / ne, fl = f - sub * ne;
   V += sub * nv * D;
-  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
   const int i0 = E[2 * fl], i1 = E[2 * fl + 1];
   if (i0 < 0 || i1 < 0 || i0 >= nv || i1 >= nv) {
     for (int k = 0; k < D; k++) C[D * f + k] = 0.0, T[D * f + k] = 0.0;
-    for (int k = 0; k < dv; k++) W[dv * f + k] = 0.0, A[dv * f + k] = 0.0;
+    for (int k = 0; k < D; k++) W[D * f + k] = 0.0, A[D * f + k] = 0.0;
     *bad = 1.0;
     return;
   }
@@ -322,13 +287,7 @@ __global__ __launch_bounds__(256) void k_seg_geometry(int kind, const double *V,
     return;
   }
   const double s = sqrt(sqrt(t2) + 1e-20);
-  for (int k = 0; k < D; k++) T[D * f + k] = t[k] /= s;
-  int idx = 0;
-  for (int p = 0; p < D; p++)
-    for (int q = p; q < D; q++, idx++) {
-      W[dv * f + idx] = t[p] * t[q];
-      A[dv * f + idx] = (p == q ? 1.0 : 2.0) * t[p] * t[q];
-    }
+  for (int k = 0; k < D; k++) W[D * f + k] = A[D * f + k] = T[D * f + k] = t[k] / s;
 }
 
 // contrib[f] (2 endpoints x D) of the match gradient, from kt = (K W_S)(c_S) - (K W_T)(c_S)
@@ -353,14 +312,12 @@ __global__ __launch_bounds__(256) void k_seg_contrib(int kind, const int *E, lon
   if (kind == DFM_CURRENT) {
     for (int k = 0; k < D; k++) u[k] = 2.0 * (ktS[D * f + k] - ktT[D * f + k]);
   } else {
-    constexpr int dv = D * (D + 1) / 2;
     const double *n = T + D * f;
     double n2 = 0.0;
     for (int k = 0; k < D; k++) n2 += n[k] * n[k];
     if (!(n2 > 1e-20)) return;
-    double dm[dv];
-    for (int k = 0; k < dv; k++) dm[k] = ktS[dv * f + k] - ktT[dv * f + k];
-    sym_times_d<D>(dm, n, u);
+    // S(kt) T = V_S - V_T, the Convolve parts of the two OP_VARIFOLD sums
+    for (int k = 0; k < D; k++) u[k] = ktS[D * f + k] - ktT[D * f + k];
     double nu = 0.0;
     for (int k = 0; k < D; k++) nu += n[k] * (u[k] *= 4.0);
     const double r = 1.0 / sqrt(n2);
@@ -466,7 +423,7 @@ struct SampleLayout {
   void *pw;
   size_t pw_bytes, total;
   SampleLayout(char *base, int kind, int prec, long ns, long nt, bool gradient) {
-    const int dv = kind == DFM_CURRENT ? 3 : 6;
+    const int dv = 3;
     size_t off = 0;
     auto take = [&](size_t count) {
       double *p = base ? (double *)(base + off) : nullptr;
@@ -479,10 +436,10 @@ struct SampleLayout {
     g1 = gradient ? take(3 * ns) : nullptr;
     g2 = gradient ? take(3 * ns) : nullptr;
     p1 = take(kRedBlocks); p2 = take(kRedBlocks); p3 = take(kRedBlocks);
-    const int op = gradient ? OP_CONVGRAD1 : OP_CONV;
+    const int op = gradient ? data_grad_op(kind) : data_conv_op(kind);
     pw_bytes = std::max({pairsum_workspace(op, prec, 3, dv, ns, ns),
                          pairsum_workspace(op, prec, 3, dv, ns, nt),
-                         pairsum_workspace(OP_CONV, prec, 3, dv, nt, nt)});
+                         pairsum_workspace(data_conv_op(kind), prec, 3, dv, nt, nt)});
     pw = base ? (void *)(base + off) : nullptr;
     off += a256(pw_bytes);
     total = off;
@@ -498,7 +455,7 @@ struct Layout {
   void *pw;
   size_t pw_bytes, total;
   Layout(char *base, int kind, int prec, long nfs, long nft, bool gradient) {
-    const int dv = kind == DFM_CURRENT ? 3 : 6;
+    const int dv = 3;
     size_t off = 0;
     auto take = [&](size_t count) {
       double *p = base ? (double *)(base + off) : nullptr;
@@ -515,12 +472,14 @@ struct Layout {
     bad = take(1);
     pw_bytes = 0;
     if (gradient) {
-      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONVGRAD1, prec, 3, dv, nfs, nfs));
-      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONVGRAD1, prec, 3, dv, nfs, nft));
+      const int op = data_grad_op(kind);
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(op, prec, 3, dv, nfs, nfs));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(op, prec, 3, dv, nfs, nft));
     } else {
-      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nfs, nfs));
-      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nft, nft));
-      pw_bytes = std::max(pw_bytes, pairsum_workspace(OP_CONV, prec, 3, dv, nft, nfs));
+      const int op = data_conv_op(kind);
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(op, prec, 3, dv, nfs, nfs));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(op, prec, 3, dv, nft, nft));
+      pw_bytes = std::max(pw_bytes, pairsum_workspace(op, prec, 3, dv, nft, nfs));
     }
     pw = base ? (void *)(base + off) : nullptr;
     off += a256(pw_bytes);
@@ -537,7 +496,7 @@ struct CurveLayout {
   void *pw;
   size_t pw_bytes, total;
   CurveLayout(char *base, int kind, int D, int prec, long nes, long net, bool gradient) {
-    const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+    const int dv = D;
     size_t off = 0;
     auto take = [&](size_t count) {
       double *p = base ? (double *)(base + off) : nullptr;
@@ -553,12 +512,12 @@ struct CurveLayout {
     contrib = gradient ? take(2 * D * nes) : nullptr;
     bad = take(1);
     if (gradient)
-      pw_bytes = std::max(pairsum_workspace(OP_CONVGRAD1, prec, D, dv, nes, nes),
-                          pairsum_workspace(OP_CONVGRAD1, prec, D, dv, nes, net));
+      pw_bytes = std::max(pairsum_workspace(data_grad_op(kind), prec, D, dv, nes, nes),
+                          pairsum_workspace(data_grad_op(kind), prec, D, dv, nes, net));
     else
-      pw_bytes = std::max({pairsum_workspace(OP_CONV, prec, D, dv, nes, nes),
-                           pairsum_workspace(OP_CONV, prec, D, dv, net, net),
-                           pairsum_workspace(OP_CONV, prec, D, dv, net, nes)});
+      pw_bytes = std::max({pairsum_workspace(data_conv_op(kind), prec, D, dv, nes, nes),
+                           pairsum_workspace(data_conv_op(kind), prec, D, dv, net, net),
+                           pairsum_workspace(data_conv_op(kind), prec, D, dv, net, nes)});
     pw = base ? (void *)(base + off) : nullptr;
     off += a256(pw_bytes);
     total = off;
@@ -670,18 +629,19 @@ template <int D>
 int curve_match(hipStream_t st, int kind, double sigma, int prec, const CurveLayout &L,
                 const double *vs, long nvs, const int *es, long nes, const double *vt, long nvt,
                 const int *et, long net, double *match) {
-  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  const int dv = D, op = data_conv_op(kind);
   int rc = curve_geometry<D>(st, kind, L, vs, nvs, es, nes, vt, nvt, et, net);
   if (rc) return rc;
   KParams p = blank();
   // |S|^2 = sum_f A_S,f . (K W_S)(c_S,f) (OrientedPolyLine.cxx:279-296,
   // NonOrientedPolyLine.cxx:304-324), |T|^2 likewise, <S, T> = sum_f A_T,f . (K W_S)(c_T,f)
-  p.x = L.cS; p.nx = nes; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k1;
-  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, nes * dv, 0, L.pw, L.pw_bytes))) return rc;
-  p.x = L.cT; p.nx = net; p.y = L.cT; p.w = L.wT; p.ny = net; p.out0 = L.k2;
-  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
-  p.x = L.cT; p.nx = net; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k3;
-  if ((rc = pairsum(st, OP_CONV, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
+  // (the varifold's OP_VARIFOLD takes the targets' own u as A)
+  p.x = L.cS; p.ta = L.aS; p.nx = nes; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k1;
+  if ((rc = pairsum(st, op, prec, D, dv, sigma, p, nes * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.ta = L.aT; p.nx = net; p.y = L.cT; p.w = L.wT; p.ny = net; p.out0 = L.k2;
+  if ((rc = pairsum(st, op, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.ta = L.aT; p.nx = net; p.y = L.cS; p.w = L.wS; p.ny = nes; p.out0 = L.k3;
+  if ((rc = pairsum(st, op, prec, D, dv, sigma, p, net * dv, 0, L.pw, L.pw_bytes))) return rc;
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, nes, dv, L.p1);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aT, net, dv, L.p2);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, L.aT, net, dv, L.p3);
@@ -695,19 +655,19 @@ int curve_match_gradient(hipStream_t st, int kind, double sigma, int prec, const
                          const double *vs, long nvs, const int *es, long nes, const int *offsets,
                          const int *incidences, const double *vt, long nvt, const int *et,
                          long net, double *grad) {
-  const int dv = kind == DFM_CURRENT ? D : D * (D + 1) / 2;
+  const int dv = D, op = data_grad_op(kind);
   int rc = curve_geometry<D>(st, kind, L, vs, nvs, es, nes, vt, nvt, et, net);
   if (rc) return rc;
   KParams p = blank();
   // (K W_S)(c_S) with ConvolveGradient(c_S, A_S), then the same from the target; for the
-  // varifold, Grad1 with alpha = A is the reference's contraction of the per-weight
+  // varifold (OP_VARIFOLD) the Grad1 part is the reference's contraction of the per-weight
   // gradients with T T^T (NonOrientedPolyLine.cxx:239-249)
   p.x = L.cS; p.ta = L.aS; p.nx = nes; p.y = L.cS; p.w = L.wS; p.ny = nes;
   p.out0 = L.k1; p.out1 = L.g1;
-  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
+  if ((rc = pairsum(st, op, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
     return rc;
   p.y = L.cT; p.w = L.wT; p.ny = net; p.out0 = L.k2; p.out1 = L.g2;
-  if ((rc = pairsum(st, OP_CONVGRAD1, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
+  if ((rc = pairsum(st, op, prec, D, dv, sigma, p, nes * dv, nes * D, L.pw, L.pw_bytes)))
     return rc;
   if (nes > 0)
     hipLaunchKernelGGL(k_seg_contrib<D>, dim3(blocks_for(nes)), dim3(256), 0, st, kind, es, nes,
@@ -1091,18 +1051,18 @@ int dfm_surface_match(void *stream, int kind, double sigma, int prec, const doub
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   Layout L((char *)work, kind, prec, nfs, nft, false);
   hipStream_t st = (hipStream_t)stream;
-  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  const int dv = 3, op = data_conv_op(kind);
   if ((rc = geometry(st, kind, L, vs, nvs, fs, nfs, vt, nvt, ft, nft))) return rc;
   KParams p = blank();
-  // |S|^2: (K W_S)(c_S) . A_S
-  p.x = L.cS; p.nx = nfs; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k1;
-  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nfs * dv, 0, L.pw, L.pw_bytes))) return rc;
+  // |S|^2: (K W_S)(c_S) . A_S (the varifold's OP_VARIFOLD takes the targets' own u as A)
+  p.x = L.cS; p.ta = L.aS; p.nx = nfs; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k1;
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, nfs * dv, 0, L.pw, L.pw_bytes))) return rc;
   // |T|^2
-  p.x = L.cT; p.nx = nft; p.y = L.cT; p.w = L.wT; p.ny = nft; p.out0 = L.k2;
-  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.ta = L.aT; p.nx = nft; p.y = L.cT; p.w = L.wT; p.ny = nft; p.out0 = L.k2;
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
   // <S, T>: (K W_S)(c_T) . A_T
-  p.x = L.cT; p.nx = nft; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k3;
-  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = L.cT; p.ta = L.aT; p.nx = nft; p.y = L.cS; p.w = L.wS; p.ny = nfs; p.out0 = L.k3;
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, nft * dv, 0, L.pw, L.pw_bytes))) return rc;
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, nfs, dv, L.p1);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aT, nft, dv, L.p2);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k3, L.aT, nft, dv, L.p3);
@@ -1125,17 +1085,17 @@ int dfm_surface_match_gradient(void *stream, int kind, double sigma, int prec, c
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   Layout L((char *)work, kind, prec, nfs, nft, true);
   hipStream_t st = (hipStream_t)stream;
-  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  const int dv = 3, op = data_grad_op(kind);
   if ((rc = geometry(st, kind, L, vs, nvs, fs, nfs, vt, nvt, ft, nft))) return rc;
   KParams p = blank();
   // template on template: (K W_S)(c_S) and ConvolveGradient(c_S, A_S)
   p.x = L.cS; p.ta = L.aS; p.nx = nfs; p.y = L.cS; p.w = L.wS; p.ny = nfs;
   p.out0 = L.k1; p.out1 = L.g1;
-  if ((rc = pairsum(st, OP_CONVGRAD1, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
     return rc;
   // target on template
   p.y = L.cT; p.w = L.wT; p.ny = nft; p.out0 = L.k2; p.out1 = L.g2;
-  if ((rc = pairsum(st, OP_CONVGRAD1, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
+  if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, nfs * dv, nfs * 3, L.pw, L.pw_bytes)))
     return rc;
   if (nfs > 0)
     hipLaunchKernelGGL(k_face_contrib, dim3(blocks_for(nfs)), dim3(256), 0, st, kind, vs, nvs, fs,
@@ -1283,7 +1243,7 @@ int dfm_samples_match_gradient(void *stream, int kind, double sigma, int prec, c
     return set_error(DFM_ERR_ARG, "workspace too small: need %zu bytes", need.total);
   SampleLayout L((char *)work, kind, prec, ns, nt, gradient);
   hipStream_t st = (hipStream_t)stream;
-  const int dv = kind == DFM_CURRENT ? 3 : 6;
+  const int dv = 3;
   if (ns > 0)
     hipLaunchKernelGGL(k_sample_geometry, dim3(blocks_for(ns)), dim3(256), 0, st, kind, Ns, ns,
                        L.wS, L.aS, L.tS);
@@ -1292,7 +1252,7 @@ int dfm_samples_match_gradient(void *stream, int kind, double sigma, int prec, c
                        L.wT, L.aT, L.tT);
   if ((rc = launched("k_sample_geometry"))) return rc;
   KParams p = blank();
-  const int op = gradient ? OP_CONVGRAD1 : OP_CONV;
+  const int op = gradient ? data_grad_op(kind) : data_conv_op(kind);
   // (K W_S)(c_S) [+ Grad1(c_S, A_S)] and (K W_T)(c_S) [+ Grad1(c_S, A_S)]
   p.x = cs; p.ta = L.aS; p.nx = ns; p.y = cs; p.w = L.wS; p.ny = ns; p.out0 = L.k1; p.out1 = L.g1;
   if ((rc = pairsum(st, op, prec, 3, dv, sigma, p, ns * dv, gradient ? ns * 3 : 0, L.pw,
@@ -1304,8 +1264,10 @@ int dfm_samples_match_gradient(void *stream, int kind, double sigma, int prec, c
     return rc;
   // (K W_T)(c_T): the target's squared norm
   p = blank();
-  p.x = ct; p.nx = nt; p.y = ct; p.w = L.wT; p.ny = nt; p.out0 = L.kT;
-  if ((rc = pairsum(st, OP_CONV, prec, 3, dv, sigma, p, nt * dv, 0, L.pw, L.pw_bytes))) return rc;
+  p.x = ct; p.ta = L.aT; p.nx = nt; p.y = ct; p.w = L.wT; p.ny = nt; p.out0 = L.kT;
+  if ((rc = pairsum(st, data_conv_op(kind), prec, 3, dv, sigma, p, nt * dv, 0, L.pw,
+                    L.pw_bytes)))
+    return rc;
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k1, L.aS, ns, dv, L.p1);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.kT, L.aT, nt, dv, L.p2);
   hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, st, L.k2, L.aS, ns, dv, L.p3);

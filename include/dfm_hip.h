@@ -66,8 +66,9 @@ enum dfm_precision { DFM_FP64 = 0, DFM_FP32C = 1, DFM_FP32X = 2, DFM_BF16MFMA = 
  *   2  dfm_atlas_desc gained dimension and cell_corners (appended); new entry points for
  *      polylines, volume meshes, point clouds, landmarks, ConvolveHessian and the
  *      adjoint's jump conditions; DFM_OP_HESSIAN
- *   3  dfm_memcpy_async; DFM_BF16MFMA also serves ragged batches */
-#define DFM_ABI_VERSION 3
+ *   3  dfm_memcpy_async; DFM_BF16MFMA also serves ragged batches
+ *   4  DFM_OP_VARIFOLD and DFM_OP_VARIFOLD_CONV in the workspace and flops queries */
+#define DFM_ABI_VERSION 4
 int dfm_abi_version(void);
 /* Last error message of the calling thread ("" when none). */
 const char *dfm_last_error(void);
@@ -104,7 +105,8 @@ int dfm_profile_work(double *pairs, double *flops);
 /* Algorithmic flops of one (target, source) pair of an op (DFM_OP_*), counted on the
  * reference's per-pair arithmetic with exp as one: 3 dimpoint + 1 for the kernel value,
  * then 2 dimvect (Convolve), 2 dimvect + 2 dimpoint + 1 (ConvolveGradient(X, alpha)),
- * both for the fused sum (29 at dimpoint = dimvect = 3). */
+ * both for the fused sum (29 at dimpoint = dimvect = 3), one more product for the
+ * varifold's fused sum (30) and 2 dimvect for its Convolve part's weight K (u.u) (22). */
 double dfm_flops_per_pair(int op, int dimpoint, int dimvect);
 
 /* Autotuner of the pair-sum work split (no reference counterpart: the reference launches
@@ -209,7 +211,13 @@ enum dfm_op {
   DFM_OP_GRADDIM = 4,
   DFM_OP_GRADDIFF = 5,
   DFM_OP_SHOOT_VJP = 6,
-  DFM_OP_HESSIAN = 7
+  DFM_OP_HESSIAN = 7,
+  /* the varifold data term's sums over unit-root normals u (dimvect = dimpoint):
+   * V_i = sum_j K_ij (u_i . u_j) u_j and its gradient sum_j grad_1 K_ij (u_i . u_j)^2
+   * (no entry point of its own: the varifold match and gradient entry points run it) */
+  DFM_OP_VARIFOLD = 8,
+  /* its Convolve part V_i alone (the varifold match) */
+  DFM_OP_VARIFOLD_CONV = 9
 };
 
 /* Bytes of workspace the device entry point for `op` needs at these sizes. */

@@ -127,7 +127,9 @@ def test_flops_per_pair_table(lib):
     """dfm_flops_per_pair counts the reference's per-pair arithmetic (ExactKernel.cxx):
     3d + 1 for the kernel value, 2 Dv per Convolve weight, 2 Dv + 2d + 1 for
     ConvolveGradient(X, alpha): 16 / 23 / 29 at d = Dv = 3, 41 for the fused Dv = 6 sum,
-    50 for the Dv = 6 varlin list, 58 for the Dv = 3 Hessian list."""
+    50 for the Dv = 6 varlin list, 58 for the Dv = 3 Hessian list; the varifold's sums over
+    u: 30 for the fused sum (one more product, K (u.u)) and 22 for its Convolve part
+    (u.u, the scale, 2 Dv).  Their workspace query needs dimvect == dimpoint."""
     from deformetrica_code_amd import _lib
     f = lib.dfm_flops_per_pair
     assert f(_lib.OP_CONV, 3, 3) == 16
@@ -138,3 +140,9 @@ def test_flops_per_pair_table(lib):
     assert f(_lib.OP_HESSIAN, 3, 3) == 58
     assert f(_lib.OP_CONV_GRAD1, 2, 3) == 24   # 2D: 7 + 6 + 11
     assert f(_lib.OP_CONV, 2, 3) == 13
+    assert f(_lib.OP_VARIFOLD, 3, 3) == 30
+    assert f(_lib.OP_VARIFOLD_CONV, 3, 3) == 22
+    assert f(_lib.OP_VARIFOLD, 2, 2) == 21     # 7 + 4 + 9 + 1
+    for op in (_lib.OP_VARIFOLD, _lib.OP_VARIFOLD_CONV):
+        assert lib.dfm_workspace_size(op, 3, 3, 1 << 20, 1 << 20, _lib.FP32C) > 0
+        assert lib.dfm_workspace_size(op, 3, 6, 1 << 20, 1 << 20, _lib.FP32C) == 0

@@ -10,7 +10,9 @@ Checks:
   * d/d centres and d/d normals on 256 sampled source rows against the oracle's kernel
     sums with Dv = 6 weights over all 2,000,000 samples
     (oracle/surfaces.py samples_varifold_gradient_rows, pinned by central differences of
-    the definitional double sum): FP64 to 1e-10 norm-wise, FP32C and DFM_BF16MFMA to 1e-6;
+    the definitional double sum): FP64 to 1e-10 norm-wise, FP32C and DFM_BF16MFMA to 1e-6
+    (the library sums Dv = 3 weights, the unit-root normals u, with the per-pair factor
+    u_i . u_j: an independent formulation of the same sums);
   * FP32C and DFM_BF16MFMA (the distances' cross term on the matrix cores, as the workload
     names it) against FP64 over every row: gradients norm-wise <= 1e-6, the match to 1e-8
     relative (the match is a difference of three sums of 1e12 terms each).
