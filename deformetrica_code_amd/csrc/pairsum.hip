@@ -318,11 +318,19 @@ static int tile_of(int prec) {
 
 // DFM_BF16MFMA runs the matrix-core kernel for Convolve / Grad1 / fused sums: single calls
 // large enough to be Hilbert-sorted, and ragged batches (nseg > 1, the caller's target
-// order: groups of 256 targets that span too far take the FP32C code inside the kernel);
-// everything else runs as DFM_FP32C.
-static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx) {
+// order: groups of 256 targets that span too far take the FP32C code inside the kernel)
+// whose largest segment has kBfBatchMinSources sources; everything else runs as DFM_FP32C.
+// 64 segments of 10,242 icosphere vertices in mesh order against control grids of 2,048,
+// 4,096, 8,192 and 16,384 points (sigma 0.25) ran the fused sum 1.11, 1.18, 1.04 and 0.99
+// times as long as DFM_FP32C (Convolve 1.13, 1.08, 0.94, 0.92): each group's set-up (the
+// target splits, the FP64 epilogue, the FP32C code of its wide groups) pays off only over
+// many source tiles.
+constexpr long kBfBatchMinSources = 8192;
+
+static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx, long ny) {
   if (prec != DFM_BF16MFMA) return prec;
-  return ((nseg > 1 || nx >= kSortMinTargets) && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
+  const bool big = nseg > 1 ? ny >= kBfBatchMinSources : nx >= kSortMinTargets;
+  return (big && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
 }
 
 static int nacc_of(int op, int D, int DV) {
@@ -376,10 +384,14 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
       op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1 || op == OP_VARIFOLD ||
       op == OP_VARIFOLD_CONV;
   // Grad1 and the fused sum with more than 3 weights keep one target per lane (the second
-  // target's accumulators cost occupancy): the 1M-sample varifold's fused Dv = 6 sum runs
-  // 446 ms with R = 1 and 262,144 wavefronts against 481 ms with R = 2; the 64-subject
-  // atlas's batched one 11.8 ms with R = 1 and 65,536 wavefronts against 15.9 with R = 2.
-  const bool two = expandable && (op == OP_CONV || op == OP_VARIFOLD_CONV || DV <= 3);
+  // target's accumulators cost occupancy): a 1M-sample fused Dv = 6 sum runs 446 ms with
+  // R = 1 and 262,144 wavefronts against 481 ms with R = 2, a 64-subject batch of them
+  // 11.8 ms with R = 1 and 65,536 wavefronts against 15.9 with R = 2.  So does the
+  // varifold's fused sum (the per-pair u_i . u_j on top of six accumulators): the
+  // 64-subject atlas's two data-term batches take 9.6 and 10.0 ms with R = 1 and 65,536
+  // wavefronts against 12.9 and 14.7 ms with R = 2.
+  const bool two =
+      expandable && (op == OP_CONV || op == OP_VARIFOLD_CONV || (DV <= 3 && op != OP_VARIFOLD));
   if (prec != DFM_FP64 && expandable && nseg == 1 && blocks >= 4096)
     return two ? Choice{2, 65536, 4} : Choice{1, 262144, 4};
   // ragged batches on grids of >= 1,024 target workgroups (the atlas's flow, adjoint and
@@ -504,7 +516,7 @@ static Plan single_plan(int op, int prec, int D, int DV, long nx, long ny, const
 // Workspace bound over every candidate, so that a caller's buffer fits whatever plan the
 // autotuner picks.
 Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
-  prec = effective_prec(op, prec, D, DV, 1, nx);
+  prec = effective_prec(op, prec, D, DV, 1, nx, ny);
   Plan best = single_plan(op, prec, D, DV, nx, ny, default_choice(op, prec, D, DV, 1, nx));
   for (const Choice &c : candidates(op, prec, D, DV, 1, nx)) {
     Plan p = single_plan(op, prec, D, DV, nx, ny, c);
@@ -515,7 +527,7 @@ Plan make_plan(int op, int prec, int D, int DV, long nx, long ny) {
 
 Plan make_batch_plan(int op, int prec, int D, int DV, int nseg, long nx_total, long max_nx,
                      long max_ny) {
-  prec = effective_prec(op, prec, D, DV, nseg, max_nx);
+  prec = effective_prec(op, prec, D, DV, nseg, max_nx, max_ny);
   Plan best = batch_plan(op, prec, D, DV, nseg, nx_total, max_nx, max_ny,
                          default_choice(op, prec, D, DV, nseg, max_nx));
   for (const Choice &c : candidates(op, prec, D, DV, nseg, max_nx)) {
@@ -646,8 +658,10 @@ static Choice tune_choose(int op, int prec, int D, int DV, int batch, int nseg, 
 
 int tune_report(char *buf, size_t len) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  static const char *ops[] = {"conv", "grad1", "conv_grad1", "varlin", "graddim", "graddiff",
-                              "shoot_vjp", "hessian"};
+  static const char *ops[] = {"conv",     "grad1",     "conv_grad1", "varlin",
+                              "graddim",  "graddiff",  "shoot_vjp",  "hessian",
+                              "varifold", "varifold_conv"};
+  constexpr int nops = (int)(sizeof(ops) / sizeof(ops[0]));
   std::string out;
   for (auto &kv : g_tune) {
     resolve_pending(kv.second);
@@ -658,7 +672,8 @@ int tune_report(char *buf, size_t len) {
     snprintf(line, sizeof(line),
              "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d grid=%d -> R=%d waves=%d "
              "min_tiles=%d mx=%d %s %.4f ms (default %.4f ms) |",
-             ops[k.op], k.prec, k.D, k.DV, k.batch, k.lx, k.ly, k.grid, w.R, w.waves, w.min_tiles,
+             k.op >= 0 && k.op < nops ? ops[k.op] : "?", k.prec, k.D, k.DV, k.batch, k.lx, k.ly,
+             k.grid, w.R, w.waves, w.min_tiles,
              w.mx,
              t.done ? "tuned" : "exploring", t.done ? t.best_ms[t.winner] : 0.0,
              t.best_ms.empty() ? 0.0 : t.best_ms[0]);
@@ -848,7 +863,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
                   const Seg &xs, const Seg &ys, long out0_n, long out1_n, void *work,
                   size_t work_bytes) {
   if (xs.n != ys.n) return set_error(DFM_ERR_ARG, "target and source segment counts differ");
-  prec = effective_prec(op, prec, D, DV, xs.n, xs.max);
+  prec = effective_prec(op, prec, D, DV, xs.n, xs.max, ys.max);
   p.nx = xs.total;
   p.ny = ys.total;
   p.xoff = xs.n > 1 ? xs.d : nullptr;
@@ -878,7 +893,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
 int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KParams p,
             long out0_n, long out1_n, void *work, size_t work_bytes, const char *ws_label) {
   if (p.nx == 0) return DFM_OK;
-  prec = effective_prec(op, prec, D, DV, 1, p.nx);
+  prec = effective_prec(op, prec, D, DV, 1, p.nx, p.ny);
   if (p.ny == 0) {  // empty sums
     fill(st, p.out0, out0_n, 0.0);
     fill(st, p.out1, out1_n, 0.0);
