@@ -62,17 +62,17 @@ constexpr int kBfTile = 256;
 // 2^24 + 1 - 2^24 gives 1, 2^25 + 1 - 2^25 gives 0) and the groups in FP32, so
 // r' = |y|^2 - 2 x.y rounds at ~2^-24 of |x|^2 for a target at distance |x| from the group
 // centre: a target norm added in the K dimension would not cancel exactly (and
-// exp2(-r') reaches 2^|x|^2: 2^27 at this bound, far from FP32 overflow).  The gradient's
+// exp2(-r') reaches 2^|x|^2: 2^48 at this bound, far from FP32 overflow).  The gradient's
 // x sum(t) - sum(t y) also loses more to cancellation the farther a target sits from the
-// centre.  A bound of 8 ran the varifold sum of 1M samples of the unit sphere at sigma 0.02
-// in 1089 against 1194 ms (a third of its Hilbert-ordered groups span more than 6 units), but
-// the 64-subject atlas gradient's template part then differed from FP64 by 1.28e-6 (7:
-// 1.14e-6, 6: 9.6e-7; FP32C 6.0e-7).  1M uniform points at sigma 0.05 have no group above 6.
-constexpr double kBfExtent = 6.0;
-// Ragged batches: the 64-subject atlas's varifold gradient sums (OP_VARIFOLD, weights
-// K (u_i . u_j)^2) put its template gradient 1.08e-6 from FP64 at 6 and 7.7e-7 at 5 (FP32C
-// 7.5e-7; eager gradient 82.8 against 79.2 ms), while the single 1M-sample varifold call
-// runs 953 ms at 6 against 1076 ms at 5 with errors of 2.5e-7 and 2.0e-7.
+// centre.  Single calls (Hilbert-sorted): the 1M-sample sphere varifold at sigma 0.02
+// (34% of its groups span more than 6 units, 12% more than 8) runs its match + gradient in
+// 810 ms at 8 against 905 at 7 and 950 at 6, with gradient errors against FP64 of 3.2e-7,
+// 2.7e-7 and 2.5e-7 (centres; normals 2.5e-7, 2.0e-7, 1.8e-7).  1M uniform points at
+// sigma 0.05 have no group above 6.
+constexpr double kBfExtent = 8.0;
+// Ragged batches (the caller's order): the 64-subject atlas's varifold gradient sums
+// (OP_VARIFOLD, weights K (u_i . u_j)^2) put its template gradient 1.08e-6 from FP64 at 6
+// and 7.7e-7 at 5 (FP32C 7.5e-7; eager gradient 82.8 against 79.2 ms).
 constexpr double kBfExtentBatch = 5.0;
 
 // |a0|_F < 1e-20 leaves the points where they are (Diffeos.cxx:609-611, 684-685)
