@@ -136,8 +136,16 @@ struct KParams {
 };
 
 // Work decomposition and workspace layout of one call.
+// Wavefronts per workgroup of the small-grid form of k_pairsum (FP32, one target per lane):
+// the W wavefronts of a workgroup share its 64 targets and take W consecutive source
+// splits, whose sums are added in LDS (wavefront order, FP64) before one store, so a grid
+// of few target workgroups fills the chip without split partials and their reduction
+// launch.
+constexpr int kWaveSplit = 16;
+
 struct Plan {
   int R;            // targets per lane
+  int W;            // wavefronts per workgroup sharing its targets (FP32, R = 1: kWaveSplit)
   int mx;           // FP32C: accumulate on the matrix cores (k_pairsum_mx), R = 1
   int nseg;         // segments of a ragged batch (grid.z)
   bool sort;        // Hilbert-order the targets first (single-segment FP32 calls)

@@ -355,7 +355,25 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Choice {
   int R, waves, min_tiles;
   int mx = 0;  // FP32C fused Convolve + Grad1 on the matrix cores (R = 1)
+  int w = 1;   // wavefronts per workgroup on the same targets (kWaveSplit: FP32, R = 1)
 };
+
+// FP32 grids of fewer target workgroups than this may run k_pairsum's small-grid form
+// (autotuner candidates): kWaveSplit wavefronts to a workgroup on the same targets, so the
+// source splits are added in LDS instead of in split partials.  It pays where it allows
+// more wavefronts than split partials would: the 4,096-point registration's fused shooting
+// step (64 target workgroups) takes 0.025 ms with 16 one-tile splits per workgroup and 4
+// workgroup splits against 0.033 ms with 16 four-tile splits; with 160 or 320 workgroups
+// (its flow and data-term sums) it lost 3-25% to the one-wavefront form, whose 1,024-lane
+// workgroups fill fewer CUs.
+constexpr long kWaveSplitGrid = 1024;
+
+// the ops of the small-grid form: those of the expanded-distance form with up to 3 weights
+// (one target per lane fits the 128 VGPRs of a 1,024-lane workgroup without spills)
+static bool wave_split_op(int op, int DV) {
+  return DV <= 3 && (op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1 ||
+                     op == OP_VARIFOLD || op == OP_VARIFOLD_CONV);
+}
 
 // Heuristic, from the autotuner's measurements on the shapes of the workloads: one target
 // per lane (64-target wavefronts: the fewest registers, the most wavefronts per SIMD and
@@ -403,6 +421,11 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
   // one-tile splits against 0.030 ms with four-tile ones (FP64: 0.021 against 0.053 ms);
   // from 64 workgroups up (a 4,096-point registration's sums) one-tile FP32 splits cost
   // more in split partials than they gain: 16.4 against 11.8 ms for that gradient
+  // the small-grid form where it pays (see kWaveSplitGrid), aiming for 256 workgroups: the
+  // registration's 4,096 x 8,192 fused sum takes 0.040 ms so against 0.054 ms aiming for
+  // 1,024 workgroups of one-tile splits
+  if (prec != DFM_FP64 && wave_split_op(op, DV) && blocks >= 32 && blocks < 128)
+    return Choice{1, 4096, 1, 0, kWaveSplit};
   const int min_tiles = (blocks < 64 || (prec == DFM_FP64 && blocks < 1024)) ? 1 : 4;
   return Choice{1, blocks >= 4096 ? 262144 : 16384, min_tiles};
 }
@@ -424,7 +447,8 @@ static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg,
   if (Rmax >= 2) Rs.push_back(1);
   std::vector<Choice> out{d};
   auto same = [](const Choice &a, const Choice &b) {
-    return a.R == b.R && a.waves == b.waves && a.min_tiles == b.min_tiles && a.mx == b.mx;
+    return a.R == b.R && a.waves == b.waves && a.min_tiles == b.min_tiles && a.mx == b.mx &&
+           a.w == b.w;
   };
   // long grids: finer splits shorten the tail further (one target per lane only, so the
   // split partials of the largest candidate, and the workspace, do not grow)
@@ -442,6 +466,12 @@ static std::vector<Choice> candidates(int op, int prec, int D, int DV, int nseg,
         Choice c{1, waves, mt, 1};
         if (!same(c, d)) out.push_back(c);
       }
+  if (prec != DFM_FP64 && blocks < kWaveSplitGrid && wave_split_op(op, DV))
+    for (int waves : {4096, 16384, 65536})
+      for (int mt : {4, 1}) {
+        Choice c{1, waves, mt, 0, kWaveSplit};
+        if (!same(c, d)) out.push_back(c);
+      }
   return out;
 }
 
@@ -453,6 +483,9 @@ static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long
   (void)op; (void)D; (void)DV;
   pl.R = c.mx ? 1 : c.R;
   pl.mx = c.mx;
+  pl.W = (c.w > 1 && !c.mx && pl.R == 1 && prec != DFM_FP64 && wave_split_op(op, DV))
+             ? kWaveSplit
+             : 1;
   pl.nseg = nseg;
   pl.chunks = (max_nx + 64L * pl.R - 1) / (64L * pl.R);
   const int TS = tile_of(prec);
@@ -462,6 +495,15 @@ static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long
   long maxsplit = std::max(1L, tiles / c.min_tiles);
   long ns = std::max(1L, std::min(want, maxsplit));
   ns = std::min(ns, 65535L);
+  if (pl.W > 1) {
+    // ns wavefront splits, W to a workgroup: ceil(ns / W) workgroup splits of W * split_len
+    const long nsw = (ns + pl.W - 1) / pl.W;
+    const long per = (tiles + nsw * pl.W - 1) / (nsw * pl.W);
+    pl.split_len = std::max(1L, per) * TS;
+    const long span = pl.split_len * pl.W;
+    pl.nsplit = (int)std::max(1L, (max_ny + span - 1) / span);
+    return;
+  }
   const long per = (tiles + ns - 1) / ns;
   pl.split_len = std::max(1L, per) * TS;
   pl.nsplit = (int)std::max(1L, (max_ny + pl.split_len - 1) / pl.split_len);
@@ -671,16 +713,20 @@ int tune_report(char *buf, size_t len) {
     const Choice &w = t.cand[t.done ? t.winner : 0];
     snprintf(line, sizeof(line),
              "op=%s prec=%d d=%d dv=%d batch=%d nx~2^%d ny~2^%d grid=%d -> R=%d waves=%d "
-             "min_tiles=%d mx=%d %s %.4f ms (default %.4f ms) |",
+             "min_tiles=%d mx=%d w=%d %s %.4f ms (default %.4f ms) |",
              k.op >= 0 && k.op < nops ? ops[k.op] : "?", k.prec, k.D, k.DV, k.batch, k.lx, k.ly,
-             k.grid, w.R, w.waves, w.min_tiles,
-             w.mx,
+             k.grid, w.R, w.waves, w.min_tiles, w.mx, w.w,
              t.done ? "tuned" : "exploring", t.done ? t.best_ms[t.winner] : 0.0,
              t.best_ms.empty() ? 0.0 : t.best_ms[0]);
     out += line;
     for (size_t i = 0; i < t.cand.size(); i++) {
-      snprintf(line, sizeof(line), " %d/%d/%d/%d:%.3f", t.cand[i].R, t.cand[i].waves,
-               t.cand[i].min_tiles, t.cand[i].mx, t.best_ms[i] < 3e38f ? t.best_ms[i] : -1.0);
+      const double ms = t.best_ms[i] < 3e38f ? t.best_ms[i] : -1.0;
+      if (t.cand[i].w > 1)
+        snprintf(line, sizeof(line), " %d/%d/%d/%d/w%d:%.3f", t.cand[i].R, t.cand[i].waves,
+                 t.cand[i].min_tiles, t.cand[i].mx, t.cand[i].w, ms);
+      else
+        snprintf(line, sizeof(line), " %d/%d/%d/%d:%.3f", t.cand[i].R, t.cand[i].waves,
+                 t.cand[i].min_tiles, t.cand[i].mx, ms);
       out += line;
     }
     out += "\n";

@@ -13,7 +13,7 @@ void launch_mx(hipStream_t st, const KParams &p, const Plan &pl) {
   hipLaunchKernelGGL((k_pairsum_mx<OP, DV>), grid, dim3(64), 0, st, p);
   KParams q = p;
   q.wave_filter = 1;
-  hipLaunchKernelGGL((k_pairsum<float, OP, 3, DV, 0, 1, MODE_ADAPTIVE>), grid, dim3(64), 0, st, q);
+  hipLaunchKernelGGL((k_pairsum<float, OP, 3, DV, 0, 1, MODE_ADAPTIVE, 1>), grid, dim3(64), 0, st, q);
   if (pl.nsplit > 1)
     hipLaunchKernelGGL((k_reduce_splits<OP, 3, DV>), dim3((unsigned)((p.nx + 255) / 256)),
                        dim3(256), 0, st, p);

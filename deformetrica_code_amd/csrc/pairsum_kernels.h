@@ -42,6 +42,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "dfm_hip.h"
 #include "dfm_internal.h"
@@ -413,12 +414,15 @@ __device__ __forceinline__ double wave_max(double v) {
 // ------------------------------------------------------------------------------------
 // the pair-sum kernel
 
-// Stream the sources [js, je) through the wavefront's LDS tile and accumulate.  EXPAND
-// selects the expanded-distance form (pair_update_expand; FP32 only).
+// Stream the sources [js, je) through the wavefront's LDS tile and accumulate, in `ntiles`
+// tiles (at least enough for [js, je); the wavefronts of a workgroup pass the same number of
+// barriers, so they take the same count, and tiles past je stage padding and sum nothing).
+// EXPAND selects the expanded-distance form (pair_update_expand; FP32 only).
 template <typename T, int OP, int D, int DV, int AUX, int R, bool EXACT, bool EXPAND, int NTE,
-          int NA>
+          int NA, bool PF>
 __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, long js, long je,
-                                          const double *c, double s, const T (&xt)[R][D],
+                                          long ntiles, const double *c, double s,
+                                          const T (&xt)[R][D],
                                           const T (&xl)[R][D], const T (&te)[R][NTE],
                                           double (&accd)[R][NA]) {
   using S = Shape<OP, D, DV>;
@@ -445,7 +449,38 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
 #pragma unroll
     for (int k = 0; k < D; k++) xm2[r][k] = T(-2) * xt[r][k];
 
-  for (long j0 = js; j0 < je; j0 += TS) {
+  // PF (the small-grid form): one-record-per-lane tiles read the next tile's raw FP64
+  // sources into registers before summing the current one, so the global loads are in
+  // flight during the sums, where few wavefronts per SIMD would not hide them.  The
+  // one-wavefront form leaves it off: its registers would cost the large grids occupancy
+  // (the fused FP32C sum at N = M = 1M: 332 against 306 ms).
+  constexpr int NRAW = S::NSRC;
+  constexpr bool PREFETCH = PF && REC_PER_LANE == 1 && NRAW <= 9;
+  double raw[PREFETCH ? NRAW : 1];
+  auto fetch = [&](long j) {
+    if constexpr (PREFETCH) {
+      if (j < je) {
+#pragma unroll
+        for (int k = 0; k < D; k++) raw[k] = p.y[j * D + k];
+#pragma unroll
+        for (int k = 0; k < DV; k++) raw[D + k] = p.w[j * DV + k];
+        if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
+#pragma unroll
+          for (int k = 0; k < D; k++) raw[D + DV + k] = p.se[j * D + k];
+        }
+        if constexpr (OP == OP_SHOOTVJP) {
+#pragma unroll
+          for (int k = 0; k < D; k++) raw[D + DV + D + k] = p.se2[j * D + k];
+        }
+      }
+    }
+  };
+  if constexpr (PREFETCH) {
+    if (ntiles > 0) fetch(js + lane);
+  }
+
+  for (long t = 0; t < ntiles; t++) {
+    const long j0 = js + t * TS;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < REC_PER_LANE; q++) {
@@ -454,22 +489,33 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
 #pragma unroll
       for (int k = 0; k < NSP; k++) rec[k] = T(0);
       if (j < je) {
+        // the record's raw values: prefetched, or read now
+        double v[NRAW];
+        if constexpr (PREFETCH) {
+#pragma unroll
+          for (int k = 0; k < NRAW; k++) v[k] = raw[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < D; k++) v[k] = p.y[j * D + k];
+#pragma unroll
+          for (int k = 0; k < DV; k++) v[D + k] = p.w[j * DV + k];
+          if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
+#pragma unroll
+            for (int k = 0; k < D; k++) v[D + DV + k] = p.se[j * D + k];
+          }
+          if constexpr (OP == OP_SHOOTVJP) {
+#pragma unroll
+            for (int k = 0; k < D; k++) v[D + DV + D + k] = p.se2[j * D + k];
+          }
+        }
 #pragma unroll
         for (int k = 0; k < D; k++) {
-          const double v = (p.y[j * D + k] - c[k]) * s;
-          rec[k] = (T)v;
-          if constexpr (EXACT) rec[LO + k] = (T)(v - (double)rec[k]);
+          const double u = (v[k] - c[k]) * s;
+          rec[k] = (T)u;
+          if constexpr (EXACT) rec[LO + k] = (T)(u - (double)rec[k]);
         }
 #pragma unroll
-        for (int k = 0; k < DV; k++) rec[D + k] = (T)p.w[j * DV + k];
-        if constexpr (OP == OP_GRADDIFF || OP == OP_SHOOTVJP) {
-#pragma unroll
-          for (int k = 0; k < D; k++) rec[D + DV + k] = (T)p.se[j * D + k];
-        }
-        if constexpr (OP == OP_SHOOTVJP) {
-#pragma unroll
-          for (int k = 0; k < D; k++) rec[D + DV + D + k] = (T)p.se2[j * D + k];
-        }
+        for (int k = D; k < NRAW; k++) rec[k] = (T)v[k];
       } else {
 #pragma unroll
         for (int k = 0; k < D; k++) rec[k] = far;
@@ -485,10 +531,13 @@ __device__ __forceinline__ void run_tiles(const KParams &p, T *sm, int lane, lon
       for (int k = 0; k < NSP; k++) dst[k] = rec[k];
     }
     __syncthreads();
+    if constexpr (PREFETCH) {
+      if (t + 1 < ntiles) fetch(j0 + TS + lane);
+    }
 
     const long left = je - j0;
     // records past the end of the sources were staged with zero weight far away
-    const int cnt = (int)(left < TS ? ((left + FOLD - 1) / FOLD) * FOLD : TS);
+    const int cnt = (int)(left <= 0 ? 0 : left < TS ? ((left + FOLD - 1) / FOLD) * FOLD : TS);
     for (int jf = 0; jf < cnt; jf += FOLD) {
       for (int jj = jf; jj < jf + FOLD; jj += 4) {
 #pragma unroll
@@ -556,22 +605,15 @@ __device__ __forceinline__ void load_targets(const KParams &p, int lane, long ba
 }
 
 // The sums of one wavefront's targets over the sources [js, je) streamed through its LDS
-// tile `sm`, stored through finalize (one split) or as split partials.  Every lane of the
-// wavefront takes part (invalid lanes sit at the centre and store nothing), and the number
-// of __syncthreads() depends on js, je only, so the wavefronts of a larger workgroup that
-// call this with the same source range pass the same barriers.
-//
-// With a fused integrator step (p.e_on, OP_CONVGRAD1) each target's sums are finished and
-// the step applied here: directly with one split, else by the wavefront of target tile
-// `tile` (segment `seg`) that is the last of the p.nsplit splits to store its partials.
-template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
-__device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane, int split,
-                                              long js, long je, const long (&pos)[R],
-                                              const long (&orig)[R], const bool (&valid)[R],
-                                              const double (&xd)[R][D], const double (&c)[D],
-                                              double extent, long tile, int seg) {
+// tile `sm` in `ntiles` tiles, into accd (FP64, the layout finalize takes).  Every lane of
+// the wavefront takes part (invalid lanes sit at the centre and their sums are discarded).
+template <typename T, int OP, int D, int DV, int AUX, int R, int MODE, bool PF>
+__device__ __forceinline__ void wave_sums(const KParams &p, T *sm, int lane, long js, long je,
+                                          long ntiles, const long (&orig)[R],
+                                          const bool (&valid)[R], const double (&xd)[R][D],
+                                          const double (&c)[D], double extent,
+                                          double (&accd)[R][Shape<OP, D, DV>::NACCMAX]) {
   using S = Shape<OP, D, DV>;
-  constexpr int NACC = S::NACC;
   constexpr int NA = S::NACCMAX;
   constexpr int NTE = S::NTE > 0 ? S::NTE : 1;
   const double s = p.s;
@@ -605,7 +647,6 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
     }
   }
 
-  double accd[R][NA];
 #pragma unroll
   for (int r = 0; r < R; r++)
 #pragma unroll
@@ -616,23 +657,35 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
     // widths lose too much to FP32 rounding, so such wavefronts form exact differences
     const bool exact = MODE == MODE_EXACT || extent * s > p.exact_extent;
     if (exact) {
-      run_tiles<T, OP, D, DV, AUX, R, true, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
-                                                            accd);
+      run_tiles<T, OP, D, DV, AUX, R, true, false, NTE, NA, PF>(p, sm, lane, js, je, ntiles, c, s,
+                                                                xt, xl, te, accd);
     } else if constexpr (S::EXPANDABLE) {
-      run_tiles<T, OP, D, DV, AUX, R, false, true, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
-                                                            accd);
+      run_tiles<T, OP, D, DV, AUX, R, false, true, NTE, NA, PF>(p, sm, lane, js, je, ntiles, c, s,
+                                                                xt, xl, te, accd);
 #pragma unroll
       for (int r = 0; r < R; r++) expand_finish<OP, D, DV, NA>(accd[r], xt[r], te[r]);
     } else {
-      run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
-                                                             accd);
+      run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA, PF>(p, sm, lane, js, je, ntiles, c,
+                                                                 s, xt, xl, te, accd);
     }
   } else {
-    run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA>(p, sm, lane, js, je, c, s, xt, xl, te,
-                                                           accd);
+    run_tiles<T, OP, D, DV, AUX, R, false, false, NTE, NA, PF>(p, sm, lane, js, je, ntiles, c, s,
+                                                               xt, xl, te, accd);
   }
+}
 
-  // ---- results
+// A wavefront's finished sums (split `split` of p.nsplit), stored through finalize (one
+// split) or as split partials.  With a fused integrator step (p.e_on, OP_CONVGRAD1) each
+// target's sums are finished and the step applied here: directly with one split, else by
+// the wavefront of target tile `tile` (segment `seg`) that is the last of the p.nsplit
+// splits to store its partials.
+template <int OP, int D, int DV, int R>
+__device__ __forceinline__ void store_sums(const KParams &p, int lane, int split,
+                                           const long (&pos)[R], const long (&orig)[R],
+                                           const bool (&valid)[R],
+                                           double (&accd)[R][Shape<OP, D, DV>::NACCMAX],
+                                           long tile, int seg) {
+  constexpr int NACC = Shape<OP, D, DV>::NACC;
 #pragma unroll
   for (int r = 0; r < R; r++) {
     if (!valid[r]) continue;
@@ -683,18 +736,45 @@ __device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane,
   }
 }
 
+// wave_sums over [js, je) and store_sums.  The number of __syncthreads() depends on js, je
+// only, so the wavefronts of a larger workgroup that call this with the same source range
+// pass the same barriers.
 template <typename T, int OP, int D, int DV, int AUX, int R, int MODE>
-__global__ __launch_bounds__(64) void k_pairsum(KParams p) {
+__device__ __forceinline__ void sum_wavefront(const KParams &p, T *sm, int lane, int split,
+                                              long js, long je, const long (&pos)[R],
+                                              const long (&orig)[R], const bool (&valid)[R],
+                                              const double (&xd)[R][D], const double (&c)[D],
+                                              double extent, long tile, int seg) {
+  constexpr int TS = Prec<T>::kTile;
+  double accd[R][Shape<OP, D, DV>::NACCMAX];
+  const long ntiles = je > js ? (je - js + TS - 1) / TS : 0;
+  wave_sums<T, OP, D, DV, AUX, R, MODE, false>(p, sm, lane, js, je, ntiles, orig, valid, xd, c,
+                                               extent, accd);
+  store_sums<OP, D, DV, R>(p, lane, split, pos, orig, valid, accd, tile, seg);
+}
+
+// One workgroup = W wavefronts on the same 64 R targets (W = 1: one wavefront, the form of
+// the large grids).  grid = (target chunks, source splits of W wavefronts each, segments).
+template <typename T, int OP, int D, int DV, int AUX, int R, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void k_pairsum(KParams p) {
   using S = Shape<OP, D, DV>;
   constexpr int TS = Prec<T>::kTile;
   constexpr int FOLD = Prec<T>::kFold;
+  constexpr int NA = S::NACCMAX;
+  constexpr int NACC = S::NACC;
   static_assert(TS % FOLD == 0 && FOLD % 4 == 0, "fold interval divides the tile");
   static_assert(S::NSP % 4 == 0 && S::NSPX % 4 == 0, "records are whole 16-byte groups");
+  static_assert(W == 1 || R == 1, "several wavefronts per workgroup: one target per lane");
+  constexpr int TILE = TS * (Prec<T>::kCenter ? S::NSPX : S::NSP);  // T per wavefront tile
+  constexpr int RCH = 4;  // accumulators per round of the workgroup's LDS sum
+  constexpr size_t RED = W > 1 ? sizeof(double) * W * 64 * RCH : 0;
+  constexpr size_t BYTES = sizeof(T) * W * TILE > RED ? sizeof(T) * W * TILE : RED;
 
-  __shared__ __attribute__((aligned(16))) T sm[TS * (Prec<T>::kCenter ? S::NSPX : S::NSP)];
+  __shared__ __attribute__((aligned(16))) unsigned char smraw[BYTES];
 
-  const int lane = threadIdx.x;
-  const int split = blockIdx.y;
+  const int wave = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+  const int lane = W > 1 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  T *sm = reinterpret_cast<T *>(smraw) + wave * TILE;
   // segment of a ragged batch (one segment: the whole target and source sets)
   long xb = 0, xe = p.nx, yb = 0, ye = p.ny;
   if (p.xoff) {
@@ -706,7 +786,7 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
     ye = p.yoff[blockIdx.z + 1];
   }
   const long base = xb + (long)blockIdx.x * (64 * R);
-  if (base >= xe) return;  // the whole wavefront lies past its segment
+  if (base >= xe) return;  // the whole workgroup lies past its segment
 
   long pos[R], orig[R];
   bool valid[R];
@@ -716,11 +796,42 @@ __global__ __launch_bounds__(64) void k_pairsum(KParams p) {
     // the compact wavefronts of this call run on the matrix cores (k_pairsum_mx)
     if (p.wave_filter == 1 && !(extent * p.s > p.exact_extent)) return;
   }
-  const long js = yb + (long)split * p.split_len;
-  const long je = std::min(ye, js + p.split_len);
-  sum_wavefront<T, OP, D, DV, AUX, R, MODE>(p, sm, lane, split, js, je, pos, orig, valid, xd, c,
-                                            extent, (long)blockIdx.z * gridDim.x + blockIdx.x,
-                                            (int)blockIdx.z);
+  const long tile = (long)blockIdx.z * gridDim.x + blockIdx.x;
+  if constexpr (W == 1) {
+    const long js = yb + (long)blockIdx.y * p.split_len;
+    const long je = std::min(ye, js + p.split_len);
+    sum_wavefront<T, OP, D, DV, AUX, R, MODE>(p, sm, lane, (int)blockIdx.y, js, je, pos, orig,
+                                              valid, xd, c, extent, tile, (int)blockIdx.z);
+  } else {
+    // every wavefront runs the tiles of a whole split (the workgroup's barriers), those past
+    // its segment's sources staging padding
+    const long js = yb + ((long)blockIdx.y * W + wave) * p.split_len;
+    const long je = std::min(ye, js + p.split_len);
+    const long ntiles = (p.split_len + TS - 1) / TS;
+    double accd[R][NA];
+    wave_sums<T, OP, D, DV, AUX, R, MODE, true>(p, sm, lane, js, je, ntiles, orig, valid, xd, c,
+                                                extent, accd);
+    double *red = reinterpret_cast<double *>(smraw);
+    for (int a0 = 0; a0 < NACC; a0 += RCH) {
+      __syncthreads();  // the tiles, or the previous round's sums, are consumed
+#pragma unroll
+      for (int q = 0; q < RCH; q++)
+        if (a0 + q < NACC) red[(wave * RCH + q) * 64 + lane] = accd[0][a0 + q];
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < RCH; q++) {
+          if (a0 + q >= NACC) continue;
+          double v = 0.0;
+          for (int w = 0; w < W; w++) v += red[(w * RCH + q) * 64 + lane];
+          accd[0][a0 + q] = v;
+        }
+      }
+    }
+    if (wave != 0) return;
+    store_sums<OP, D, DV, R>(p, lane, (int)blockIdx.y, pos, orig, valid, accd, tile,
+                             (int)blockIdx.z);
+  }
 }
 
 template <int OP, int D, int DV>
@@ -746,10 +857,20 @@ __global__ __launch_bounds__(256) void k_reduce_splits(KParams p) {
 template <typename T, int MODE, int OP, int D, int DV, int AUX>
 static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
   dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit, (unsigned)pl.nseg);
+  if constexpr (std::is_same<T, float>::value && Shape<OP, D, DV>::EXPANDABLE && DV <= 3) {
+    if (pl.W > 1) {  // the plan holds R = 1 and W = kWaveSplit for these ops
+      hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE, kWaveSplit>), grid,
+                         dim3(64 * kWaveSplit), 0, st, p);
+      if (pl.nsplit > 1 && !p.done)
+        hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
+                           dim3(256), 0, st, p);
+      return;
+    }
+  }
   switch (pl.R) {
-    case 1: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE>), grid, dim3(64), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE>), grid, dim3(64), 0, st, p); break;
-    default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4, MODE>), grid, dim3(64), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE, 1>), grid, dim3(64), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 2, MODE, 1>), grid, dim3(64), 0, st, p); break;
+    default: hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 4, MODE, 1>), grid, dim3(64), 0, st, p); break;
   }
   // with completion counters k_pairsum finishes its own source splits
   if (pl.nsplit > 1 && !p.done)

@@ -92,7 +92,7 @@ def test_every_explored_plan_is_correct(tuner, oracle_mod, n, m):
     tuner.autotune(1)
     for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6), ("bf16mfma", 1e-6)):
         for k, f in ops.items():
-            for _ in range(50):           # > twice the largest candidate count
+            for _ in range(64):           # > twice the largest candidate count (28)
                 got = f(prec)
                 if isinstance(got, list):
                     for u, v in zip(got, refs[k]):

@@ -117,6 +117,38 @@ def test_bf16_batch_below_the_source_bound_equals_fp32c():
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", ["fp32c", "fp32x"])
+@pytest.mark.parametrize("m", [1, 100, 4096, 50000])
+def test_small_grid_form_matches_fp64(prec, m):
+    """FP32 calls on 32-127 target workgroups run sixteen wavefronts per workgroup on the
+    same 64 targets, their source splits added in LDS: single calls of 4,096 targets and a
+    batch of two segments (2,500 and 4,000 targets) against FP64, Convolve, Grad1 and the
+    fused sum, norm-wise 1e-6."""
+    from deformetrica_code_amd import device
+    rng = np.random.default_rng(m)
+    n = 4096
+    X, A = rng.uniform(size=(n, 3)), rng.normal(size=(n, 3))
+    Y, W = rng.uniform(size=(m, 3)), rng.normal(size=(m, 3))
+    dX, dA, dY, dW = (device.DeviceArray.from_numpy(v) for v in (X, A, Y, W))
+    for conv, grad in ((True, True), (True, False), (False, True)):
+        ref = device.conv_grad1(dX, dA, dY, dW, 0.1, "fp64", conv=conv, grad=grad)
+        got = device.conv_grad1(dX, dA, dY, dW, 0.1, prec, conv=conv, grad=grad)
+        for a, b in zip(got, ref):
+            if b is not None:
+                assert rel(a.numpy(), b.numpy()) < 1e-6, (conv, grad)
+    xo, yo = np.array([0, 2500, n]), np.array([0, m // 3, m])
+    ref = device.batch_conv_grad1(dX, dA, dY, dW, xo, yo, 0.1, "fp64")
+    got = device.batch_conv_grad1(dX, dA, dY, dW, xo, yo, 0.1, prec)
+    for s in range(2):
+        xs = slice(xo[s], xo[s + 1])
+        for a, b in zip(got, ref):
+            a, b = a.numpy()[xs], b.numpy()[xs]
+            if yo[s + 1] == yo[s]:
+                np.testing.assert_array_equal(a, 0.0)
+            else:
+                assert rel(a, b) < 1e-6, s
+
+
 def _subjects(seed, counts_cp, counts_lm, d=3, scale=0.2):
     rng = np.random.default_rng(seed)
     p0 = [rng.uniform(size=(n, d)) for n in counts_cp]
