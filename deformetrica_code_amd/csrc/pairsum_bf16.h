@@ -26,8 +26,10 @@
 //
 // A group whose targets span more than kBfExtent scaled units (kBfExtentBatch in ragged
 // batches of several segments; none of the Hilbert-ordered groups of 1M uniform points at
-// sigma 0.05; 34% of those of 1M samples of a sphere at sigma 0.02) is summed in the same
-// workgroup by the DFM_FP32C wavefront code (sum_wavefront), one wavefront per 64 targets.
+// sigma 0.05; 12% of those of 1M samples of a sphere at sigma 0.02) runs the same sweep on
+// one centre per wavefront when each wavefront's 64 targets fit the bound (each wavefront
+// staging the sources on its own centre, 64 to a tile), else the DFM_FP32C wavefront code
+// (sum_wavefront) in the same workgroup, one wavefront per 64 targets.
 // Ragged batches (grid.z = segment) keep the caller's target order: the groups of a
 // triangle mesh's faces in mesh order are as compact as Hilbert-ordered ones.  Same
 // accumulator layout, split partials and finalize as k_pairsum: k_reduce_splits is shared.
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
   __shared__ __attribute__((aligned(16))) bf16x8 smW2[NDOT == 2 ? TS : 1];
   __shared__ __attribute__((aligned(16))) float smV[TS * NV];
   __shared__ double sbox[4][2][3];
+  __shared__ int sok[4];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -211,7 +214,35 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     extent = fmax(extent, hi[k] - lo[k]);
   }
   const double s = p.s;
+  // A group too wide for one centre: if each of its four wavefronts is compact on a centre
+  // of its own, every wavefront runs the matrix-core sweep on that centre over tiles of 64
+  // sources that it stages itself (into its own rows of the tile arrays); otherwise the
+  // FP32C code below.  Either way the choice is the workgroup's (one barrier sequence).
+  bool per_wave = false;
   if (!(extent * s <= p.exact_extent_bf)) {
+    double wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const long wp = gbase + 64 * wave + lane;
+    if (wp < gend) {
+      const long o = p.perm ? (long)p.perm[wp] : wp;
+#pragma unroll
+      for (int k = 0; k < D; k++) wlo[k] = whi[k] = p.x[o * D + k];
+    }
+    double wext = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      wlo[k] = wave_min(wlo[k]);
+      whi[k] = wave_max(whi[k]);
+      if (wlo[k] <= whi[k]) wext = fmax(wext, whi[k] - wlo[k]);
+    }
+    if (lane == 0) sok[wave] = wext * s <= p.exact_extent_bf;
+    __syncthreads();
+    per_wave = sok[0] && sok[1] && sok[2] && sok[3];
+    if (per_wave) {
+#pragma unroll
+      for (int k = 0; k < D; k++) c[k] = (wlo[k] <= whi[k]) ? 0.5 * (wlo[k] + whi[k]) : 0.0;
+    }
+  }
+  if (!per_wave && !(extent * s <= p.exact_extent_bf)) {
     // a wide group (its sorted targets straddle a fold of the Hilbert curve): each wavefront
     // sums its 64 targets as DFM_FP32C does, on its own centre (exact differences where its
     // own targets are wide), with a slice of smA as its LDS tile.  All four wavefronts pass
@@ -279,11 +310,18 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
 
   const long js = yb + (long)split * p.split_len;
   const long je = std::min(ye, js + p.split_len);
-  for (long j0 = js; j0 < je; j0 += TS) {
+  // PW: tiles of 64 sources, each wavefront staging the same sources on its own centre into
+  // its rows jb .. jb + 63 (row tid in both forms)
+  auto sweep = [&](auto pw) {
+  constexpr bool PW = decltype(pw)::value;
+  constexpr int TSL = PW ? 64 : TS;
+  const int jb = PW ? 64 * wave : 0;
+  for (long j0 = js; j0 < je; j0 += TSL) {
     __syncthreads();
-    // ---- stage source j0 + tid: A rows (BF16 splits in slot order) and FP32 y, w
+    // ---- stage source j0 + tid (j0 + lane per wavefront): A rows (BF16 splits in slot
+    // order) and FP32 y, w
     {
-      const long j = j0 + tid;
+      const long j = j0 + (PW ? lane : tid);
       float yf[D], w[DV];
       double n2 = 0.0;
       if (j < je) {
@@ -347,32 +385,32 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
     __syncthreads();
 
     const long left = je - j0;
-    const int ngrp = (int)(left < TS ? (left + 15) / 16 : TS / 16);
+    const int ngrp = (int)(left < TSL ? (left + 15) / 16 : TSL / 16);
     float acc[NT][NU];
     // 16 sources of the tile; the first group (FIRST) sets the partial sums instead of
     // adding to zero (the same values: an FMA into 0 rounds as the product does)
     auto group = [&](int g, auto first) {
       constexpr bool FIRST = decltype(first)::value;
       // r' of sources 16 g + 4 h + v against each tile's column n (and a . w)
-      const bf16x8 a = smA[h * TS + 16 * g + n];
+      const bf16x8 a = smA[h * TS + jb + 16 * g + n];
       floatx4 r[NT];
 #pragma unroll
       for (int t = 0; t < NT; t++) r[t] = bf_mfma(a, bop[t]);
       floatx4 dt[DOT ? NT : 1];
       if constexpr (DOT) {
-        const bf16x8 aw = smW[h * TS + 16 * g + n];
+        const bf16x8 aw = smW[h * TS + jb + 16 * g + n];
 #pragma unroll
         for (int t = 0; t < NT; t++) dt[t] = bf_mfma(aw, bdot[t][0]);
         if constexpr (NDOT == 2) {
           // slots 32..39 (every lane reads them; the B columns of lane groups h > 0 are zero)
-          const bf16x8 aw2 = smW2[16 * g + n];
+          const bf16x8 aw2 = smW2[jb + 16 * g + n];
 #pragma unroll
           for (int t = 0; t < NT; t++) dt[t] = bf_mfma_acc(aw2, bdot[t][1], dt[t]);
         }
       }
       // FP32 y, w of sources 16 g + 4 h + v, v = 0..3 (4 NV consecutive floats)
       float sv[4][NV];
-      const float4 *src4 = reinterpret_cast<const float4 *>(smV + (16 * g + 4 * h) * NV);
+      const float4 *src4 = reinterpret_cast<const float4 *>(smV + (jb + 16 * g + 4 * h) * NV);
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         const float4 f4 = src4[q];
@@ -422,6 +460,11 @@ __global__ __launch_bounds__(256, 3) void k_pairsum_bf(KParams p) {
       accd[a] += (double)mine;
     }
   }
+  };
+  if (per_wave)
+    sweep(std::true_type{});
+  else
+    sweep(std::false_type{});
 
   // ---- lane (h, n) finishes and stores target tile h, column n (lane groups h >= NT hold
   // no tile)
