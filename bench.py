@@ -57,7 +57,7 @@ FORMS = {
     "bf16mfma": "squared distances' cross term -2 x.y + |y|^2 on the matrix cores "
                 "(v_mfma_f32_16x16x32_bf16 over exact three-part BF16 splits of the FP32 "
                 "group-centred coordinates, FP32 accumulation), target norm applied exactly, "
-                "FP32 exp2 and accumulation with FP64 folds per 256-source tile",
+                "FP32 exp2 and accumulation with FP64 folds per 512 sources",
     "fp32c": "FP32 exp2/FMA on FP64-re-centred coordinates (expanded distance), FP64 folds "
              "per 64-source tile, exact differences in wide wavefronts",
     "fp32x": "FP32 exp2/FMA with exact (hi + lo) coordinate differences in every wavefront",
@@ -66,15 +66,15 @@ FORMS = {
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a 24 MB device copy.
 # Counter runs cannot be made from inside this process, so the figures are the profiled ones.
 PROFILED = {
-    # k_pairsum_bf: FETCH_SIZE 7,689,250 KiB x 1.997 (the copy kernel's 24 MB read counts
-    # 11,734 KiB) = 15.73 GB, WRITE_SIZE 796,877 KiB x 1.0 = 0.82 GB (the FP64 split partials)
-    "bf16mfma": {"traffic": 15.73e9 + 0.82e9,
+    # k_pairsum_bf: FETCH_SIZE 7,688,425 KiB x 1.997 (the copy kernel's 24 MB read counts
+    # 11,735 KiB) = 15.72 GB, WRITE_SIZE 796,878 KiB x 1.0 = 0.82 GB (the FP64 split partials)
+    "bf16mfma": {"traffic": 15.72e9 + 0.82e9,
                  "source": "profiles/k_pairsum_bf_convgrad1_N1M_sigma0.05.md",
                  "note": "VALU-issue bound with matrix-core co-issue: two "
                          "v_mfma_f32_16x16x32_bf16 per 256 pairs form the distances' cross "
                          "term and alpha.w, the VALU keeps exp2 and the FP32 accumulation "
-                         "(8.7 VALU instructions per pair against 14.7 for fp32c); 16.5 GB per "
-                         "launch is 68 GB/s, under 1% of HBM bandwidth; the peak is the FP32 "
+                         "(8.8 VALU instructions per pair against 14.7 for fp32c); 16.5 GB per "
+                         "launch is 69 GB/s, under 1% of HBM bandwidth; the peak is the FP32 "
                          "vector peak of the 29 algorithmic flops per pair"},
     # k_pairsum<float, ..., R=2>: FETCH_SIZE 2,920,758 KiB x 1.997 = 5.97 GB, WRITE_SIZE
     # 421,875 KiB = 0.43 GB
@@ -107,7 +107,11 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="workload legs: skip the autotuned repeat (for kernel traces of the "
                          "default plans)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps < 1 or args.fp64_steps < 1 or args.warmup < 0:
+        ap.error("--steps and --fp64-steps must be at least 1 (the FP64 step is the error "
+                 "reference), --warmup at least 0")
+    return args
 
 
 def main():
