@@ -24,7 +24,7 @@ def test_batch_conv_grad1_matches_per_segment_calls(prec):
     from deformetrica_code_amd import device
     rng = np.random.default_rng(1)
     nx = [300, 0, 1000, 77, 5000, 1]
-    ny = [400, 200, 0, 1500, 9000, 9]     # DFM_BF16MFMA batches need 8,192 sources somewhere
+    ny = [400, 200, 0, 1500, 9000, 9]     # DFM_BF16MFMA batches need 2,048 sources somewhere
     xo, yo = _offsets(nx), _offsets(ny)
     X = rng.uniform(size=(xo[-1], 3))
     A = rng.normal(size=(xo[-1], 3))
@@ -99,12 +99,12 @@ def test_bf16_batch_on_compact_segments_matches_fp64(op):
 
 
 def test_bf16_batch_below_the_source_bound_equals_fp32c():
-    """A DFM_BF16MFMA batch whose largest segment has fewer than 8,192 sources runs the
+    """A DFM_BF16MFMA batch whose largest segment has fewer than 2,048 sources runs the
     DFM_FP32C kernels (the matrix-core kernel's per-group set-up does not pay off over few
     source tiles): equal bit for bit."""
     from deformetrica_code_amd import device
     rng = np.random.default_rng(8)
-    nx, ny = [9000, 300, 2500], [8191, 4000, 2048]
+    nx, ny = [9000, 300, 2500], [2047, 1000, 512]
     xo, yo = _offsets(nx), _offsets(ny)
     X, A = rng.uniform(size=(xo[-1], 3)), rng.normal(size=(xo[-1], 3))
     Y, W = rng.uniform(size=(yo[-1], 3)), rng.normal(size=(yo[-1], 3))
