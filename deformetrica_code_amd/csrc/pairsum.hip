@@ -316,10 +316,11 @@ static int tile_of(int prec) {
   return prec == DFM_FP64 ? kTileF64 : prec == DFM_BF16MFMA ? kBfTile : kTileF32;
 }
 
-// DFM_BF16MFMA runs the matrix-core kernel for Convolve / Grad1 / fused sums: single calls
-// large enough to be Hilbert-sorted, and ragged batches (nseg > 1, the caller's target
-// order: groups of 256 targets that span too far take the FP32C code inside the kernel)
-// whose largest segment has kBfBatchMinSources sources; everything else runs as DFM_FP32C.
+// DFM_BF16MFMA runs the matrix-core kernel for Convolve / Grad1 / fused sums with at least
+// kBfMinSources sources (in the largest segment): single calls large enough to be
+// Hilbert-sorted, and ragged batches (nseg > 1, the caller's target order: groups of 256
+// targets that span too far take the FP32C code inside the kernel); everything else runs
+// as DFM_FP32C.
 // 64 segments of the 10,242 icosphere vertices (in the order of first use by the faces)
 // against control grids of 2,048, 4,096, 8,192 and 16,384 points (sigma 0.25), with splits
 // of at least four source tiles, ran the fused sum 0.84, 0.84, 0.86 and 0.81 times as long
@@ -327,11 +328,11 @@ static int tile_of(int prec) {
 // one-tile splits the same batches took 1.02-1.16 times as long up to 4,096 sources: each
 // group's set-up (the target splits, the FP64 epilogue) and its split partials pay off only
 // over several source tiles.
-constexpr long kBfBatchMinSources = 2048;
+constexpr long kBfMinSources = 2048;
 
 static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx, long ny) {
   if (prec != DFM_BF16MFMA) return prec;
-  const bool big = nseg > 1 ? ny >= kBfBatchMinSources : nx >= kSortMinTargets;
+  const bool big = ny >= kBfMinSources && (nseg > 1 || nx >= kSortMinTargets);
   return (big && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
 }
 
@@ -393,10 +394,10 @@ static Choice default_choice(int op, int prec, int D, int DV, int nseg, long max
   // in 264 ms against 280 ms aiming for 16384 (4 splits) and 303 ms for 4096, same box,
   // every candidate timed by the autotuner: the shorter splits also spread the FP32C
   // wavefronts of the few wide groups over more of the chip.
-  // Ragged batches: splits of at least four tiles, aiming for 16,384 workgroups (the
-  // batched sums of the 64-subject atlas: see kBfBatchMinSources; its data-term sums take
-  // 9.4-9.7 ms either way).
-  if (prec == DFM_BF16MFMA) return nseg > 1 ? Choice{4, 16384, 4} : Choice{4, 65536, 1};
+  // Splits of at least four tiles (see kBfMinSources; the 1M-source headline takes 17
+  // splits of 230 tiles either way); ragged batches aim for 16,384 workgroups (the
+  // 64-subject atlas's data-term sums take 9.4-9.7 ms either way).
+  if (prec == DFM_BF16MFMA) return nseg > 1 ? Choice{4, 16384, 4} : Choice{4, 65536, 4};
   const long blocks = (max_nx + 63) / 64 * nseg;
   // FP32 Convolve / Grad1 / fused sums (the expanded-distance form) on large single grids:
   // two targets per lane share each LDS source record, halving the LDS reads that bound

@@ -23,14 +23,16 @@
  *              summation).  A wavefront whose targets span more than 4 scaled kernel
  *              widths forms its coordinate differences exactly (FP32 hi + lo parts).
  *   DFM_FP32X  as DFM_FP32C with exact coordinate differences in every wavefront.
- *   DFM_BF16MFMA  Convolve, ConvolveGradient(X, alpha) and the fused sum of single calls
- *              from 8,192 targets on: the distances' cross term -2 x.y and |y|^2 on the
- *              matrix cores (v_mfma_f32_16x16x32_bf16 over three-part BF16 splits of the
- *              FP32 centred coordinates, FP32 accumulation), the target norm applied
- *              exactly per target, exp2 and accumulation as DFM_FP32C.  Groups of 256
- *              sorted targets spanning more than 6 scaled kernel units take the DFM_FP32C
- *              wavefront code in the same launch; every other operation, ragged batches
- *              and smaller calls run as DFM_FP32C.
+ *   DFM_BF16MFMA  Convolve, ConvolveGradient(X, alpha), the fused sum and the varifold
+ *              sums of single calls from 8,192 targets and 2,048 sources on, and of ragged
+ *              batches from 2,048 sources per segment: the distances' cross term -2 x.y
+ *              and |y|^2 on the matrix cores (v_mfma_f32_16x16x32_bf16 over three-part
+ *              BF16 splits of the FP32 centred coordinates, FP32 accumulation), the target
+ *              norm applied exactly per target, exp2 and accumulation as DFM_FP32C.
+ *              Groups of 256 targets spanning more than 8 scaled kernel units (5 in
+ *              ragged batches) run on one centre per wavefront of 64 targets, or the
+ *              DFM_FP32C wavefront code, in the same launch; every other operation and
+ *              smaller calls run as DFM_FP32C.
  *
  * Supported shapes: dimpoint in {2, 3}; dimvect in {1, ..., 6} (the reference supports
  * dimvect in {D, 2D, D(D+1)/2}, CUDAExactKernel.cxx:56-82).  Counts are non-negative and

@@ -6,7 +6,7 @@ scaled coordinates; FP32 accumulation), applies |x|^2 per target exactly, and ke
 and every accumulation on the VALU as DFM_FP32C does.  It replaces the per-pair distance
 arithmetic of ScalarRadialKernel.h:61-73 (tiled in GpuConv1D.cu:55-78).  The matrix-core
 kernel runs single Convolve / ConvolveGradient(X, alpha) / fused calls from 8,192 targets
-on; groups of 256 sorted targets wider than 8 scaled kernel units run the DFM_FP32C code in
+and 2,048 sources on; groups of 256 sorted targets wider than 8 scaled kernel units run the DFM_FP32C code in
 the same kernel, and every other op and smaller calls the DFM_FP32C kernels, so there the
 result equals DFM_FP32C's bit for bit.
 
