@@ -133,6 +133,15 @@ struct KParams {
   double *e_p1, *e_a1;
   const double *e_pb, *e_ab, *e_px, *e_ax, *e_gate;
   double e_c, e_cx;
+  // Linear epilogue of the Convolve / Grad1 / fused / SHOOTVJP sums (o_on != 0): output k
+  // (out0, out1) takes  o_cb[k] base_k + o_cs[k] S + o_ce[k] extra_k  (added in that order)
+  // instead of the finished sum S, where a null base / extra term is left out.  base_k and
+  // extra_k are indexed like out_k and may alias it (each element is read, then written, by
+  // the one thread that finishes its target).  It saves the integrators the elementwise
+  // launch that would otherwise follow the sum.
+  int o_on;
+  const double *o_base[2], *o_extra[2];
+  double o_cb[2], o_cs[2], o_ce[2];
 };
 
 // Work decomposition and workspace layout of one call.

@@ -189,30 +189,65 @@ int ps(const Ctx &cx, int op, KParams p, const Seg &xs, const Seg &ys, long o0, 
   return pairsum(cx.st, op, cx.prec, cx.D, cx.D, cx.sigma, p, o0, o1, cx.pw, cx.pw_bytes);
 }
 
-// conv (nx x D) and/or grad (nx x D) of sum_j K(x_i, y_j) w_j with alpha (nx x D)
+// Linear epilogue of one output of a pair sum (KParams o_*): the output takes
+// cb base + cs S + ce extra instead of the sum S (a null base or extra is left out).
+struct Epi {
+  const double *base;
+  double cb, cs;
+  const double *extra;
+  double ce;
+};
+
+void set_epi(KParams &p, int k, const Epi *e) {
+  if (!e) return;
+  if (!p.o_on) {  // the other output stays S
+    p.o_on = 1;
+    for (int j = 0; j < 2; j++) {
+      p.o_base[j] = p.o_extra[j] = nullptr;
+      p.o_cb[j] = p.o_ce[j] = 0.0;
+      p.o_cs[j] = 1.0;
+    }
+  }
+  p.o_base[k] = e->base;
+  p.o_cb[k] = e->cb;
+  p.o_cs[k] = e->cs;
+  p.o_extra[k] = e->extra;
+  p.o_ce[k] = e->ce;
+}
+
+// conv (nx x D) and/or grad (nx x D) of sum_j K(x_i, y_j) w_j with alpha (nx x D), each
+// through its epilogue ec / eg when given
 int conv_grad1(const Ctx &cx, const double *x, const double *alpha, const Seg &xs,
-               const double *y, const double *w, const Seg &ys, double *conv, double *grad) {
+               const double *y, const double *w, const Seg &ys, double *conv, double *grad,
+               const Epi *ec = nullptr, const Epi *eg = nullptr) {
   KParams p = blank();
   p.x = x; p.ta = alpha; p.y = y; p.w = w;
   const long n = xs.total * cx.D;
   if (conv && grad) {
     p.out0 = conv; p.out1 = grad;
+    set_epi(p, 0, ec);
+    set_epi(p, 1, eg);
     return ps(cx, OP_CONVGRAD1, p, xs, ys, n, n);
   }
   if (conv) {
     p.out0 = conv;
+    set_epi(p, 0, ec);
     return ps(cx, OP_CONV, p, xs, ys, n, 0);
   }
   p.out0 = grad;
+  set_epi(p, 0, eg);
   return ps(cx, OP_GRAD1, p, xs, ys, n, 0);
 }
 
 int shoot_vjp(const Ctx &cx, const double *p_, const double *a, const double *mp,
-              const double *ma, const Seg &cs, double *jp, double *ja) {
+              const double *ma, const Seg &cs, double *jp, double *ja, const Epi *ep = nullptr,
+              const Epi *ea = nullptr) {
   KParams p = blank();
   p.x = p_; p.ta = a; p.tb = mp; p.tc = ma;
   p.y = p_; p.w = a; p.se = mp; p.se2 = ma;
   p.out0 = jp; p.out1 = ja;
+  set_epi(p, 0, ep);
+  set_epi(p, 1, ea);
   return ps(cx, OP_SHOOTVJP, p, cs, cs, cs.total * cx.D, cs.total * cx.D);
 }
 
@@ -318,13 +353,17 @@ struct FlowTmp {
 int flow_step(const Ctx &cx, int improved, const double *p, const double *a, const double *p1,
               const double *a1, const Seg &cs, const double *x, const Seg &ls, double dt,
               double *x1, const FlowTmp &t, const double *gate) {
-  const long md = ls.total * cx.D;
-  int rc = conv_grad1(cx, x, nullptr, ls, p, a, cs, t.v1, nullptr);
-  if (rc) return rc;
-  if (!improved) return glin(cx, ls, x1, x, 1.0, t.v1, dt, nullptr, 0.0, gate, 0);
-  if ((rc = lin(cx, t.xs, x, 1.0, t.v1, dt, nullptr, 0.0, md))) return rc;
+  int rc;
+  if (!improved) {
+    if ((rc = conv_grad1(cx, x, nullptr, ls, p, a, cs, t.v1, nullptr))) return rc;
+    return glin(cx, ls, x1, x, 1.0, t.v1, dt, nullptr, 0.0, gate, 0);
+  }
+  // xs = x + dt v(x; z) from the first sum's epilogue; x1 = x + dt/2 (v1 + v2) as
+  // (x + xs)/2 + dt/2 v2, gated
+  const Epi e{x, 1.0, dt, nullptr, 0.0};
+  if ((rc = conv_grad1(cx, x, nullptr, ls, p, a, cs, t.xs, nullptr, &e))) return rc;
   if ((rc = conv_grad1(cx, t.xs, nullptr, ls, p1, a1, cs, t.v2, nullptr))) return rc;
-  return glin(cx, ls, x1, x, 1.0, t.v1, 0.5 * dt, t.v2, 0.5 * dt, gate, 0);
+  return glin(cx, ls, x1, x, 0.5, t.xs, 0.5, t.v2, 0.5 * dt, gate, 0);
 }
 
 int check_common(int D, int prec, double sigma, long n, long m, int T) {
@@ -997,34 +1036,41 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
         continue;
       }
       // flow step t: cotangent th of x_{t+1} -> thn of x_t, plus its pull onto z_t, z_{t+1}
+      // (the elementwise updates ride on the sums' epilogues)
       if (m > 0) {
-        if ((rc = conv_grad1(cx, X_(zt), nullptr, ls, P_(zt), A_(zt), cs, L.ft.v1, nullptr))) return rc;
         if (improved_euler) {
-          if ((rc = lin(cx, L.ft.xs, X_(zt), 1.0, L.ft.v1, dt, nullptr, 0.0, md))) return rc;
+          // xs = x_t + dt v(x_t; z_t)
+          const Epi exs{X_(zt), 1.0, dt, nullptr, 0.0};
+          if ((rc = conv_grad1(cx, X_(zt), nullptr, ls, P_(zt), A_(zt), cs, L.ft.xs, nullptr, &exs)))
+            return rc;
           // ths = dt/2 J_x v(xs; z_{t+1})^T th
-          if ((rc = conv_grad1(cx, L.ft.xs, L.th, ls, P_(z1), A_(z1), cs, nullptr, L.ths))) return rc;
-          if ((rc = lin(cx, L.ths, L.ths, 0.5 * dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
+          const Epi ehs{nullptr, 0.0, 0.5 * dt, nullptr, 0.0};
+          if ((rc = conv_grad1(cx, L.ft.xs, L.th, ls, P_(z1), A_(z1), cs, nullptr, L.ths, nullptr,
+                               &ehs)))
+            return rc;
           // lambda(z_{t+1}) += dt/2 J_z v(xs; z_{t+1})^T th
-          if ((rc = conv_grad1(cx, P_(z1), A_(z1), cs, L.ft.xs, L.th, ls, L.laf, L.lpf))) return rc;
-          if ((rc = lin(cx, L.lp, L.lp, 1.0, L.lpf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
-          if ((rc = lin(cx, L.la, L.la, 1.0, L.laf, 0.5 * dt, nullptr, 0.0, nd))) return rc;
+          const Epi ela{L.la, 1.0, 0.5 * dt, nullptr, 0.0}, elp{L.lp, 1.0, 0.5 * dt, nullptr, 0.0};
+          if ((rc = conv_grad1(cx, P_(z1), A_(z1), cs, L.ft.xs, L.th, ls, L.la, L.lp, &ela, &elp)))
+            return rc;
           // mu = dt/2 th + dt ths
           if ((rc = lin(cx, L.mu, L.th, 0.5 * dt, L.ths, dt, nullptr, 0.0, md))) return rc;
         } else {
           if ((rc = lin(cx, L.mu, L.th, dt, nullptr, 0.0, nullptr, 0.0, md))) return rc;
         }
-        // thn = th (+ ths) + J_x v(x_t; z_t)^T mu
-        if ((rc = conv_grad1(cx, X_(zt), L.mu, ls, P_(zt), A_(zt), cs, nullptr, L.thn))) return rc;
-        if ((rc = lin(cx, L.thn, L.thn, 1.0, L.th, 1.0, improved_euler ? L.ths : nullptr, 1.0, md)))
+        // thn = th + J_x v(x_t; z_t)^T mu (+ ths)
+        const Epi etn{L.th, 1.0, 1.0, improved_euler ? L.ths : nullptr, 1.0};
+        if ((rc = conv_grad1(cx, X_(zt), L.mu, ls, P_(zt), A_(zt), cs, nullptr, L.thn, nullptr,
+                             &etn)))
           return rc;
         // flow pull onto z_t: J_z v(x_t; z_t)^T mu
         if ((rc = conv_grad1(cx, P_(zt), A_(zt), cs, X_(zt), L.mu, ls, L.laf, L.lpf))) return rc;
       }
-      // shoot step t: cotangent (lp, la) of z_{t+1} -> (lpn, lan) of z_t
+      // shoot step t: cotangent (lp, la) of z_{t+1} -> (lpn, lan) of z_t, plus the flow pull
       if (scheme == DFM_EULER) {
-        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.lp, L.la, cs, L.jp, L.ja))) return rc;
-        if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jp, dt, nullptr, 0.0, nd))) return rc;
-        if ((rc = lin(cx, L.lan, L.la, 1.0, L.ja, dt, nullptr, 0.0, nd))) return rc;
+        // lpn = lp + dt J_F(z_t)^T l (+ lpf), in the epilogue
+        const Epi ep{L.lp, 1.0, dt, m > 0 ? L.lpf : nullptr, 1.0};
+        const Epi ea{L.la, 1.0, dt, m > 0 ? L.laf : nullptr, 1.0};
+        if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.lp, L.la, cs, L.lpn, L.lan, &ep, &ea))) return rc;
       } else {
         // z* = z_t + dt F(z_t)
         if ((rc = conv_grad1(cx, P_(zt), A_(zt), cs, P_(zt), A_(zt), cs, L.st.dpos, L.st.dmom))) return rc;
@@ -1036,18 +1082,17 @@ int adjoint_impl(void *stream, int D, double sigma, int prec, int scheme, int im
         if ((rc = lin(cx, L.nup, L.lp, 0.5 * dt, L.jsp, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
         if ((rc = lin(cx, L.nua, L.la, 0.5 * dt, L.jsa, 0.5 * dt * dt, nullptr, 0.0, nd))) return rc;
         if ((rc = shoot_vjp(cx, P_(zt), A_(zt), L.nup, L.nua, cs, L.jp, L.ja))) return rc;
-        // lpn = l + l* + J_F(z_t)^T nu
+        // lpn = l + l* + J_F(z_t)^T nu (+ lpf)
         if ((rc = lin(cx, L.lpn, L.lp, 1.0, L.jsp, 0.5 * dt, L.jp, 1.0, nd))) return rc;
         if ((rc = lin(cx, L.lan, L.la, 1.0, L.jsa, 0.5 * dt, L.ja, 1.0, nd))) return rc;
+        if (m > 0) {
+          if ((rc = lin(cx, L.lpn, L.lpn, 1.0, L.lpf, 1.0, nullptr, 0.0, nd))) return rc;
+          if ((rc = lin(cx, L.lan, L.lan, 1.0, L.laf, 1.0, nullptr, 0.0, nd))) return rc;
+        }
       }
-      if (m > 0) {
-        if ((rc = lin(cx, L.lp, L.lpn, 1.0, L.lpf, 1.0, nullptr, 0.0, nd))) return rc;
-        if ((rc = lin(cx, L.la, L.lan, 1.0, L.laf, 1.0, nullptr, 0.0, nd))) return rc;
-        std::swap(L.th, L.thn);
-      } else {
-        std::swap(L.lp, L.lpn);
-        std::swap(L.la, L.lan);
-      }
+      std::swap(L.lp, L.lpn);
+      std::swap(L.la, L.lan);
+      if (m > 0) std::swap(L.th, L.thn);
     }
   }
   if ((rc = copy(cx, grad_p0, L.lp, nd)) || (rc = copy(cx, grad_a0, L.la, nd))) return rc;

@@ -298,6 +298,14 @@ __global__ __launch_bounds__(256) void k_fill(double *dst, long n, double v) {
   if (i < n) dst[i] = v;
 }
 
+// Empty sums (S = 0) of a call with the linear epilogue (KParams o_*) on: out0[i < n0] and
+// out1[i < n1] take the epilogue's other terms.
+__global__ __launch_bounds__(256) void k_put_empty(KParams p, long n0, long n1) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p.out0 && i < n0) put<0>(p, p.out0, i, 0.0);
+  if (p.out1 && i < n1) put<1>(p, p.out1, i, 0.0);
+}
+
 // ------------------------------------------------------------------------------------
 // planning
 
@@ -787,6 +795,21 @@ int fill(hipStream_t st, double *dst, long n, double v) {
   return DFM_OK;
 }
 
+// The outputs of a call whose sums are all empty: zero, or the linear epilogue of zero.
+static int store_empty(hipStream_t st, const KParams &p, long out0_n, long out1_n) {
+  if (p.o_on) {
+    const long n = std::max(out0_n, out1_n);
+    if (n > 0)
+      hipLaunchKernelGGL(k_put_empty, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p,
+                         out0_n, out1_n);
+  } else {
+    fill(st, p.out0, out0_n, 0.0);
+    fill(st, p.out1, out1_n, 0.0);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
+}
+
 int put_longs(hipStream_t st, long *dst, const long *src, int n) {
   for (int i = 0; i < n; i += 32) {
     LongChunk c{};
@@ -922,12 +945,7 @@ int pairsum_batch(hipStream_t st, int op, int prec, int D, int DV, double sigma,
   p.yoff = ys.n > 1 ? ys.d : nullptr;
   if (xs.n > 1 && (!xs.d || !ys.d)) return set_error(DFM_ERR_ARG, "device segment offsets missing");
   if (p.nx == 0) return DFM_OK;
-  if (ys.max == 0) {  // every segment has an empty source set
-    fill(st, p.out0, out0_n, 0.0);
-    fill(st, p.out1, out1_n, 0.0);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
-  }
+  if (ys.max == 0) return store_empty(st, p, out0_n, out1_n);  // every source set is empty
   std::shared_ptr<TimedLaunch> timed;
   const Choice c = tune_choose(op, prec, D, DV, 1, xs.n, xs.max, ys.max, &timed);
   Plan pl = batch_plan(op, prec, D, DV, xs.n, xs.total, xs.max, ys.max, c);
@@ -946,12 +964,7 @@ int pairsum(hipStream_t st, int op, int prec, int D, int DV, double sigma, KPara
             long out0_n, long out1_n, void *work, size_t work_bytes, const char *ws_label) {
   if (p.nx == 0) return DFM_OK;
   prec = effective_prec(op, prec, D, DV, 1, p.nx, p.ny);
-  if (p.ny == 0) {  // empty sums
-    fill(st, p.out0, out0_n, 0.0);
-    fill(st, p.out1, out1_n, 0.0);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? DFM_OK : set_error(DFM_ERR_HIP, "fill: %s", hipGetErrorString(e));
-  }
+  if (p.ny == 0) return store_empty(st, p, out0_n, out1_n);  // empty sums
   std::shared_ptr<TimedLaunch> timed;
   const Choice c = tune_choose(op, prec, D, DV, 0, 1, p.nx, p.ny, &timed);
   Plan pl = single_plan(op, prec, D, DV, p.nx, p.ny, c);

@@ -346,23 +346,35 @@ __device__ __forceinline__ void step_update(const KParams &p, long o, int seg, c
   }
 }
 
+// Store element i of output K (0: out0, 1: out1) with value v, through the linear
+// epilogue (KParams o_*) when it is on.
+template <int K>
+__device__ __forceinline__ void put(const KParams &p, double *out, long i, double v) {
+  if (p.o_on) {
+    const double s = p.o_cs[K] * v;
+    v = p.o_base[K] ? p.o_cb[K] * p.o_base[K][i] + s : s;
+    if (p.o_extra[K]) v += p.o_ce[K] * p.o_extra[K][i];
+  }
+  out[i] = v;
+}
+
 // Scale the FP64 accumulators of one target and store them at original index `o`.
 template <int OP, int D, int DV>
 __device__ __forceinline__ void finalize(const double *a, const KParams &p, long o) {
   if constexpr (OP == OP_CONV || OP == OP_VARIFOLD_CONV) {
 #pragma unroll
-    for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
+    for (int k = 0; k < DV; k++) put<0>(p, p.out0, o * DV + k, a[k]);
   } else if constexpr (OP == OP_GRAD1) {
 #pragma unroll
-    for (int l = 0; l < D; l++) p.out0[o * D + l] = p.cg * a[l];
+    for (int l = 0; l < D; l++) put<0>(p, p.out0, o * D + l, p.cg * a[l]);
   } else if constexpr (OP == OP_CONVGRAD1 || OP == OP_VARIFOLD) {
     if (p.out0) {
 #pragma unroll
-      for (int k = 0; k < DV; k++) p.out0[o * DV + k] = a[k];
+      for (int k = 0; k < DV; k++) put<0>(p, p.out0, o * DV + k, a[k]);
     }
     if (p.out1) {
 #pragma unroll
-      for (int l = 0; l < D; l++) p.out1[o * D + l] = p.cg * a[DV + l];
+      for (int l = 0; l < D; l++) put<1>(p, p.out1, o * D + l, p.cg * a[DV + l]);
     }
   } else if constexpr (OP == OP_VARLIN) {
 #pragma unroll
@@ -394,8 +406,8 @@ __device__ __forceinline__ void finalize(const double *a, const KParams &p, long
   } else {  // OP_SHOOTVJP: out0 = J_p, out1 = J_a
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      p.out0[o * D + k] = p.cg * a[D + k] + p.ch1 * a[3 * D + k] + p.ch2 * a[4 * D + k];
-      p.out1[o * D + k] = a[k] - p.cg * a[2 * D + k];
+      put<0>(p, p.out0, o * D + k, p.cg * a[D + k] + p.ch1 * a[3 * D + k] + p.ch2 * a[4 * D + k]);
+      put<1>(p, p.out1, o * D + k, a[k] - p.cg * a[2 * D + k]);
     }
   }
 }

@@ -12,8 +12,10 @@ central differences of the functional along directions):
   * the FP64 adjoint gradient against central differences (eps 1e-4) of the FP64
     log-likelihood along 3 random unit directions per block (momenta, control points,
     template vertices) to 1e-5 relative;
-  * the FP32C gradient against the FP64 one: norm-wise <= 1e-6 for the momenta, the
-    control points and the template (measured 1.1e-7, 1.3e-7 and 5.6e-7).
+  * the FP32C and BF16MFMA gradients against the FP64 one: norm-wise <= 1e-6 for the
+    momenta, the control points and the template (measured FP32C 1.1e-7, 1.3e-7 and
+    7.7e-7; BF16MFMA, whose flow and data-term sums run on the matrix cores, 1.1e-7,
+    1.3e-7 and 8.2e-7).
 """
 import numpy as np
 import pytest
@@ -55,7 +57,7 @@ def _atlas(problem, prec):
 def gradients(problem):
     V, F, target, cp, mom = problem
     out = {}
-    for prec in ("fp64", "fp32c"):
+    for prec in ("fp64", "fp32c", "bf16mfma"):
         print("registration 20,480 triangles, 4,096 control points:", prec, flush=True)
         A = _atlas(problem, prec)
         ll, gcp, gmom, gtpl = A.ComputeCompleteLogLikelihoodGradient(cp, mom, V)
@@ -101,13 +103,14 @@ def test_fp64_adjoint_gradient_matches_central_differences(problem, gradients):
         assert max(e) <= 1e-5, (name, e)
 
 
-def test_fp32c_gradient_matches_fp64(gradients):
+@pytest.mark.parametrize("prec", ["fp32c", "bf16mfma"])
+def test_fp32_gradient_matches_fp64(gradients, prec):
     _, ll64, gcp64, gmom64, gtpl64 = gradients["fp64"]
-    _, ll32, gcp32, gmom32, gtpl32 = gradients["fp32c"]
+    _, ll32, gcp32, gmom32, gtpl32 = gradients[prec]
     errs = {"momenta": nrel(gmom32, gmom64), "control_points": nrel(gcp32, gcp64),
             "template": nrel(gtpl32, gtpl64),
             "loglik": float(abs(np.sum(ll32) - np.sum(ll64)) / abs(np.sum(ll64)))}
-    print(errs)
+    print(prec, errs)
     for name in ("momenta", "control_points", "template"):
         assert errs[name] <= 1e-6, (name, errs)
     assert errs["loglik"] <= 1e-6, errs
