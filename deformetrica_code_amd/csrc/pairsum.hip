@@ -337,10 +337,18 @@ static int tile_of(int prec) {
 // group's set-up (the target splits, the FP64 epilogue) and its split partials pay off only
 // over several source tiles.
 constexpr long kBfMinSources = 2048;
+// Ragged batches also need this many targets in their largest segment.  The 64-subject
+// atlas's sums over its 2,048 control points per subject (a 16 x 16 x 8 grid over
+// [-1.5, 1.5]^3 at sigma 0.25: every group of 256 targets, and every wavefront, spans too far
+// for the matrix-core path) took 169 us per fused shooting step and 733 us per flow pull
+// (onto 10,242 template points) in the FP32C code inside the matrix-core kernel, against 141
+// and 653 us as DFM_FP32C with two targets per lane.
+constexpr long kBfMinBatchTargets = 4096;
 
 static int effective_prec(int op, int prec, int D, int DV, int nseg, long nx, long ny) {
   if (prec != DFM_BF16MFMA) return prec;
-  const bool big = ny >= kBfMinSources && (nseg > 1 || nx >= kSortMinTargets);
+  const bool big =
+      ny >= kBfMinSources && (nseg > 1 ? nx >= kBfMinBatchTargets : nx >= kSortMinTargets);
   return (big && bf_supported(op, D, DV)) ? prec : DFM_FP32C;
 }
 

@@ -98,13 +98,14 @@ def test_bf16_batch_on_compact_segments_matches_fp64(op):
                     assert rel(a[xs], b[xs]) < 1e-6, (op, dv, s, rel(a[xs], b[xs]))
 
 
-def test_bf16_batch_below_the_source_bound_equals_fp32c():
-    """A DFM_BF16MFMA batch whose largest segment has fewer than 2,048 sources runs the
-    DFM_FP32C kernels (the matrix-core kernel's per-group set-up does not pay off over few
-    source tiles): equal bit for bit."""
+@pytest.mark.parametrize("nx,ny", [([9000, 300, 2500], [2047, 1000, 512]),
+                                   ([4095, 300, 2048], [5000, 3000, 2048])])
+def test_bf16_batch_below_the_size_bounds_equals_fp32c(nx, ny):
+    """A DFM_BF16MFMA batch whose largest segment has fewer than 2,048 sources (the
+    matrix-core kernel's per-group set-up does not pay off over few source tiles) or fewer
+    than 4,096 targets runs the DFM_FP32C kernels: equal bit for bit."""
     from deformetrica_code_amd import device
     rng = np.random.default_rng(8)
-    nx, ny = [9000, 300, 2500], [2047, 1000, 512]
     xo, yo = _offsets(nx), _offsets(ny)
     X, A = rng.uniform(size=(xo[-1], 3)), rng.normal(size=(xo[-1], 3))
     Y, W = rng.uniform(size=(yo[-1], 3)), rng.normal(size=(yo[-1], 3))
