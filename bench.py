@@ -28,7 +28,8 @@ Other workloads (one JSON line each, their own metric; the default is the headli
     --workload registration  varifold registration gradient: icosphere template with
                              20,480 triangles (radius 1), bump-displaced target, 4,096
                              control points on a 16^3 grid, 20 steps, checkpoints every 5,
-                             adjoint gradient vs central differences (eps 1e-4)
+                             adjoint gradient vs central differences (eps 1e-4);
+                             with --trace-only, W + K gradients in --precision alone
     --workload atlas         batched atlas gradient: 64 subjects of 2,000-40,000 triangles,
                              2,048 shared control points, eager and HIP-graph replay
     --workload varifold      varifold match + gradient between two sets of 1,000,000
@@ -107,6 +108,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="workload legs: skip the autotuned repeat (for kernel traces of the "
                          "default plans)")
+    ap.add_argument("--trace-only", action="store_true",
+                    help="registration leg: only --warmup untimed and --steps timed gradients "
+                         "in --precision at the default plans, no other precision and no "
+                         "checks (for kernel traces)")
     args = ap.parse_args()
     if args.steps < 1 or args.fp64_steps < 1 or args.warmup < 0:
         ap.error("--steps and --fp64-steps must be at least 1 (the FP64 step is the error "
@@ -465,6 +470,22 @@ def workload_registration(args):
     L = _lib.load()
     kw = dict(kind="varifold", deformation_width=0.3, attachment_width=0.1, data_sigma_squared=1.0,
               number_of_time_points=21, checkpoint_every=5)
+    if args.trace_only:
+        A = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], [(target, F)],
+                                  precision=args.precision, **kw)
+        d = [DeviceArray.from_numpy(x, A.stream) for x in (cp, mom, V)]
+        for _ in range(args.warmup):
+            A.gradient_device(*d)
+        A.stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            A.gradient_device(*d)
+        A.stream.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        return _line("registration gradient time (varifold, 20,480 triangles, 4,096 control "
+                     "points, 20 steps)", ms, "ms", False, ms,
+                     {"workload": "registration leg, --trace-only", "precision": args.precision},
+                     dtype="f32", steps=args.steps, warmup=args.warmup)
     out = {}
     for prec in ("fp32c", "bf16mfma", "fp64"):
         A = HIPDeterministicAtlas(F, V.shape[0], cp.shape[0], [(target, F)], precision=prec, **kw)
