@@ -151,6 +151,9 @@ struct KParams {
 // of few target workgroups fills the chip without split partials and their reduction
 // launch.
 constexpr int kWaveSplit = 16;
+// SHOOTVJP's fifteen accumulators per target: eight wavefronts (256 VGPRs per lane; at
+// sixteen, 128 VGPRs spill 53)
+constexpr int wave_split_of(int op) { return op == OP_SHOOTVJP ? 8 : kWaveSplit; }
 
 struct Plan {
   int R;            // targets per lane

@@ -391,7 +391,7 @@ constexpr long kWaveSplitGrid = 1024;
 // (one target per lane fits the 128 VGPRs of a 1,024-lane workgroup without spills)
 static bool wave_split_op(int op, int DV) {
   return DV <= 3 && (op == OP_CONV || op == OP_GRAD1 || op == OP_CONVGRAD1 ||
-                     op == OP_VARIFOLD || op == OP_VARIFOLD_CONV);
+                     op == OP_VARIFOLD || op == OP_VARIFOLD_CONV || op == OP_SHOOTVJP);
 }
 
 // Heuristic, from the autotuner's measurements on the shapes of the workloads: one target
@@ -506,7 +506,7 @@ static void split_plan(Plan &pl, int op, int prec, int D, int DV, int nseg, long
   pl.R = c.mx ? 1 : c.R;
   pl.mx = c.mx;
   pl.W = (c.w > 1 && !c.mx && pl.R == 1 && prec != DFM_FP64 && wave_split_op(op, DV))
-             ? kWaveSplit
+             ? wave_split_of(op)
              : 1;
   pl.nseg = nseg;
   pl.chunks = (max_nx + 64L * pl.R - 1) / (64L * pl.R);

@@ -869,10 +869,12 @@ __global__ __launch_bounds__(256) void k_reduce_splits(KParams p) {
 template <typename T, int MODE, int OP, int D, int DV, int AUX>
 static void launch_one(hipStream_t st, const KParams &p, const Plan &pl) {
   dim3 grid((unsigned)pl.chunks, (unsigned)pl.nsplit, (unsigned)pl.nseg);
-  if constexpr (std::is_same<T, float>::value && Shape<OP, D, DV>::EXPANDABLE && DV <= 3) {
-    if (pl.W > 1) {  // the plan holds R = 1 and W = kWaveSplit for these ops
-      hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE, kWaveSplit>), grid,
-                         dim3(64 * kWaveSplit), 0, st, p);
+  if constexpr (std::is_same<T, float>::value &&
+                (Shape<OP, D, DV>::EXPANDABLE || OP == OP_SHOOTVJP) && DV <= 3) {
+    if (pl.W > 1) {  // the plan holds R = 1 and W = wave_split_of(OP) for these ops
+      constexpr int WS = wave_split_of(OP);
+      hipLaunchKernelGGL((k_pairsum<T, OP, D, DV, AUX, 1, MODE, WS>), grid, dim3(64 * WS), 0,
+                         st, p);
       if (pl.nsplit > 1 && !p.done)
         hipLaunchKernelGGL((k_reduce_splits<OP, D, DV>), dim3((unsigned)((p.nx + 255) / 256)),
                            dim3(256), 0, st, p);
