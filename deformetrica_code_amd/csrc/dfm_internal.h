@@ -151,13 +151,13 @@ struct KParams {
 // of few target workgroups fills the chip without split partials and their reduction
 // launch.
 constexpr int kWaveSplit = 16;
-// SHOOTVJP's fifteen accumulators per target: eight wavefronts (256 VGPRs per lane; at
-// sixteen, 128 VGPRs spill 53)
+// SHOOTVJP's fifteen accumulators per target: eight wavefronts (101 VGPRs under a budget of
+// 256; at sixteen the budget of 128 spills 53)
 constexpr int wave_split_of(int op) { return op == OP_SHOOTVJP ? 8 : kWaveSplit; }
 
 struct Plan {
   int R;            // targets per lane
-  int W;            // wavefronts per workgroup sharing its targets (FP32, R = 1: kWaveSplit)
+  int W;            // wavefronts per workgroup sharing its targets (FP32, R = 1: wave_split_of)
   int mx;           // FP32C: accumulate on the matrix cores (k_pairsum_mx), R = 1
   int nseg;         // segments of a ragged batch (grid.z)
   bool sort;        // Hilbert-order the targets first (single-segment FP32 calls)
